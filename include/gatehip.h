@@ -1,0 +1,180 @@
+/* gatehip.h -- C-ABI of the MI355X photon transport + voxel dose library.
+ *
+ * This is the boundary a host binding (ctypes here; a pybind11 shim in
+ * opengate_core) calls. Only plain C types, pointers and sizes cross it.
+ *
+ * What each entry point replaces in the reference (OpenGATE 10.1.1):
+ *   gh_set_physics      per-material cross sections the Geant4 gamma processes
+ *                       use (physics.py:206-438) and the mu tables of
+ *                       GateMaterialMuHandler (GateMaterialMuHandler.cpp:159-254)
+ *   gh_hu_to_labels     ImageVolume.create_label_image (geometry/volumes.py:1200-1243)
+ *   gh_add_simulation   ImageVolume construction + GateImageNestedParameterisation
+ *                       (volumes.py:1050-1142, GateImageNestedParameterisation.cpp:19-75),
+ *                       GateGenericSource / GatePhaseSpaceSource initialisation
+ *                       (GateGenericSource.cpp:312-664, GatePhaseSpaceSource.cpp:28-273),
+ *                       GateDoseActor::InitializeCpp / BeginOfRunActionMasterThread
+ *                       (GateDoseActor.cpp:54-117)
+ *   gh_run              the Geant4 event loop driven by GateSourceManager::StartMasterThread
+ *                       (GateSourceManager.cpp:198-244) with GateDoseActor::SteppingAction,
+ *                       ScoreSquaredValue and FlushSquaredValues (GateDoseActor.cpp:180-243,
+ *                       359-397) and BeginOfEventAction event counting (:139-142)
+ *   gh_get_tally        DoseActor.EndOfRunActionMasterThread image fetch
+ *                       (actors/doseactors.py:662-715, cpp_edep_image.to_pyarray)
+ *   gh_get_phsp         PhaseSpaceActor recording of photons leaving a volume
+ *                       (GatePhaseSpaceActor.cpp:119) restricted to one exit face
+ *
+ * Conventions: every function returns 0 on success and a non-zero GH_E* code on
+ * failure; gh_last_error() then holds a message. Host buffers are caller-owned
+ * and only read/written during the call. Device memory is owned by the context.
+ * A context is used from one host thread at a time. Units: mm, MeV, g/cm3.
+ */
+#ifndef GATEHIP_H
+#define GATEHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GH_ABI_VERSION 1
+
+enum {
+  GH_OK = 0,
+  GH_EINVAL = 1,   /* invalid argument */
+  GH_EHIP = 2,     /* HIP runtime error */
+  GH_ESTATE = 3,   /* call out of order (e.g. run before physics) */
+  GH_ERANGE = 4    /* size beyond a supported limit */
+};
+
+/* position types of gh_source_desc.pos_type (GateGenericSource.cpp:312-384) */
+enum { GH_POS_POINT = 0, GH_POS_BOX = 1, GH_POS_SPHERE = 2, GH_POS_ELLIPSOID = 3,
+       GH_POS_DISC = 4, GH_POS_CYLINDER = 5, GH_POS_PHSP = 6 };
+/* direction types (GateGenericSource.cpp:386-480) */
+enum { GH_DIR_ISO = 0, GH_DIR_MOMENTUM = 1, GH_DIR_FOCUSED = 2 };
+/* energy types (GateGenericSource.cpp:492-664, GateSPSEneDistribution.cpp:16-159) */
+enum { GH_ENE_MONO = 0, GH_ENE_LINES = 1, GH_ENE_HISTOGRAM = 2, GH_ENE_RANGE = 3 };
+/* tally flags */
+enum { GH_TALLY_EDEP2 = 1, GH_TALLY_DOSE = 2, GH_TALLY_DOSE2 = 4, GH_TALLY_COUNTS = 8,
+       GH_TALLY_ALIGNED = 16 };
+
+typedef struct gh_ctx gh_ctx;
+
+/* Per-material FP32 tables on the float-bit energy grid (node j has the FP32 energy
+ * with bits (bucket_base + j) << 17, bucket_base = 7488, 64 buckets per octave from
+ * 2^-10 MeV) and on the q^2 grid ray_x (ascending, ray_x[0] = 0). */
+typedef struct gh_physics_desc {
+  int32_t n_materials;
+  int32_t n_buckets;         /* nodes per material = n_buckets + 1 */
+  int32_t n_x_nodes;
+  int32_t reserved0;
+  const float *density;      /* [M] g/cm3 */
+  const float *mu_total;     /* [M][NB+1] 1/mm */
+  const float *mu_majorant;  /* [NB+1]   1/mm, >= every mu_total node */
+  const float *p_compton;    /* [M][NB+1] sigma_C / mu */
+  const float *p_rayleigh;   /* [M][NB+1] (sigma_C + sigma_R) / mu */
+  const float *mu_en;        /* [M][NB+1] 1/mm */
+  const float *ray_x;        /* [NX] q^2 nodes, 1/Angstrom^2 */
+  const float *ray_cdf;      /* [M][NX] normalised int_0^x F^2 */
+  const float *incoh_s;      /* [M][NX] S(q)/Z */
+  float e_cut;               /* photons below e_cut deposit locally and stop */
+  int32_t reserved1;
+} gh_physics_desc;
+
+/* Voxel phantom. Transport runs in the phantom frame: the box spans [0, n*spacing)
+ * per axis, labels are x-fastest (index = x + nx*(y + ny*z)). */
+typedef struct gh_phantom_desc {
+  int32_t nx, ny, nz;
+  int32_t world_material;    /* material index of the world outside the box, -1 = vacuum */
+  float spacing[3];
+  float world_lo[3];         /* world box in the phantom frame (ignored for vacuum) */
+  float world_hi[3];
+  const uint8_t *labels;     /* material index per voxel */
+} gh_phantom_desc;
+
+typedef struct gh_source_desc {
+  int32_t pos_type, dir_type, ene_type, n_energy;
+  float pos_param[4];        /* box: size x,y,z; sphere: r; ellipsoid: a,b,c; disc: r; cylinder: r, half z */
+  float pos_mat[12];         /* row-major 3x3 then translation: source frame -> phantom frame */
+  float dir_mat[9];          /* row-major 3x3: source direction frame -> phantom frame */
+  float cos_theta_min, cos_theta_max, phi_min, phi_max; /* iso: ranges in radians */
+  float momentum[3];         /* GH_DIR_MOMENTUM, source direction frame */
+  float focus[3];            /* GH_DIR_FOCUSED, phantom frame */
+  float energy_mono;         /* mono energy; range: Emin */
+  float energy_max;          /* range: Emax */
+  float weight;              /* history weight (non-phsp) */
+  int32_t phi_full;          /* 1 if phi range is the full [0, 2pi) */
+  const float *energy_value; /* lines: energies; histogram: bin lower edges */
+  const float *energy_width; /* histogram: bin widths */
+  const float *energy_cdf;   /* normalised cumulative weights, last = 1 */
+  int64_t phsp_n;            /* GH_POS_PHSP: number of rows */
+  int64_t phsp_start;        /* first row used by history 0 */
+  int32_t phsp_iso;          /* 1: replace directions by isotropic ones */
+  int32_t reserved0;
+  const float *phsp[8];      /* x, y, z, dx, dy, dz, E, weight (source frame) */
+} gh_source_desc;
+
+typedef struct gh_tally_desc {
+  int32_t nx, ny, nz;
+  int32_t flags;             /* GH_TALLY_* ; ALIGNED: the tally grid is the phantom grid */
+  float corner[3];           /* lower corner of the tally grid in the phantom frame */
+  float spacing[3];
+  int32_t record_face;       /* -1 off, else 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z (vacuum world only) */
+  uint32_t record_capacity;  /* max photons recorded */
+  int32_t detector_label_min;/* -1 off, else per-history energy in labels >= this value */
+  int32_t reserved0;
+} gh_tally_desc;
+
+typedef struct gh_launch {
+  int32_t block;             /* threads per workgroup (multiple of 64) */
+  int32_t grid;              /* workgroups; 0 = automatic */
+  int32_t refill_interval;   /* Woodcock iterations between wave refills */
+  int32_t chunk;             /* histories per work-queue chunk */
+} gh_launch;
+
+typedef struct gh_counters {
+  uint64_t histories, steps, deposits, compton, rayleigh, photo;
+  uint64_t escaped, world_interactions, cut_kills, sq_evictions, recorded, kn_rejections;
+  double e_deposited, e_outside_tally, e_escaped, e_world;
+  double kernel_ms;
+} gh_counters;
+
+int gh_abi_version(void);
+int gh_create(int device, gh_ctx **out);
+void gh_destroy(gh_ctx *ctx);
+const char *gh_last_error(const gh_ctx *ctx);
+
+int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *phys);
+
+/* Adds one simulation (phantom + source + tally); returns its id in *sim_id. */
+int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_desc *src,
+                      const gh_tally_desc *tal, int32_t *sim_id);
+int gh_clear_simulations(gh_ctx *ctx);
+int gh_reset_tally(gh_ctx *ctx, int32_t sim_id);
+
+/* Transports n_histories[s] histories of every simulation s in one persistent launch.
+ * History h of simulation s draws from the Philox4x32-10 stream with key
+ * (seed lo, seed hi) and counter (first_history[s] + h, sim s, block). */
+int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_history,
+           uint64_t seed, const gh_launch *launch, gh_counters *out);
+
+/* Copies a tally to host. Any pointer may be NULL. counts/dose images only if enabled. */
+int gh_get_tally(gh_ctx *ctx, int32_t sim_id, double *edep, double *edep2, double *dose,
+                 double *dose2, double *counts);
+/* Recorded exit-face photons: 8 float arrays of length *n (x y z dx dy dz E w, phantom frame). */
+int gh_get_phsp(gh_ctx *ctx, int32_t sim_id, float *const out[8], uint64_t capacity, uint64_t *n);
+/* Per-history detector energy (tally.detector_label_min >= 0), n_histories floats. */
+int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n);
+
+/* HU image -> uint16 labels: out[i] = labels_sorted[#{k : bins[k] <= hu[i]}]
+ * (numpy.digitize with ascending bins, right=False). */
+int gh_hu_to_labels(gh_ctx *ctx, const int16_t *hu, int64_t n_voxels, const double *bins,
+                    int32_t n_bins, const uint16_t *labels_sorted, uint16_t *out);
+
+/* Device properties used by the autotuner and reports. */
+int gh_device_info(gh_ctx *ctx, int32_t *n_cu, int32_t *lds_bytes, char *name, int32_t name_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

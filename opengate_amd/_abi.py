@@ -1,0 +1,144 @@
+"""ctypes mirror of include/gatehip.h (structures, enums) and the library loader.
+
+The host side reaches the GPU only through these C entry points. If the compiled
+library is missing the import of the runtime fails loudly: there is no CPU fallback.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libgatehip.so"
+
+GH_OK, GH_EINVAL, GH_EHIP, GH_ESTATE, GH_ERANGE = 0, 1, 2, 3, 4
+POS = {"point": 0, "box": 1, "sphere": 2, "ellipsoid": 3, "disc": 4, "cylinder": 5, "phsp": 6}
+DIR = {"iso": 0, "momentum": 1, "focused": 2}
+ENE = {"mono": 0, "spectrum_discrete": 1, "spectrum_histogram": 2, "range": 3}
+TALLY_EDEP2, TALLY_DOSE, TALLY_DOSE2, TALLY_COUNTS, TALLY_ALIGNED = 1, 2, 4, 8, 16
+
+FP = C.POINTER(C.c_float)
+DP = C.POINTER(C.c_double)
+
+
+class PhysicsDesc(C.Structure):
+    _fields_ = [
+        ("n_materials", C.c_int32), ("n_buckets", C.c_int32), ("n_x_nodes", C.c_int32), ("reserved0", C.c_int32),
+        ("density", FP), ("mu_total", FP), ("mu_majorant", FP), ("p_compton", FP), ("p_rayleigh", FP),
+        ("mu_en", FP), ("ray_x", FP), ("ray_cdf", FP), ("incoh_s", FP),
+        ("e_cut", C.c_float), ("reserved1", C.c_int32),
+    ]
+
+
+class PhantomDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("world_material", C.c_int32),
+        ("spacing", C.c_float * 3), ("world_lo", C.c_float * 3), ("world_hi", C.c_float * 3),
+        ("labels", C.POINTER(C.c_uint8)),
+    ]
+
+
+class SourceDesc(C.Structure):
+    _fields_ = [
+        ("pos_type", C.c_int32), ("dir_type", C.c_int32), ("ene_type", C.c_int32), ("n_energy", C.c_int32),
+        ("pos_param", C.c_float * 4), ("pos_mat", C.c_float * 12), ("dir_mat", C.c_float * 9),
+        ("cos_theta_min", C.c_float), ("cos_theta_max", C.c_float), ("phi_min", C.c_float), ("phi_max", C.c_float),
+        ("momentum", C.c_float * 3), ("focus", C.c_float * 3),
+        ("energy_mono", C.c_float), ("energy_max", C.c_float), ("weight", C.c_float), ("phi_full", C.c_int32),
+        ("energy_value", FP), ("energy_width", FP), ("energy_cdf", FP),
+        ("phsp_n", C.c_int64), ("phsp_start", C.c_int64), ("phsp_iso", C.c_int32), ("reserved0", C.c_int32),
+        ("phsp", FP * 8),
+    ]
+
+
+class TallyDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("flags", C.c_int32),
+        ("corner", C.c_float * 3), ("spacing", C.c_float * 3),
+        ("record_face", C.c_int32), ("record_capacity", C.c_uint32),
+        ("detector_label_min", C.c_int32), ("reserved0", C.c_int32),
+    ]
+
+
+class Launch(C.Structure):
+    _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("refill_interval", C.c_int32), ("chunk", C.c_int32)]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("histories", C.c_uint64), ("steps", C.c_uint64), ("deposits", C.c_uint64), ("compton", C.c_uint64),
+        ("rayleigh", C.c_uint64), ("photo", C.c_uint64), ("escaped", C.c_uint64),
+        ("world_interactions", C.c_uint64), ("cut_kills", C.c_uint64), ("sq_evictions", C.c_uint64),
+        ("recorded", C.c_uint64), ("kn_rejections", C.c_uint64),
+        ("e_deposited", C.c_double), ("e_outside_tally", C.c_double), ("e_escaped", C.c_double),
+        ("e_world", C.c_double), ("kernel_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+COUNTER_NAMES = [f for f, _ in Counters._fields_[:12]]
+ENERGY_NAMES = ["e_deposited", "e_outside_tally", "e_escaped", "e_world"]
+
+
+def fptr(a):
+    """float32 C-contiguous numpy array -> float*"""
+    if a is None:
+        return FP()
+    assert a.dtype == np.float32 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(FP)
+
+
+def dptr(a):
+    if a is None:
+        return DP()
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(DP)
+
+
+# exported symbols of libgatehip.so, with (restype, argtypes)
+SIGNATURES = {
+    "gh_abi_version": (C.c_int, []),
+    "gh_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "gh_destroy": (None, [C.c_void_p]),
+    "gh_last_error": (C.c_char_p, [C.c_void_p]),
+    "gh_set_physics": (C.c_int, [C.c_void_p, C.POINTER(PhysicsDesc)]),
+    "gh_add_simulation": (C.c_int, [C.c_void_p, C.POINTER(PhantomDesc), C.POINTER(SourceDesc),
+                                    C.POINTER(TallyDesc), C.POINTER(C.c_int32)]),
+    "gh_clear_simulations": (C.c_int, [C.c_void_p]),
+    "gh_reset_tally": (C.c_int, [C.c_void_p, C.c_int32]),
+    "gh_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint64,
+                         C.POINTER(Launch), C.POINTER(Counters)]),
+    "gh_get_tally": (C.c_int, [C.c_void_p, C.c_int32, DP, DP, DP, DP, DP]),
+    "gh_get_phsp": (C.c_int, [C.c_void_p, C.c_int32, FP * 8, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "gh_get_event_edep": (C.c_int, [C.c_void_p, C.c_int32, FP, C.c_uint64]),
+    "gh_hu_to_labels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int16), C.c_int64, DP, C.c_int32,
+                                  C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]),
+    "gh_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
+}
+
+_LIB = None
+
+
+def load_library(path=None):
+    """Load libgatehip.so (built by opengate_amd.build). Raises if it is absent."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} not found: the MI355X transport library is not built. "
+            "Run `python -m opengate_amd.build` (hipcc, gfx950). There is no CPU fallback."
+        )
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.gh_abi_version() != 1:
+        raise RuntimeError("libgatehip.so ABI version mismatch")
+    if path is None:
+        _LIB = lib
+    return lib

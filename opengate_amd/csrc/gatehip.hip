@@ -1,0 +1,1204 @@
+// MI355X photon transport + voxel dose: persistent Woodcock kernel and the C-ABI.
+//
+// One lane owns one photon history at a time. A wave keeps its 64 lanes busy by
+// refilling lanes whose photon died: every `refill_interval` Woodcock iterations the
+// wave takes a ballot of dead lanes, hands them consecutive history ids from its
+// current work chunk by a prefix count (mbcnt) over the ballot mask, and grabs a new
+// chunk from a global queue with one atomic when the chunk runs dry. The work queue
+// spans every simulation of the launch, so one launch runs many simulations
+// (batched form) and a single simulation is the case n_sims = 1.
+//
+// Per step a photon reads one label byte (gather from HBM/L2) and the interpolated
+// mu of that material from LDS, where the per-material mu tables and the majorant are
+// staged at kernel start. Deposits go through a 4-slot per-lane cache that merges
+// deposits of the same history in the same voxel in FP32 and flushes each (history,
+// voxel) sum into FP64 edep / edep^2 with hardware FP64 atomics: the squared image is
+// the per-history sum squared, as GateDoseActor::ScoreSquaredValue
+// (GateDoseActor.cpp:359-381) computes it.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gh_device.h"
+#include "../../include/gatehip.h"
+
+using namespace gh;
+
+namespace {
+
+struct SimDev {
+  // phantom
+  const uint8_t *labels;
+  int nx, ny, nz, world_mat;
+  float inv_s[3];
+  float L[3];
+  float wlo[3], whi[3];
+  // source
+  int pos_type, dir_type, ene_type, n_energy;
+  float pos_param[4];
+  float pos_mat[12];
+  float dir_mat[9];
+  float ct_min, d_ct, phi_min, d_phi;
+  int phi_full, phsp_iso;
+  float mom[3], focus[3];
+  float e_mono, e_max, weight;
+  const float *e_val, *e_width, *e_cdf;
+  int64_t phsp_n, phsp_start;
+  const float *phsp[8];
+  // tally
+  double *edep, *edep2, *dose, *dose2, *counts;
+  int aligned, tnx, tny, tnz;
+  float tcorner[3], tinv[3];
+  // recorder / detector
+  int rec_face;
+  uint32_t rec_cap;
+  unsigned long long *rec_count;
+  float *rec[8];
+  int det_label_min;
+  float *event_edep;
+  // run
+  uint64_t n_hist, first_hist;
+};
+
+struct Launch {
+  const SimDev *sims;
+  int n_sims;
+  const uint64_t *chunk_offsets;  // [n_sims+1] prefix sum of chunks per sim
+  uint64_t total_chunks;
+  uint32_t chunk;
+  unsigned long long *next_chunk;
+  uint64_t seed;
+  int refill;
+  Phys P;
+  unsigned long long *counters;   // [12]
+  double *energies;               // [4]
+};
+
+// --------------------------------------------------------------------- wave helpers
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// --------------------------------------------------------------------- hot fields
+struct Hot {
+  const uint8_t *labels;
+  int nx, ny, nz, world_mat;
+  float isx, isy, isz;
+  float Lx, Ly, Lz;
+  int aligned;
+  __device__ __forceinline__ void load(const SimDev &s) {
+    labels = s.labels; nx = s.nx; ny = s.ny; nz = s.nz; world_mat = s.world_mat;
+    isx = s.inv_s[0]; isy = s.inv_s[1]; isz = s.inv_s[2];
+    Lx = s.L[0]; Ly = s.L[1]; Lz = s.L[2];
+    aligned = s.aligned;
+  }
+};
+
+// slab test of one axis of the box [0, L]
+__device__ __forceinline__ void slab(float p, float d, float L, int axis, float &tnear, float &tfar, int &face) {
+  if (d != 0.0f) {
+    const float inv = 1.0f / d;
+    const float t0 = (-p) * inv;
+    const float t1 = (L - p) * inv;
+    const float tmn = fminf(t0, t1), tmx = fmaxf(t0, t1);
+    if (tmn > tnear) tnear = tmn;
+    if (tmx < tfar) { tfar = tmx; face = 2 * axis + (d > 0.0f ? 1 : 0); }
+  } else if (p < 0.0f || p > L) {
+    tnear = INFINITY;
+  }
+}
+
+// ray vs box [0,L]: entry/exit distance and exit face (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)
+__device__ __forceinline__ void ray_box(float px, float py, float pz, float dx, float dy, float dz,
+                                        float Lx, float Ly, float Lz, float &tnear, float &tfar,
+                                        int &face) {
+  tnear = -INFINITY; tfar = INFINITY; face = -1;
+  slab(px, dx, Lx, 0, tnear, tfar, face);
+  slab(py, dy, Ly, 1, tnear, tfar, face);
+  slab(pz, dz, Lz, 2, tnear, tfar, face);
+}
+
+__device__ __forceinline__ float world_axis(float p, float d, float lo, float hi, float t) {
+  if (d > 0.0f) return fminf(t, (hi - p) / d);
+  if (d < 0.0f) return fminf(t, (lo - p) / d);
+  return t;
+}
+
+// distance to the world box boundary from inside
+__device__ __forceinline__ float world_exit(const SimDev &s, float px, float py, float pz, float dx,
+                                            float dy, float dz) {
+  float t = INFINITY;
+  t = world_axis(px, dx, s.wlo[0], s.whi[0], t);
+  t = world_axis(py, dy, s.wlo[1], s.whi[1], t);
+  t = world_axis(pz, dz, s.wlo[2], s.whi[2], t);
+  return fmaxf(t, 0.0f);
+}
+
+// --------------------------------------------------------------------- photon
+enum { MODE_BOX = 0, MODE_WORLD = 1 };
+
+struct Photon {
+  float x, y, z, u, v, w, E, wt;
+  int mode;
+  uint32_t hist;   // local history index within its simulation
+  int sim;
+};
+
+// squared-value cache: up to 4 (voxel, sum) entries of the current history, as scalar
+// registers (slot k: v_k voxel, s_k FP32 energy sum, d_k dose sum, c_k deposit count)
+struct SqCache {
+  int v0, v1, v2, v3;
+  float s0, s1, s2, s3;
+  float d0, d1, d2, d3;
+  uint32_t c0, c1, c2, c3;
+  int n;
+};
+
+// Counters: steps, rejections and deposited energy stay in lane registers; the rare
+// event counters are summed per workgroup in LDS with one atomic per wave event
+// (ballot + popcount), and flushed to global memory once at kernel end.
+enum { C_HIST = 0, C_STEPS, C_DEPOSITS, C_COMPTON, C_RAYLEIGH, C_PHOTO, C_ESCAPED, C_WORLD, C_CUT,
+       C_EVICT, C_RECORDED, C_REJ, N_COUNTERS };
+enum { E_DEP = 0, E_OUT, E_ESC, E_WORLD, N_ENERGIES };
+
+struct LaneStats {
+  uint32_t steps, rej;
+  double e_dep;
+  unsigned long long *cnt;  // LDS [N_COUNTERS]
+  double *en;               // LDS [N_ENERGIES]
+  __device__ __forceinline__ void inc(int k) {
+    const uint64_t m = __ballot(1);
+    if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+      atomicAdd(cnt + k, (unsigned long long)__popcll(m));
+  }
+  __device__ __forceinline__ void add_e(int k, double v) { unsafeAtomicAdd(en + k, v); }
+};
+
+__device__ __forceinline__ void flush_slot(const SimDev &s, int vox, float sum, float dsum, uint32_t cnt,
+                                           LaneStats &st) {
+  unsafeAtomicAdd(s.edep + vox, (double)sum);
+  if (s.edep2) unsafeAtomicAdd(s.edep2 + vox, (double)sum * (double)sum);
+  if (s.counts) unsafeAtomicAdd(s.counts + vox, (double)cnt);
+  if (s.dose) unsafeAtomicAdd(s.dose + vox, (double)dsum);
+  if (s.dose2) unsafeAtomicAdd(s.dose2 + vox, (double)dsum * (double)dsum);
+  st.inc(C_DEPOSITS);
+  st.e_dep += (double)sum;
+}
+
+__device__ __forceinline__ void sq_flush_all(const SimDev &s, SqCache &c, LaneStats &st) {
+  if (c.n > 0) flush_slot(s, c.v0, c.s0, c.d0, c.c0, st);
+  if (c.n > 1) flush_slot(s, c.v1, c.s1, c.d1, c.c1, st);
+  if (c.n > 2) flush_slot(s, c.v2, c.s2, c.d2, c.c2, st);
+  if (c.n > 3) flush_slot(s, c.v3, c.s3, c.d3, c.c3, st);
+  c.n = 0;
+}
+
+// Adds deposit x (dose dx) of the current history to tally voxel tv. A history's
+// deposits in the same voxel are summed in FP32 before the squared value is formed;
+// with more than 4 distinct voxels the oldest slot is flushed (counted as an eviction).
+__device__ __forceinline__ void deposit(const SimDev &s, SqCache &c, int tv, float x, float dx,
+                                        LaneStats &st) {
+  if (tv < 0) { st.add_e(E_OUT, (double)x); return; }
+#define GH_SLOT_HIT(K)                                                     \
+  if (c.n > K && c.v##K == tv) {                                           \
+    c.s##K = c.s##K + x; c.d##K = c.d##K + dx; c.c##K += 1u; return;        \
+  }
+  GH_SLOT_HIT(0) GH_SLOT_HIT(1) GH_SLOT_HIT(2) GH_SLOT_HIT(3)
+#undef GH_SLOT_HIT
+  if (c.n == 4) {
+    flush_slot(s, c.v0, c.s0, c.d0, c.c0, st);
+    c.v0 = c.v1; c.s0 = c.s1; c.d0 = c.d1; c.c0 = c.c1;
+    c.v1 = c.v2; c.s1 = c.s2; c.d1 = c.d2; c.c1 = c.c2;
+    c.v2 = c.v3; c.s2 = c.s3; c.d2 = c.d3; c.c2 = c.c3;
+    c.n = 3;
+    st.inc(C_EVICT);
+  }
+  if (c.n == 0) { c.v0 = tv; c.s0 = x; c.d0 = dx; c.c0 = 1u; }
+  else if (c.n == 1) { c.v1 = tv; c.s1 = x; c.d1 = dx; c.c1 = 1u; }
+  else if (c.n == 2) { c.v2 = tv; c.s2 = x; c.d2 = dx; c.c2 = 1u; }
+  else { c.v3 = tv; c.s3 = x; c.d3 = dx; c.c3 = 1u; }
+  c.n += 1;
+}
+
+// tally voxel of a deposit at (x,y,z) in the phantom frame; vox = phantom voxel
+__device__ __forceinline__ int tally_voxel(const SimDev &s, const Hot &h, int vox, float x, float y,
+                                           float z) {
+  if (h.aligned) return vox;
+  const int ix = (int)floorf((x - s.tcorner[0]) * s.tinv[0]);
+  const int iy = (int)floorf((y - s.tcorner[1]) * s.tinv[1]);
+  const int iz = (int)floorf((z - s.tcorner[2]) * s.tinv[2]);
+  if ((unsigned)ix >= (unsigned)s.tnx || (unsigned)iy >= (unsigned)s.tny || (unsigned)iz >= (unsigned)s.tnz)
+    return -1;
+  return ix + s.tnx * (iy + s.tny * iz);
+}
+
+// ---------------------------------------------------------------------- source
+__device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
+  float x = 0.0f, y = 0.0f, z = 0.0f, u = 0.0f, v = 0.0f, w = 1.0f, E = s.e_mono, wt = s.weight;
+  if (s.pos_type == GH_POS_PHSP) {
+    const int64_t row = (s.phsp_start + (int64_t)h) % s.phsp_n;
+    x = s.phsp[0][row]; y = s.phsp[1][row]; z = s.phsp[2][row];
+    u = s.phsp[3][row]; v = s.phsp[4][row]; w = s.phsp[5][row];
+    E = s.phsp[6][row]; wt = s.phsp[7][row];
+  } else {
+    // ---- position (GateGenericSource.cpp:312-384; G4SPSPosDistribution shapes)
+    switch (s.pos_type) {
+      case GH_POS_BOX:
+        x = (u01(g) - 0.5f) * s.pos_param[0];
+        y = (u01(g) - 0.5f) * s.pos_param[1];
+        z = (u01(g) - 0.5f) * s.pos_param[2];
+        break;
+      case GH_POS_SPHERE:
+      case GH_POS_ELLIPSOID: {
+        float a, b, c, r2;
+        int it = 0;
+        do {
+          a = fmaf(2.0f, u01(g), -1.0f);
+          b = fmaf(2.0f, u01(g), -1.0f);
+          c = fmaf(2.0f, u01(g), -1.0f);
+          r2 = fmaf(c, c, fmaf(b, b, a * a));
+        } while (r2 > 1.0f && ++it < MAX_REJECT);
+        if (s.pos_type == GH_POS_SPHERE) {
+          x = s.pos_param[0] * a; y = s.pos_param[0] * b; z = s.pos_param[0] * c;
+        } else {
+          x = s.pos_param[0] * a; y = s.pos_param[1] * b; z = s.pos_param[2] * c;
+        }
+        break;
+      }
+      case GH_POS_DISC:
+      case GH_POS_CYLINDER: {
+        float a, b, r2;
+        int it = 0;
+        do {
+          a = fmaf(2.0f, u01(g), -1.0f);
+          b = fmaf(2.0f, u01(g), -1.0f);
+          r2 = fmaf(b, b, a * a);
+        } while (r2 > 1.0f && ++it < MAX_REJECT);
+        x = s.pos_param[0] * a; y = s.pos_param[0] * b;
+        if (s.pos_type == GH_POS_CYLINDER) z = fmaf(2.0f, u01(g), -1.0f) * s.pos_param[1];
+        break;
+      }
+      default: break;
+    }
+    // ---- direction (G4SPSAngDistribution iso convention: momentum = -(st cp, st sp, ct))
+    if (s.dir_type == GH_DIR_ISO) {
+      const float ct = fmaf(-u01(g), s.d_ct, s.ct_min);
+      const float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+      float cp, sp;
+      if (s.phi_full) {
+        azimuth(g, cp, sp);
+      } else {
+        const float phi = fmaf(u01(g), s.d_phi, s.phi_min);
+        gsincos(phi, sp, cp);
+      }
+      u = -(st * cp); v = -(st * sp); w = -ct;
+    } else if (s.dir_type == GH_DIR_MOMENTUM) {
+      u = s.mom[0]; v = s.mom[1]; w = s.mom[2];
+    }
+    // ---- energy (GateSPSEneDistribution.cpp:16-159)
+    if (s.ene_type == GH_ENE_LINES) {
+      const int k = cdf_index(s.e_cdf, s.n_energy, u01(g));
+      E = s.e_val[k];
+    } else if (s.ene_type == GH_ENE_HISTOGRAM) {
+      const int k = cdf_index(s.e_cdf, s.n_energy, u01(g));
+      E = fmaf(u01(g), s.e_width[k], s.e_val[k]);
+    } else if (s.ene_type == GH_ENE_RANGE) {
+      E = fmaf(u01(g), s.e_max - s.e_mono, s.e_mono);
+    }
+  }
+  // source frame -> phantom frame
+  const float *M = s.pos_mat;
+  p.x = fmaf(M[0], x, fmaf(M[1], y, fmaf(M[2], z, M[9])));
+  p.y = fmaf(M[3], x, fmaf(M[4], y, fmaf(M[5], z, M[10])));
+  p.z = fmaf(M[6], x, fmaf(M[7], y, fmaf(M[8], z, M[11])));
+  if (s.pos_type == GH_POS_PHSP && s.phsp_iso) {
+    const float ct = fmaf(2.0f, u01(g), -1.0f);
+    const float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+    float cp, sp;
+    azimuth(g, cp, sp);
+    p.u = st * cp; p.v = st * sp; p.w = ct;
+  } else if (s.dir_type == GH_DIR_FOCUSED && s.pos_type != GH_POS_PHSP) {
+    p.u = s.focus[0] - p.x; p.v = s.focus[1] - p.y; p.w = s.focus[2] - p.z;
+    normalize3(p.u, p.v, p.w);
+  } else {
+    const float *D = s.dir_mat;
+    p.u = fmaf(D[0], u, fmaf(D[1], v, D[2] * w));
+    p.v = fmaf(D[3], u, fmaf(D[4], v, D[5] * w));
+    p.w = fmaf(D[6], u, fmaf(D[7], v, D[8] * w));
+    normalize3(p.u, p.v, p.w);
+  }
+  p.E = E;
+  p.wt = wt;
+}
+
+__device__ __forceinline__ bool inside_box(const Hot &h, float x, float y, float z) {
+  const int ix = (int)floorf(x * h.isx), iy = (int)floorf(y * h.isy), iz = (int)floorf(z * h.isz);
+  return (unsigned)ix < (unsigned)h.nx && (unsigned)iy < (unsigned)h.ny && (unsigned)iz < (unsigned)h.nz;
+}
+
+// ---------------------------------------------------------------------- kernel
+template <bool BATCHED, bool LDS_TABLES>
+__global__ void __launch_bounds__(256) transport_kernel(Launch L) {
+  extern __shared__ float lds[];
+  const Phys &P = L.P;
+  const int NB1 = P.nb + 1;
+  const float *mu_tab = P.mu;
+  const float *maj_tab = P.maj;
+  if (LDS_TABLES) {
+    const int n_mu = P.n_mat * NB1;
+    for (int i = threadIdx.x; i < n_mu; i += blockDim.x) lds[i] = P.mu[i];
+    for (int i = threadIdx.x; i < NB1; i += blockDim.x) lds[n_mu + i] = P.maj[i];
+    __syncthreads();
+    mu_tab = lds;
+    maj_tab = lds + n_mu;
+  }
+
+  Hot hot;
+  if (!BATCHED) hot.load(L.sims[0]);
+
+  Photon ph;
+  ph.mode = MODE_BOX; ph.sim = 0; ph.hist = 0;
+  ph.x = ph.y = ph.z = ph.u = ph.v = ph.w = ph.E = ph.wt = 0.0f;
+  ECache ec;
+  ec.b = 0; ec.frac = 0.0f; ec.maj = 1.0f; ec.inv_maj = 1.0f;
+  Rng rng;
+  rng_init(rng, 0, 0, 0);
+  SqCache sq = {};
+  float ev_edep = 0.0f;
+  __shared__ unsigned long long s_cnt[N_COUNTERS];
+  __shared__ double s_en[N_ENERGIES];
+  if (threadIdx.x < N_COUNTERS) s_cnt[threadIdx.x] = 0ull;
+  if (threadIdx.x < N_ENERGIES) s_en[threadIdx.x] = 0.0;
+  __syncthreads();
+  LaneStats st;
+  st.steps = 0u; st.rej = 0u; st.e_dep = 0.0; st.cnt = s_cnt; st.en = s_en;
+  bool alive = false;
+
+  // wave-uniform work state
+  int cur_sim = 0;
+  uint64_t cur_next = 0, cur_end = 0;
+  bool have_work = true;
+
+  for (;;) {
+    // ------------------------------------------------------------- refill
+    uint64_t dead = __ballot(!alive);
+    while (dead != 0ull && have_work) {
+      if (cur_next >= cur_end) {
+        unsigned long long c = 0;
+        if (lane_id() == 0) c = atomicAdd(L.next_chunk, 1ull);
+        c = __shfl(c, 0, 64);
+        if (c >= L.total_chunks) { have_work = false; break; }
+        int s = 0;
+        while (s + 1 < L.n_sims && L.chunk_offsets[s + 1] <= c) ++s;
+        cur_sim = s;
+        const uint64_t local = (c - L.chunk_offsets[s]) * (uint64_t)L.chunk;
+        cur_next = local;
+        const uint64_t e = local + L.chunk;
+        cur_end = e < L.sims[s].n_hist ? e : L.sims[s].n_hist;
+        continue;
+      }
+      const uint64_t avail = cur_end - cur_next;
+      const uint32_t nd = (uint32_t)__popcll(dead);
+      const uint32_t take = (uint32_t)(avail < nd ? avail : nd);
+      if (!alive) {
+        const uint32_t rank = mbcnt(dead);
+        if (rank < take) {
+          const SimDev &s = L.sims[BATCHED ? cur_sim : 0];
+          if (BATCHED) hot.load(s);
+          ph.sim = BATCHED ? cur_sim : 0;
+          ph.hist = (uint32_t)(cur_next + rank);
+          rng_init(rng, L.seed, s.first_hist + ph.hist, (uint32_t)ph.sim);
+          sample_source(s, rng, ph.hist, ph);
+          st.inc(C_HIST);
+          ev_edep = 0.0f;
+          sq.n = 0;
+          ph.mode = inside_box(hot, ph.x, ph.y, ph.z) ? MODE_BOX : MODE_WORLD;
+          if (ph.E < P.e_cut) {
+            // below the tracking cut from the start: deposit locally (inside) and stop
+            if (ph.mode == MODE_BOX) {
+              const int ix = (int)floorf(ph.x * hot.isx), iy = (int)floorf(ph.y * hot.isy),
+                        iz = (int)floorf(ph.z * hot.isz);
+              const int vox = ix + hot.nx * (iy + hot.ny * iz);
+              const int lab = hot.labels[vox];
+              const float x = ph.E * ph.wt;
+              deposit(s, sq, tally_voxel(s, hot, vox, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
+              if (lab >= s.det_label_min && s.det_label_min >= 0) ev_edep += x;
+              sq_flush_all(s, sq, st);
+            } else {
+              st.add_e(E_WORLD, (double)(ph.E * ph.wt));
+            }
+            st.inc(C_CUT);
+            if (s.event_edep) s.event_edep[ph.hist] = ev_edep;
+          } else {
+            ecache_set(ec, ph.E, maj_tab);
+            alive = true;
+          }
+        }
+      }
+      cur_next += take;
+      dead = __ballot(!alive);
+    }
+    if (!have_work && __ballot(alive) == 0ull) break;
+
+    // ------------------------------------------------------------- transport
+    for (int it = 0; it < L.refill; ++it) {
+      if (!alive) break;
+      const SimDev &s = L.sims[BATCHED ? ph.sim : 0];
+      bool kill = false;
+      if (ph.mode == MODE_BOX) {
+        // ---- Woodcock tentative step
+        const float d = -glog(u01(rng)) * ec.inv_maj;
+        const float nx = fmaf(d, ph.u, ph.x), ny = fmaf(d, ph.v, ph.y), nz = fmaf(d, ph.w, ph.z);
+        const int ix = (int)floorf(nx * hot.isx), iy = (int)floorf(ny * hot.isy), iz = (int)floorf(nz * hot.isz);
+        if ((unsigned)ix < (unsigned)hot.nx && (unsigned)iy < (unsigned)hot.ny && (unsigned)iz < (unsigned)hot.nz) {
+          ph.x = nx; ph.y = ny; ph.z = nz;
+          st.steps++;
+          const int vox = ix + hot.nx * (iy + hot.ny * iz);
+          const int lab = hot.labels[vox];
+          const float mu = lerp_node(mu_tab + lab * NB1, ec);
+          if (u01(rng) * ec.maj < mu) {
+            // ---- real interaction
+            const float r = u01(rng);
+            const float pc = lerp_node(P.pc + lab * NB1, ec);
+            const float pr = lerp_node(P.pr + lab * NB1, ec);
+            const int tv = tally_voxel(s, hot, vox, ph.x, ph.y, ph.z);
+            const float irho = P.inv_rho[lab];
+            const bool det = s.det_label_min >= 0 && lab >= s.det_label_min;
+            if (r < pc) {
+              st.inc(C_COMPTON);
+              float q, c;
+              sample_compton(rng, P, lab, ph.E, q, c, st.rej);
+              const float e1 = ph.E / q;
+              const float x = (ph.E - e1) * ph.wt;
+              deposit(s, sq, tv, x, x * irho, st);
+              if (det) ev_edep += x;
+              float cp, sp;
+              azimuth(rng, cp, sp);
+              rotate(ph.u, ph.v, ph.w, c, cp, sp);
+              ph.E = e1;
+              if (ph.E < P.e_cut) {
+                const float x2 = ph.E * ph.wt;
+                deposit(s, sq, tv, x2, x2 * irho, st);
+                if (det) ev_edep += x2;
+                st.inc(C_CUT);
+                kill = true;
+              } else {
+                ecache_set(ec, ph.E, maj_tab);
+              }
+            } else if (r < pr) {
+              st.inc(C_RAYLEIGH);
+              const float c = sample_rayleigh(rng, P, lab, ph.E);
+              float cp, sp;
+              azimuth(rng, cp, sp);
+              rotate(ph.u, ph.v, ph.w, c, cp, sp);
+            } else {
+              st.inc(C_PHOTO);
+              const float x = ph.E * ph.wt;
+              deposit(s, sq, tv, x, x * irho, st);
+              if (det) ev_edep += x;
+              kill = true;
+            }
+          }
+        } else {
+          // ---- leaves the phantom box: exit point from the last position inside
+          float tn, tf;
+          int face;
+          ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
+          const float t = fmaxf(tf, 0.0f);
+          const float ex = fmaf(t, ph.u, ph.x), ey = fmaf(t, ph.v, ph.y), ez = fmaf(t, ph.w, ph.z);
+          if (s.rec_face >= 0 && face == s.rec_face) {
+            const uint64_t m = __ballot(1);
+            const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+            unsigned long long base = 0;
+            if (lane_id() == leader) base = atomicAdd(s.rec_count, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader, 64);
+            const uint64_t slot = base + mbcnt(m);
+            if (slot < s.rec_cap) {
+              s.rec[0][slot] = ex; s.rec[1][slot] = ey; s.rec[2][slot] = ez;
+              s.rec[3][slot] = ph.u; s.rec[4][slot] = ph.v; s.rec[5][slot] = ph.w;
+              s.rec[6][slot] = ph.E; s.rec[7][slot] = ph.wt;
+            }
+            st.inc(C_RECORDED);
+            st.add_e(E_ESC, (double)(ph.E * ph.wt));
+            kill = true;
+          } else if (hot.world_mat < 0) {
+            st.inc(C_ESCAPED);
+            st.add_e(E_ESC, (double)(ph.E * ph.wt));
+            kill = true;
+          } else {
+            ph.x = ex; ph.y = ey; ph.z = ez;
+            ph.mode = MODE_WORLD;
+          }
+        }
+      } else {
+        // ---- world outside the phantom box (homogeneous material or vacuum)
+        float tn, tf;
+        int face;
+        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
+        const bool hit = (tn < tf) && (tf > EXIT_EPS);
+        const float tin = fmaxf(tn, 0.0f);
+        if (hot.world_mat < 0) {
+          if (hit) {
+            ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
+            ph.mode = MODE_BOX;
+          } else {
+            st.inc(C_ESCAPED);
+            st.add_e(E_ESC, (double)(ph.E * ph.wt));
+            kill = true;
+          }
+        } else {
+          const int wm = hot.world_mat;
+          const float muw = lerp_node(mu_tab + wm * NB1, ec);
+          const float tw = world_exit(s, ph.x, ph.y, ph.z, ph.u, ph.v, ph.w);
+          const float d = (muw > 0.0f) ? (-glog(u01(rng)) / muw) : INFINITY;
+          if (hit && d >= tin && tin < tw) {
+            ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
+            ph.mode = MODE_BOX;
+          } else if (d >= tw) {
+            st.inc(C_ESCAPED);
+            st.add_e(E_ESC, (double)(ph.E * ph.wt));
+            kill = true;
+          } else {
+            ph.x = fmaf(d, ph.u, ph.x); ph.y = fmaf(d, ph.v, ph.y); ph.z = fmaf(d, ph.w, ph.z);
+            st.inc(C_WORLD);
+            const float r = u01(rng);
+            const float pc = lerp_node(P.pc + wm * NB1, ec);
+            const float pr = lerp_node(P.pr + wm * NB1, ec);
+            if (r < pc) {
+              float q, c;
+              sample_compton(rng, P, wm, ph.E, q, c, st.rej);
+              const float e1 = ph.E / q;
+              st.add_e(E_WORLD, (double)((ph.E - e1) * ph.wt));
+              float cp, sp;
+              azimuth(rng, cp, sp);
+              rotate(ph.u, ph.v, ph.w, c, cp, sp);
+              ph.E = e1;
+              if (ph.E < P.e_cut) {
+                st.add_e(E_WORLD, (double)(ph.E * ph.wt));
+                st.inc(C_CUT);
+                kill = true;
+              } else {
+                ecache_set(ec, ph.E, maj_tab);
+              }
+            } else if (r < pr) {
+              const float c = sample_rayleigh(rng, P, wm, ph.E);
+              float cp, sp;
+              azimuth(rng, cp, sp);
+              rotate(ph.u, ph.v, ph.w, c, cp, sp);
+            } else {
+              st.add_e(E_WORLD, (double)(ph.E * ph.wt));
+              kill = true;
+            }
+          }
+        }
+      }
+      if (kill) {
+        sq_flush_all(s, sq, st);
+        if (s.event_edep) s.event_edep[ph.hist] = ev_edep;
+        alive = false;
+      }
+    }
+  }
+
+  // ------------------------------------------------------------- counters
+  {
+    const uint64_t steps = wave_sum<uint64_t>((uint64_t)st.steps);
+    const uint64_t rej = wave_sum<uint64_t>((uint64_t)st.rej);
+    const double edep = wave_sum<double>(st.e_dep);
+    if (lane_id() == 0) {
+      atomicAdd(s_cnt + C_STEPS, (unsigned long long)steps);
+      atomicAdd(s_cnt + C_REJ, (unsigned long long)rej);
+      unsafeAtomicAdd(s_en + E_DEP, edep);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < N_COUNTERS && s_cnt[threadIdx.x])
+    atomicAdd(L.counters + threadIdx.x, s_cnt[threadIdx.x]);
+  if (threadIdx.x < N_ENERGIES && s_en[threadIdx.x] != 0.0)
+    unsafeAtomicAdd(L.energies + threadIdx.x, s_en[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------- HU -> labels
+__global__ void hu_to_labels_kernel(const int16_t *__restrict__ hu, int64_t n, const double *__restrict__ bins,
+                                    int n_bins, const uint16_t *__restrict__ lut, uint16_t *__restrict__ out) {
+  __shared__ double sb[256];
+  __shared__ uint16_t sl[257];
+  for (int i = threadIdx.x; i < n_bins; i += blockDim.x) sb[i] = bins[i];
+  for (int i = threadIdx.x; i <= n_bins; i += blockDim.x) sl[i] = lut[i];
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double x = (double)hu[i];
+    // number of bins <= x (digitize, right=False)
+    int lo = 0, hi = n_bins;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (sb[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    out[i] = sl[lo];
+  }
+}
+
+// ---------------------------------------------------------------------- host side
+#define HIPCHK(expr)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);           \
+      return GH_EHIP;                                                         \
+    }                                                                         \
+  } while (0)
+
+#define FAIL(code, msg)   \
+  do {                    \
+    ctx->err = (msg);     \
+    return (code);        \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+};
+
+struct SimHost {
+  SimDev dev;
+  std::vector<void *> owned;  // device allocations of this simulation
+  size_t n_tally = 0;
+  int flags = 0;
+  int det = -1;
+  uint64_t event_cap = 0;
+};
+
+}  // namespace
+
+struct gh_ctx {
+  int device = 0;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int n_cu = 0, lds_max = 0;
+  std::string name;
+  // physics
+  bool have_phys = false;
+  Phys P{};
+  std::vector<void *> phys_owned;
+  size_t lds_bytes = 0;
+  // simulations
+  std::vector<SimHost> sims;
+  // run scratch
+  SimDev *d_sims = nullptr;
+  uint64_t *d_chunk_off = nullptr;
+  unsigned long long *d_next = nullptr;
+  unsigned long long *d_counters = nullptr;
+  double *d_energies = nullptr;
+  size_t d_sims_cap = 0;
+};
+
+namespace {
+
+template <typename T>
+int dev_upload(gh_ctx *ctx, const T *host, size_t n, T **out, std::vector<void *> &owned) {
+  *out = nullptr;
+  if (n == 0) return GH_OK;
+  void *p = nullptr;
+  HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  owned.push_back(p);
+  if (host) HIPCHK(hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice));
+  *out = (T *)p;
+  return GH_OK;
+}
+
+void free_all(std::vector<void *> &v) {
+  for (void *p : v) (void)hipFree(p);
+  v.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gh_abi_version(void) { return GH_ABI_VERSION; }
+
+int gh_create(int device, gh_ctx **out) {
+  if (!out) return GH_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GH_EHIP;
+  if (device < 0 || device >= n) return GH_EINVAL;
+  gh_ctx *ctx = new gh_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess ||
+      hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+    delete ctx;
+    return GH_EHIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->n_cu = prop.multiProcessorCount;
+    ctx->lds_max = (int)prop.sharedMemPerBlock;
+    ctx->name = prop.gcnArchName;
+  }
+  *out = ctx;
+  return GH_OK;
+}
+
+void gh_destroy(gh_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &s : ctx->sims) free_all(s.owned);
+  free_all(ctx->phys_owned);
+  if (ctx->d_sims) (void)hipFree(ctx->d_sims);
+  if (ctx->d_chunk_off) (void)hipFree(ctx->d_chunk_off);
+  if (ctx->d_next) (void)hipFree(ctx->d_next);
+  if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+  if (ctx->d_energies) (void)hipFree(ctx->d_energies);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *gh_last_error(const gh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gh_device_info(gh_ctx *ctx, int32_t *n_cu, int32_t *lds_bytes, char *name, int32_t name_len) {
+  if (!ctx) return GH_EINVAL;
+  if (n_cu) *n_cu = ctx->n_cu;
+  if (lds_bytes) *lds_bytes = ctx->lds_max;
+  if (name && name_len > 0) {
+    std::strncpy(name, ctx->name.c_str(), (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return GH_OK;
+}
+
+int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
+  if (!ctx) return GH_EINVAL;
+  if (!ph || ph->n_materials <= 0 || ph->n_materials > 255 || ph->n_buckets <= 0 || ph->n_x_nodes < 2)
+    FAIL(GH_EINVAL, "gh_set_physics: invalid sizes");
+  if (!ph->density || !ph->mu_total || !ph->mu_majorant || !ph->p_compton || !ph->p_rayleigh ||
+      !ph->ray_x || !ph->ray_cdf || !ph->incoh_s)
+    FAIL(GH_EINVAL, "gh_set_physics: missing table");
+  if (ph->n_buckets > 64 * 24) FAIL(GH_ERANGE, "gh_set_physics: energy grid beyond 2^14 MeV");
+  if (!(ph->e_cut >= 0x1p-10f) || !std::isfinite(ph->e_cut))
+    FAIL(GH_EINVAL, "gh_set_physics: e_cut must be >= 2^-10 MeV (first energy node)");
+  {
+    const size_t M = ph->n_materials, NB1 = ph->n_buckets + 1, NX = ph->n_x_nodes;
+    for (size_t j = 0; j < NB1; ++j) {
+      if (!(ph->mu_majorant[j] > 0.0f) || !std::isfinite(ph->mu_majorant[j]))
+        FAIL(GH_EINVAL, "gh_set_physics: majorant must be positive and finite");
+      for (size_t m = 0; m < M; ++m) {
+        const float mu = ph->mu_total[m * NB1 + j];
+        if (!(mu >= 0.0f) || mu > ph->mu_majorant[j])
+          FAIL(GH_EINVAL, "gh_set_physics: mu_total must lie in [0, majorant]");
+      }
+    }
+    if (ph->ray_x[0] != 0.0f) FAIL(GH_EINVAL, "gh_set_physics: ray_x[0] must be 0");
+    for (size_t i = 1; i < NX; ++i)
+      if (!(ph->ray_x[i] > ph->ray_x[i - 1])) FAIL(GH_EINVAL, "gh_set_physics: ray_x must be ascending");
+    for (size_t m = 0; m < M; ++m)
+      for (size_t i = 1; i < NX; ++i)
+        if (!(ph->ray_cdf[m * NX + i] >= ph->ray_cdf[m * NX + i - 1]))
+          FAIL(GH_EINVAL, "gh_set_physics: ray_cdf must be non-decreasing");
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->sims.empty()) FAIL(GH_ESTATE, "gh_set_physics: clear simulations first");
+  free_all(ctx->phys_owned);
+  const size_t M = ph->n_materials, NB1 = ph->n_buckets + 1, NX = ph->n_x_nodes;
+  std::vector<float> inv_rho(M);
+  for (size_t i = 0; i < M; ++i) inv_rho[i] = ph->density[i] > 0.0f ? 1.0f / ph->density[i] : 0.0f;
+  std::vector<float> muen_zero;
+  const float *muen = ph->mu_en;
+  if (!muen) { muen_zero.assign(M * NB1, 0.0f); muen = muen_zero.data(); }
+  Phys P{};
+  float *t;
+  int rc;
+  if ((rc = dev_upload(ctx, ph->mu_total, M * NB1, &t, ctx->phys_owned))) return rc; P.mu = t;
+  if ((rc = dev_upload(ctx, ph->mu_majorant, NB1, &t, ctx->phys_owned))) return rc; P.maj = t;
+  if ((rc = dev_upload(ctx, ph->p_compton, M * NB1, &t, ctx->phys_owned))) return rc; P.pc = t;
+  if ((rc = dev_upload(ctx, ph->p_rayleigh, M * NB1, &t, ctx->phys_owned))) return rc; P.pr = t;
+  if ((rc = dev_upload(ctx, muen, M * NB1, &t, ctx->phys_owned))) return rc; P.muen = t;
+  if ((rc = dev_upload(ctx, ph->ray_x, NX, &t, ctx->phys_owned))) return rc; P.rx = t;
+  if ((rc = dev_upload(ctx, ph->ray_cdf, M * NX, &t, ctx->phys_owned))) return rc; P.rcdf = t;
+  if ((rc = dev_upload(ctx, ph->incoh_s, M * NX, &t, ctx->phys_owned))) return rc; P.sinc = t;
+  if ((rc = dev_upload(ctx, inv_rho.data(), M, &t, ctx->phys_owned))) return rc; P.inv_rho = t;
+  P.n_mat = (int)M;
+  P.nb = ph->n_buckets;
+  P.nx = (int)NX;
+  P.e_cut = ph->e_cut;
+  ctx->P = P;
+  ctx->lds_bytes = (M * NB1 + NB1) * sizeof(float);
+  ctx->have_phys = true;
+  return GH_OK;
+}
+
+int gh_clear_simulations(gh_ctx *ctx) {
+  if (!ctx) return GH_EINVAL;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (auto &s : ctx->sims) free_all(s.owned);
+  ctx->sims.clear();
+  return GH_OK;
+}
+
+int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_desc *src,
+                      const gh_tally_desc *tal, int32_t *sim_id) {
+  if (!ctx) return GH_EINVAL;
+  if (!ctx->have_phys) FAIL(GH_ESTATE, "gh_add_simulation: call gh_set_physics first");
+  if (!ph || !src || !tal) FAIL(GH_EINVAL, "gh_add_simulation: null descriptor");
+  if (ph->nx <= 0 || ph->ny <= 0 || ph->nz <= 0 || !ph->labels)
+    FAIL(GH_EINVAL, "gh_add_simulation: invalid phantom");
+  const int64_t nvox = (int64_t)ph->nx * ph->ny * ph->nz;
+  if (nvox >= (1ll << 31)) FAIL(GH_ERANGE, "gh_add_simulation: phantom has 2^31 voxels or more");
+  for (int i = 0; i < 3; ++i)
+    if (!(ph->spacing[i] > 0.0f)) FAIL(GH_EINVAL, "gh_add_simulation: spacing must be > 0");
+  if (ph->world_material >= ctx->P.n_mat) FAIL(GH_EINVAL, "gh_add_simulation: world material out of range");
+  for (int64_t i = 0; i < nvox; ++i)
+    if (ph->labels[i] >= ctx->P.n_mat) FAIL(GH_EINVAL, "gh_add_simulation: label out of range of the material table");
+  if (src->pos_type < GH_POS_POINT || src->pos_type > GH_POS_PHSP) FAIL(GH_EINVAL, "invalid pos_type");
+  if (src->dir_type < GH_DIR_ISO || src->dir_type > GH_DIR_FOCUSED) FAIL(GH_EINVAL, "invalid dir_type");
+  if (src->ene_type < GH_ENE_MONO || src->ene_type > GH_ENE_RANGE) FAIL(GH_EINVAL, "invalid ene_type");
+  if ((src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) &&
+      (src->n_energy <= 0 || !src->energy_value || !src->energy_cdf ||
+       (src->ene_type == GH_ENE_HISTOGRAM && !src->energy_width)))
+    FAIL(GH_EINVAL, "spectrum arrays missing");
+  if (src->pos_type == GH_POS_PHSP && (src->phsp_n <= 0 || src->phsp_start < 0))
+    FAIL(GH_EINVAL, "phase-space source needs rows");
+  if (src->pos_type == GH_POS_PHSP)
+    for (int k = 0; k < 8; ++k)
+      if (!src->phsp[k]) FAIL(GH_EINVAL, "phase-space array missing");
+  {
+    // every source energy must lie on the energy grid: (0, E_hi)
+    uint32_t hb = (uint32_t)(BUCKET_BASE + ctx->P.nb) << ESHIFT;
+    float e_hi;
+    std::memcpy(&e_hi, &hb, 4);
+    auto bad = [&](float e) { return !(e > 0.0f) || !(e < e_hi); };
+    bool out_of_grid = false;
+    if (src->pos_type == GH_POS_PHSP) {
+      for (int64_t i = 0; i < src->phsp_n && !out_of_grid; ++i) out_of_grid = bad(src->phsp[6][i]);
+    } else if (src->ene_type == GH_ENE_MONO) {
+      out_of_grid = bad(src->energy_mono);
+    } else if (src->ene_type == GH_ENE_RANGE) {
+      out_of_grid = bad(src->energy_mono) || bad(src->energy_max) || src->energy_max < src->energy_mono;
+    } else {
+      for (int i = 0; i < src->n_energy && !out_of_grid; ++i) {
+        const float top = src->ene_type == GH_ENE_HISTOGRAM ? src->energy_value[i] + src->energy_width[i]
+                                                            : src->energy_value[i];
+        out_of_grid = bad(src->energy_value[i]) || bad(top);
+        if (i > 0 && src->energy_cdf[i] < src->energy_cdf[i - 1]) out_of_grid = true;
+      }
+      if (!out_of_grid && !(src->energy_cdf[src->n_energy - 1] == 1.0f)) {
+        ctx->err = "gh_add_simulation: energy_cdf must end at exactly 1";
+        return GH_EINVAL;
+      }
+    }
+    if (out_of_grid) FAIL(GH_ERANGE, "gh_add_simulation: source energy outside (0, top of the physics tables)");
+  }
+  if (tal->nx <= 0 || tal->ny <= 0 || tal->nz <= 0) FAIL(GH_EINVAL, "invalid tally size");
+  if ((tal->flags & GH_TALLY_ALIGNED) && (tal->nx != ph->nx || tal->ny != ph->ny || tal->nz != ph->nz))
+    FAIL(GH_EINVAL, "aligned tally must have the phantom size");
+  if (tal->record_face >= 0 && ph->world_material >= 0)
+    FAIL(GH_EINVAL, "exit-face recording needs a vacuum world");
+  if (tal->record_face > 5) FAIL(GH_EINVAL, "record_face must be -1..5");
+  HIPCHK(hipSetDevice(ctx->device));
+
+  SimHost h;
+  SimDev &d = h.dev;
+  std::memset(&d, 0, sizeof(d));
+  int rc;
+  uint8_t *lab;
+  if ((rc = dev_upload(ctx, ph->labels, (size_t)nvox, &lab, h.owned))) { free_all(h.owned); return rc; }
+  d.labels = lab;
+  d.nx = ph->nx; d.ny = ph->ny; d.nz = ph->nz;
+  d.world_mat = ph->world_material;
+  for (int i = 0; i < 3; ++i) {
+    d.inv_s[i] = 1.0f / ph->spacing[i];
+    const int n = i == 0 ? ph->nx : (i == 1 ? ph->ny : ph->nz);
+    d.L[i] = (float)((double)n * (double)ph->spacing[i]);
+    d.wlo[i] = ph->world_lo[i];
+    d.whi[i] = ph->world_hi[i];
+  }
+  d.pos_type = src->pos_type; d.dir_type = src->dir_type; d.ene_type = src->ene_type;
+  d.n_energy = src->n_energy;
+  std::memcpy(d.pos_param, src->pos_param, sizeof(d.pos_param));
+  std::memcpy(d.pos_mat, src->pos_mat, sizeof(d.pos_mat));
+  std::memcpy(d.dir_mat, src->dir_mat, sizeof(d.dir_mat));
+  d.ct_min = src->cos_theta_min;
+  d.d_ct = src->cos_theta_min - src->cos_theta_max;
+  d.phi_min = src->phi_min;
+  d.d_phi = src->phi_max - src->phi_min;
+  d.phi_full = src->phi_full;
+  d.phsp_iso = src->phsp_iso;
+  std::memcpy(d.mom, src->momentum, sizeof(d.mom));
+  std::memcpy(d.focus, src->focus, sizeof(d.focus));
+  d.e_mono = src->energy_mono; d.e_max = src->energy_max; d.weight = src->weight;
+  float *t;
+  if (src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) {
+    if ((rc = dev_upload(ctx, src->energy_value, src->n_energy, &t, h.owned))) { free_all(h.owned); return rc; }
+    d.e_val = t;
+    if ((rc = dev_upload(ctx, src->energy_cdf, src->n_energy, &t, h.owned))) { free_all(h.owned); return rc; }
+    d.e_cdf = t;
+    if (src->ene_type == GH_ENE_HISTOGRAM) {
+      if ((rc = dev_upload(ctx, src->energy_width, src->n_energy, &t, h.owned))) { free_all(h.owned); return rc; }
+      d.e_width = t;
+    }
+  }
+  if (src->pos_type == GH_POS_PHSP) {
+    d.phsp_n = src->phsp_n; d.phsp_start = src->phsp_start;
+    for (int k = 0; k < 8; ++k) {
+      if ((rc = dev_upload(ctx, src->phsp[k], (size_t)src->phsp_n, &t, h.owned))) { free_all(h.owned); return rc; }
+      d.phsp[k] = t;
+    }
+  }
+  // tally
+  const size_t ntal = (size_t)tal->nx * tal->ny * tal->nz;
+  h.n_tally = ntal;
+  h.flags = tal->flags;
+  d.aligned = (tal->flags & GH_TALLY_ALIGNED) ? 1 : 0;
+  d.tnx = tal->nx; d.tny = tal->ny; d.tnz = tal->nz;
+  for (int i = 0; i < 3; ++i) {
+    d.tcorner[i] = tal->corner[i];
+    d.tinv[i] = 1.0f / tal->spacing[i];
+  }
+  double *dd;
+  if ((rc = dev_upload<double>(ctx, nullptr, ntal, &dd, h.owned))) { free_all(h.owned); return rc; }
+  d.edep = dd;
+  if (tal->flags & GH_TALLY_EDEP2) {
+    if ((rc = dev_upload<double>(ctx, nullptr, ntal, &dd, h.owned))) { free_all(h.owned); return rc; }
+    d.edep2 = dd;
+  }
+  if (tal->flags & GH_TALLY_DOSE) {
+    if ((rc = dev_upload<double>(ctx, nullptr, ntal, &dd, h.owned))) { free_all(h.owned); return rc; }
+    d.dose = dd;
+  }
+  if (tal->flags & GH_TALLY_DOSE2) {
+    if ((rc = dev_upload<double>(ctx, nullptr, ntal, &dd, h.owned))) { free_all(h.owned); return rc; }
+    d.dose2 = dd;
+  }
+  if (tal->flags & GH_TALLY_COUNTS) {
+    if ((rc = dev_upload<double>(ctx, nullptr, ntal, &dd, h.owned))) { free_all(h.owned); return rc; }
+    d.counts = dd;
+  }
+  d.rec_face = tal->record_face;
+  d.rec_cap = tal->record_face >= 0 ? tal->record_capacity : 0;
+  if (tal->record_face >= 0) {
+    unsigned long long *rc_ptr;
+    if ((rc = dev_upload<unsigned long long>(ctx, nullptr, 1, &rc_ptr, h.owned))) { free_all(h.owned); return rc; }
+    d.rec_count = rc_ptr;
+    for (int k = 0; k < 8; ++k) {
+      if ((rc = dev_upload<float>(ctx, nullptr, std::max<uint32_t>(d.rec_cap, 1u), &t, h.owned))) { free_all(h.owned); return rc; }
+      d.rec[k] = t;
+    }
+  }
+  d.det_label_min = tal->detector_label_min;
+  h.det = tal->detector_label_min;
+  ctx->sims.push_back(std::move(h));
+  if (sim_id) *sim_id = (int32_t)ctx->sims.size() - 1;
+  return gh_reset_tally(ctx, (int32_t)ctx->sims.size() - 1);
+}
+
+int gh_reset_tally(gh_ctx *ctx, int32_t sim_id) {
+  if (!ctx) return GH_EINVAL;
+  if (sim_id < 0 || sim_id >= (int32_t)ctx->sims.size()) FAIL(GH_EINVAL, "gh_reset_tally: bad sim id");
+  HIPCHK(hipSetDevice(ctx->device));
+  SimHost &h = ctx->sims[sim_id];
+  const size_t bytes = h.n_tally * sizeof(double);
+  double *arrs[5] = {h.dev.edep, h.dev.edep2, h.dev.dose, h.dev.dose2, h.dev.counts};
+  for (double *a : arrs)
+    if (a) HIPCHK(hipMemsetAsync(a, 0, bytes, ctx->stream));
+  if (h.dev.rec_count) HIPCHK(hipMemsetAsync(h.dev.rec_count, 0, sizeof(unsigned long long), ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
+int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_history, uint64_t seed,
+           const gh_launch *launch, gh_counters *out) {
+  if (!ctx) return GH_EINVAL;
+  if (!ctx->have_phys || ctx->sims.empty()) FAIL(GH_ESTATE, "gh_run: physics and simulations required");
+  if (!n_histories) FAIL(GH_EINVAL, "gh_run: n_histories is null");
+  HIPCHK(hipSetDevice(ctx->device));
+  const int ns = (int)ctx->sims.size();
+  gh_launch lp{256, 0, 16, 4096};
+  if (launch) {
+    if (launch->block > 0) lp.block = launch->block;
+    if (launch->grid > 0) lp.grid = launch->grid;
+    if (launch->refill_interval > 0) lp.refill_interval = launch->refill_interval;
+    if (launch->chunk > 0) lp.chunk = launch->chunk;
+  }
+  if (lp.block % 64 || lp.block > 256) FAIL(GH_EINVAL, "gh_run: block must be a multiple of 64 and <= 256");
+  if (lp.grid <= 0) lp.grid = std::max(1, ctx->n_cu) * 8;
+  std::vector<uint64_t> offs(ns + 1, 0);
+  std::vector<SimDev> devs(ns);
+  for (int s = 0; s < ns; ++s) {
+    SimHost &h = ctx->sims[s];
+    if (n_histories[s] > 0xFFFFFFFFull) FAIL(GH_ERANGE, "gh_run: more than 2^32-1 histories for one simulation in one launch");
+    h.dev.n_hist = n_histories[s];
+    h.dev.first_hist = first_history ? first_history[s] : 0;
+    if (h.det >= 0) {
+      // per-history detector energy buffer sized to this run
+      if (h.event_cap < n_histories[s]) {
+        float *t;
+        int rc;
+        if ((rc = dev_upload<float>(ctx, nullptr, (size_t)std::max<uint64_t>(n_histories[s], 1), &t, h.owned))) return rc;
+        h.dev.event_edep = t;
+        h.event_cap = n_histories[s];
+      }
+    }
+    devs[s] = h.dev;
+    offs[s + 1] = offs[s] + (n_histories[s] + lp.chunk - 1) / lp.chunk;
+  }
+  if (!ctx->d_sims || ctx->d_sims_cap < (size_t)ns) {
+    if (ctx->d_sims) { (void)hipFree(ctx->d_sims); ctx->d_sims = nullptr; }
+    if (ctx->d_chunk_off) { (void)hipFree(ctx->d_chunk_off); ctx->d_chunk_off = nullptr; }
+    HIPCHK(hipMalloc(&ctx->d_sims, ns * sizeof(SimDev)));
+    HIPCHK(hipMalloc(&ctx->d_chunk_off, (ns + 1) * sizeof(uint64_t)));
+    ctx->d_sims_cap = ns;
+  }
+  if (!ctx->d_next) {
+    HIPCHK(hipMalloc(&ctx->d_next, sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&ctx->d_counters, 12 * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&ctx->d_energies, 4 * sizeof(double)));
+  }
+  HIPCHK(hipMemcpyAsync(ctx->d_sims, devs.data(), ns * sizeof(SimDev), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_chunk_off, offs.data(), (ns + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_next, 0, sizeof(unsigned long long), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_counters, 0, 12 * sizeof(unsigned long long), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_energies, 0, 4 * sizeof(double), ctx->stream));
+
+  Launch L;
+  L.sims = ctx->d_sims;
+  L.n_sims = ns;
+  L.chunk_offsets = ctx->d_chunk_off;
+  L.total_chunks = offs[ns];
+  L.chunk = (uint32_t)lp.chunk;
+  L.next_chunk = ctx->d_next;
+  L.seed = seed;
+  L.refill = lp.refill_interval;
+  L.P = ctx->P;
+  L.counters = ctx->d_counters;
+  L.energies = ctx->d_energies;
+  const bool use_lds = ctx->lds_bytes <= 60 * 1024;
+  const size_t shmem = use_lds ? ctx->lds_bytes : 0;
+  HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+  if (L.total_chunks > 0) {
+    if (ns == 1) {
+      if (use_lds) hipLaunchKernelGGL((transport_kernel<false, true>), dim3(lp.grid), dim3(lp.block), shmem, ctx->stream, L);
+      else hipLaunchKernelGGL((transport_kernel<false, false>), dim3(lp.grid), dim3(lp.block), 0, ctx->stream, L);
+    } else {
+      if (use_lds) hipLaunchKernelGGL((transport_kernel<true, true>), dim3(lp.grid), dim3(lp.block), shmem, ctx->stream, L);
+      else hipLaunchKernelGGL((transport_kernel<true, false>), dim3(lp.grid), dim3(lp.block), 0, ctx->stream, L);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+  unsigned long long c[12];
+  double e[4];
+  HIPCHK(hipMemcpyAsync(c, ctx->d_counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(e, ctx->d_energies, sizeof(e), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  if (out) {
+    out->histories = c[0]; out->steps = c[1]; out->deposits = c[2]; out->compton = c[3];
+    out->rayleigh = c[4]; out->photo = c[5]; out->escaped = c[6]; out->world_interactions = c[7];
+    out->cut_kills = c[8]; out->sq_evictions = c[9]; out->recorded = c[10]; out->kn_rejections = c[11];
+    out->e_deposited = e[0]; out->e_outside_tally = e[1]; out->e_escaped = e[2]; out->e_world = e[3];
+    out->kernel_ms = ms;
+  }
+  return GH_OK;
+}
+
+int gh_get_tally(gh_ctx *ctx, int32_t sim_id, double *edep, double *edep2, double *dose, double *dose2,
+                 double *counts) {
+  if (!ctx) return GH_EINVAL;
+  if (sim_id < 0 || sim_id >= (int32_t)ctx->sims.size()) FAIL(GH_EINVAL, "gh_get_tally: bad sim id");
+  HIPCHK(hipSetDevice(ctx->device));
+  SimHost &h = ctx->sims[sim_id];
+  const size_t bytes = h.n_tally * sizeof(double);
+  double *dst[5] = {edep, edep2, dose, dose2, counts};
+  double *src[5] = {h.dev.edep, h.dev.edep2, h.dev.dose, h.dev.dose2, h.dev.counts};
+  const char *names[5] = {"edep", "edep2", "dose", "dose2", "counts"};
+  for (int k = 0; k < 5; ++k) {
+    if (!dst[k]) continue;
+    if (!src[k]) FAIL(GH_ESTATE, std::string("gh_get_tally: ") + names[k] + " not enabled");
+    HIPCHK(hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
+int gh_get_phsp(gh_ctx *ctx, int32_t sim_id, float *const out[8], uint64_t capacity, uint64_t *n) {
+  if (!ctx) return GH_EINVAL;
+  if (sim_id < 0 || sim_id >= (int32_t)ctx->sims.size()) FAIL(GH_EINVAL, "gh_get_phsp: bad sim id");
+  SimHost &h = ctx->sims[sim_id];
+  if (!h.dev.rec_count) FAIL(GH_ESTATE, "gh_get_phsp: recording not enabled");
+  HIPCHK(hipSetDevice(ctx->device));
+  unsigned long long cnt = 0;
+  HIPCHK(hipMemcpy(&cnt, h.dev.rec_count, sizeof(cnt), hipMemcpyDeviceToHost));
+  const uint64_t stored = std::min<uint64_t>(cnt, h.dev.rec_cap);
+  if (n) *n = cnt;
+  if (out) {
+    const uint64_t m = std::min<uint64_t>(stored, capacity);
+    for (int k = 0; k < 8; ++k)
+      if (out[k] && m) HIPCHK(hipMemcpy(out[k], h.dev.rec[k], m * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  return GH_OK;
+}
+
+int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n) {
+  if (!ctx) return GH_EINVAL;
+  if (sim_id < 0 || sim_id >= (int32_t)ctx->sims.size()) FAIL(GH_EINVAL, "gh_get_event_edep: bad sim id");
+  SimHost &h = ctx->sims[sim_id];
+  if (h.det < 0 || !h.dev.event_edep) FAIL(GH_ESTATE, "gh_get_event_edep: detector scoring not enabled");
+  if (n > h.event_cap) FAIL(GH_ERANGE, "gh_get_event_edep: n beyond the last run");
+  HIPCHK(hipSetDevice(ctx->device));
+  if (n) HIPCHK(hipMemcpy(out, h.dev.event_edep, n * sizeof(float), hipMemcpyDeviceToHost));
+  return GH_OK;
+}
+
+int gh_hu_to_labels(gh_ctx *ctx, const int16_t *hu, int64_t n_voxels, const double *bins, int32_t n_bins,
+                    const uint16_t *labels_sorted, uint16_t *out) {
+  if (!ctx) return GH_EINVAL;
+  if (!hu || !bins || !labels_sorted || !out || n_voxels < 0) FAIL(GH_EINVAL, "gh_hu_to_labels: null argument");
+  if (n_bins < 0 || n_bins > 256) FAIL(GH_ERANGE, "gh_hu_to_labels: at most 256 bins");
+  for (int i = 1; i < n_bins; ++i)
+    if (!(bins[i - 1] <= bins[i])) FAIL(GH_EINVAL, "gh_hu_to_labels: bins must be ascending");
+  if (n_voxels == 0) return GH_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<void *> tmp;
+  int16_t *d_hu;
+  double *d_bins;
+  uint16_t *d_lut, *d_out;
+  int rc;
+  double dummy = 0.0;
+  if ((rc = dev_upload(ctx, hu, (size_t)n_voxels, &d_hu, tmp))) { free_all(tmp); return rc; }
+  if ((rc = dev_upload(ctx, n_bins ? bins : &dummy, (size_t)std::max(n_bins, 1), &d_bins, tmp))) { free_all(tmp); return rc; }
+  if ((rc = dev_upload(ctx, labels_sorted, (size_t)n_bins + 1, &d_lut, tmp))) { free_all(tmp); return rc; }
+  if ((rc = dev_upload<uint16_t>(ctx, nullptr, (size_t)n_voxels, &d_out, tmp))) { free_all(tmp); return rc; }
+  const int block = 256;
+  const int64_t want = (n_voxels + block - 1) / block;
+  const int grid = (int)std::min<int64_t>(want, (int64_t)std::max(ctx->n_cu, 1) * 16);
+  hipLaunchKernelGGL(hu_to_labels_kernel, dim3(grid), dim3(block), 0, ctx->stream, d_hu, n_voxels, d_bins,
+                     n_bins, d_lut, d_out);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n_voxels * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  free_all(tmp);
+  if (e != hipSuccess) FAIL(GH_EHIP, std::string("gh_hu_to_labels: ") + hipGetErrorString(e));
+  return GH_OK;
+}
+
+}  // extern "C"

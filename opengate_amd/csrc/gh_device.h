@@ -1,0 +1,282 @@
+// Device building blocks of the photon transport: counter-based RNG, FP32 math with a
+// fixed operation order (so that a CPU build of the same sequence gives the same bits),
+// table lookups and interaction samplers. Everything here is written for wave64 CDNA4.
+//
+// Numerical contract (also followed by the CPU restatement in oracle/transport_ref.c):
+//   * compiled with -ffp-contract=off: every fused multiply-add is an explicit fmaf;
+//   * division and sqrtf are IEEE correctly rounded (HIP default for FP32);
+//   * log and sin/cos are the series below, never the math library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GH_DEV __device__ __forceinline__
+
+namespace gh {
+
+// ---------------------------------------------------------------- constants
+constexpr int KBITS = 6;
+constexpr int ESHIFT = 23 - KBITS;                    // 17
+constexpr uint32_t EMASK = (1u << ESHIFT) - 1u;
+constexpr int BUCKET_BASE = ((127 - 10) << 23) >> ESHIFT;  // 7488: E_LO = 2^-10 MeV
+constexpr float MEC2 = 0x1.05a1a8p-1f;                // 0.51099895 MeV
+constexpr float KPM = 0x1.fac5e8p+8f;                 // photon wave number per MeV [1/Angstrom]
+constexpr float EXIT_EPS = 1e-3f;                     // shortest in-box chord that counts as a hit [mm]
+constexpr int MAX_REJECT = 1000;                      // cap of every rejection loop
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC11).
+struct Rng {
+  uint32_t k0, k1;      // key = seed
+  uint32_t c0, c1, c2;  // history id lo/hi, simulation id
+  uint32_t c3;          // block counter
+  uint32_t b0, b1, b2, b3;
+  int n;                // words left in the buffer (consumed b0 first)
+};
+
+GH_DEV void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x2, uint32_t &x3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * x0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
+    const uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+    const uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+    x1 = (uint32_t)p1;
+    x3 = (uint32_t)p0;
+    x0 = y0;
+    x2 = y2;
+  }
+}
+
+GH_DEV void rng_init(Rng &g, uint64_t seed, uint64_t history, uint32_t sim) {
+  g.k0 = (uint32_t)seed; g.k1 = (uint32_t)(seed >> 32);
+  g.c0 = (uint32_t)history; g.c1 = (uint32_t)(history >> 32); g.c2 = sim; g.c3 = 0;
+  g.n = 0;
+}
+
+GH_DEV uint32_t rng_next(Rng &g) {
+  if (g.n == 0) {
+    uint32_t x0 = g.c0, x1 = g.c1, x2 = g.c2, x3 = g.c3;
+    philox10(x0, x1, x2, x3, g.k0, g.k1);
+    g.b0 = x0; g.b1 = x1; g.b2 = x2; g.b3 = x3;
+    g.c3 += 1u;
+    g.n = 4;
+  }
+  const uint32_t r = g.b0;
+  g.b0 = g.b1; g.b1 = g.b2; g.b2 = g.b3;
+  g.n -= 1;
+  return r;
+}
+
+// uniform in (0, 1): odd multiples of 2^-24
+GH_DEV float u01(Rng &g) { return (float)((rng_next(g) >> 8) | 1u) * 0x1p-24f; }
+
+// ---------------------------------------------------------------- math
+GH_DEV float glog(float x) {
+  // x > 0 normal. log(x) = e ln2 + 2 atanh(s), s = f/(2+f), f = m-1, m in (sqrt(1/2), sqrt(2)]
+  uint32_t bits = __float_as_uint(x);
+  int e = (int)(bits >> 23) - 127;
+  uint32_t mb = (bits & 0x007FFFFFu) | 0x3F800000u;
+  if (mb > 0x3FB504F3u) { mb -= 0x00800000u; e += 1; }
+  const float f = __uint_as_float(mb) - 1.0f;
+  const float s = f / (2.0f + f);
+  const float z = s * s;
+  float p = fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f);
+  p = fmaf(z, p, 0x1.99999ap-2f);
+  p = fmaf(z, p, 0x1.555556p-1f);
+  p = p * z;
+  const float l1 = fmaf(s, p, s + s);
+  const float ef = (float)e;
+  return fmaf(ef, 0x1.62ep-1f, fmaf(ef, 0x1.0bfbe8p-15f, l1));
+}
+
+GH_DEV void gsincos(float x, float &sn, float &cs) {
+  // |x| <= 16 pi. Cody-Waite reduction by pi/2, Taylor polynomials on [-pi/4, pi/4].
+  const float k = rintf(x * 0x1.45f306p-1f);
+  const int q = (int)k;
+  float r = fmaf(-k, 0x1.921fp+0f, x);
+  r = fmaf(-k, 0x1.6a88p-17f, r);
+  r = fmaf(-k, 0x1.0b4612p-34f, r);
+  const float r2 = r * r;
+  float t = fmaf(r2, 0x1.71de3ap-19f, -0x1.a01a02p-13f);
+  t = fmaf(r2, t, 0x1.111112p-7f);
+  t = fmaf(r2, t, -0x1.555556p-3f);
+  const float sp = fmaf(r * r2, t, r);
+  float c = fmaf(r2, -0x1.27e4fcp-22f, 0x1.a01a02p-16f);
+  c = fmaf(r2, c, -0x1.6c16c2p-10f);
+  c = fmaf(r2, c, 0x1.555556p-5f);
+  const float cp = fmaf(r2 * r2, c, fmaf(-0.5f, r2, 1.0f));
+  switch (q & 3) {
+    case 0: sn = sp; cs = cp; break;
+    case 1: sn = cp; cs = -sp; break;
+    case 2: sn = -sp; cs = -cp; break;
+    default: sn = -cp; cs = sp; break;
+  }
+}
+
+GH_DEV void normalize3(float &x, float &y, float &z) {
+  const float n2 = fmaf(z, z, fmaf(y, y, x * x));
+  const float inv = 1.0f / sqrtf(n2);
+  x = x * inv; y = y * inv; z = z * inv;
+}
+
+// (cos phi, sin phi) for phi uniform in [0, 2pi): point uniform in the unit disc
+GH_DEV void azimuth(Rng &g, float &cp, float &sp) {
+  float a, b, r2;
+  int it = 0;
+  do {
+    a = fmaf(2.0f, u01(g), -1.0f);
+    b = fmaf(2.0f, u01(g), -1.0f);
+    r2 = fmaf(b, b, a * a);
+  } while ((r2 > 1.0f || r2 == 0.0f) && ++it < MAX_REJECT);
+  const float inv = 1.0f / r2;
+  cp = fmaf(a, a, -(b * b)) * inv;
+  sp = ((a + a) * b) * inv;
+}
+
+// new direction after scattering by polar cosine c and azimuth (cp, sp)
+GH_DEV void rotate(float &u, float &v, float &w, float c, float cp, float sp) {
+  const float st = sqrtf(fmaxf(fmaf(-c, c, 1.0f), 0.0f));
+  float nu, nv, nw;
+  if (fabsf(w) < 0.99999f) {
+    const float tmp = sqrtf(fmaf(-w, w, 1.0f));
+    const float f = st / tmp;
+    nu = fmaf(c, u, f * fmaf(u * w, cp, -(v * sp)));
+    nv = fmaf(c, v, f * fmaf(v * w, cp, u * sp));
+    nw = fmaf(c, w, -((st * cp) * tmp));
+  } else {
+    nu = st * cp;
+    nv = st * sp;
+    nw = (w > 0.0f) ? c : -c;
+  }
+  normalize3(nu, nv, nw);
+  u = nu; v = nv; w = nw;
+}
+
+// largest i in [0, n-2] with a[i] <= x (a ascending)
+GH_DEV int bsearch_le(const float *a, int n, float x) {
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// smallest k in [0, n-1] with cdf[k] > u (cdf ascending, cdf[n-1] = 1 > u)
+GH_DEV int cdf_index(const float *cdf, int n, float u) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------- energy grid
+struct ECache {
+  int b;         // bucket
+  float frac;    // position inside the bucket
+  float maj;     // majorant at E
+  float inv_maj;
+};
+
+GH_DEV void ecache_set(ECache &c, float E, const float *maj) {
+  const uint32_t bits = __float_as_uint(E);
+  c.b = (int)(bits >> ESHIFT) - BUCKET_BASE;
+  c.frac = (float)(bits & EMASK) * 0x1p-17f;
+  const float m0 = maj[c.b], m1 = maj[c.b + 1];
+  c.maj = fmaf(c.frac, m1 - m0, m0);
+  c.inv_maj = 1.0f / c.maj;
+}
+
+GH_DEV float lerp_node(const float *row, const ECache &c) {
+  const float v0 = row[c.b], v1 = row[c.b + 1];
+  return fmaf(c.frac, v1 - v0, v0);
+}
+
+// value of a table on the q^2 grid at x (linear, clamped to the grid)
+GH_DEV float xgrid_lerp(const float *xs, const float *ys, int n, float x) {
+  const int i = bsearch_le(xs, n, x);
+  const float x0 = xs[i], x1 = xs[i + 1];
+  float f = (x - x0) / (x1 - x0);
+  f = fminf(fmaxf(f, 0.0f), 1.0f);
+  const float y0 = ys[i], y1 = ys[i + 1];
+  return fmaf(f, y1 - y0, y0);
+}
+
+struct Phys {
+  const float *mu;       // [M][NB+1] (global copy)
+  const float *maj;      // [NB+1]
+  const float *pc;       // [M][NB+1]
+  const float *pr;       // [M][NB+1]
+  const float *muen;     // [M][NB+1]
+  const float *rx;       // [NX]
+  const float *rcdf;     // [M][NX]
+  const float *sinc;     // [M][NX]
+  const float *inv_rho;  // [M]
+  int n_mat, nb, nx;
+  float e_cut;
+};
+
+// Klein-Nishina by Kahn's rejection method (H. Kahn, "Applications of Monte Carlo",
+// RM-1237-AEC, 1956) followed by acceptance with S(q)/Z. Returns q = E/E' and cos(theta).
+GH_DEV void sample_compton(Rng &g, const Phys &P, int lab, float E, float &q_out, float &c_out,
+                           uint32_t &rejections) {
+  const float alpha = E / MEC2;
+  const float a2 = alpha + alpha;
+  const float onep = 1.0f + a2;
+  const float thr = onep / (9.0f + a2);
+  const float kk = E * KPM;
+  const float two_k2 = (2.0f * kk) * kk;
+  const float *srow = P.sinc + (size_t)lab * P.nx;
+  float q = 1.0f, c = 1.0f;
+  int it = 0;
+  for (; it < MAX_REJECT; ++it) {
+    const float r1 = u01(g), r2 = u01(g), r3 = u01(g);
+    bool acc;
+    if (r1 <= thr) {
+      q = fmaf(a2, r2, 1.0f);
+      const float iq = 1.0f / q;
+      acc = r3 <= 4.0f * (iq - iq * iq);
+    } else {
+      q = onep / fmaf(a2, r2, 1.0f);
+      const float cc = 1.0f - (q - 1.0f) / alpha;
+      acc = r3 <= 0.5f * fmaf(cc, cc, 1.0f / q);
+    }
+    if (!acc) { rejections++; continue; }
+    c = 1.0f - (q - 1.0f) / alpha;
+    const float x = two_k2 * (1.0f - c);
+    const float s = xgrid_lerp(P.rx, srow, P.nx, x);
+    if (u01(g) <= s) break;
+    rejections++;
+  }
+  if (it == MAX_REJECT) { q = 1.0f; c = 1.0f; }
+  q_out = q;
+  c_out = fmaxf(c, -1.0f);
+}
+
+// Rayleigh: q^2 from the tabulated int F^2 (inverse by linear interpolation), then
+// acceptance with (1 + cos^2)/2.
+GH_DEV float sample_rayleigh(Rng &g, const Phys &P, int lab, float E) {
+  const float kk = E * KPM;
+  const float two_k2 = (2.0f * kk) * kk;
+  const float xmax = two_k2 + two_k2;
+  const float *G = P.rcdf + (size_t)lab * P.nx;
+  const float gmax = xgrid_lerp(P.rx, G, P.nx, xmax);
+  float c = 1.0f;
+  for (int it = 0; it < MAX_REJECT; ++it) {
+    const float t = u01(g) * gmax;
+    const int j = bsearch_le(G, P.nx, t);
+    const float g0 = G[j], g1 = G[j + 1];
+    const float dg = g1 - g0;
+    const float x0 = P.rx[j], x1 = P.rx[j + 1];
+    const float x = (dg > 0.0f) ? fmaf((t - g0) / dg, x1 - x0, x0) : x0;
+    c = fmaxf(1.0f - x / two_k2, -1.0f);
+    if (u01(g) <= 0.5f * fmaf(c, c, 1.0f)) break;
+  }
+  return c;
+}
+
+}  // namespace gh

@@ -1,0 +1,331 @@
+"""Host-side descriptors of one simulation and the GPU engine handle.
+
+`PhantomSpec`, `SourceSpec` and `TallySpec` hold numpy buffers and produce the C
+descriptors of include/gatehip.h; `GpuEngine` owns a gh_ctx. The same specs feed the
+CPU oracle in tests, so both sides always see identical FP32 inputs.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _abi as A
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+@dataclass
+class PhantomSpec:
+    """Label volume in the phantom frame: box [0, n*spacing), labels[z, y, x]."""
+
+    labels: np.ndarray  # uint8 (nz, ny, nx)
+    spacing: tuple  # mm (sx, sy, sz)
+    world_material: int = -1  # -1: vacuum outside the box
+    world_lo: tuple = (0.0, 0.0, 0.0)  # world box in the phantom frame
+    world_hi: tuple = (0.0, 0.0, 0.0)
+
+    def __post_init__(self):
+        self.labels = np.ascontiguousarray(self.labels, dtype=np.uint8)
+        assert self.labels.ndim == 3
+
+    @property
+    def shape_xyz(self):
+        nz, ny, nx = self.labels.shape
+        return nx, ny, nz
+
+    def desc(self):
+        nx, ny, nz = self.shape_xyz
+        d = A.PhantomDesc()
+        d.nx, d.ny, d.nz = nx, ny, nz
+        d.world_material = int(self.world_material)
+        d.spacing = _f3(self.spacing)
+        d.world_lo = _f3(self.world_lo)
+        d.world_hi = _f3(self.world_hi)
+        d.labels = self.labels.ctypes.data_as(C.POINTER(C.c_uint8))
+        return d
+
+
+def spectrum_lines(energies, weights):
+    """spectrum_discrete: normalised cumulative weights (GateGenericSource.cpp:544-573)."""
+    w = np.cumsum(np.asarray(weights, dtype=np.float64))
+    cdf = (w / w[-1]).astype(np.float32)
+    cdf[-1] = 1.0
+    return np.asarray(energies, dtype=np.float32), None, cdf
+
+
+def spectrum_histogram(edges, weights):
+    """spectrum_histogram with no interpolation: cumulative w_i * dE_i
+    (GateGenericSource.cpp:583-609), energy uniform inside the chosen bin."""
+    e = np.asarray(edges, dtype=np.float64)
+    w = np.asarray(weights, dtype=np.float64)
+    if len(e) != len(w) + 1:
+        raise ValueError("spectrum_energy_bin_edges must have one more element than spectrum_weights")
+    acc = np.cumsum(w * np.diff(e))
+    cdf = (acc / acc[-1]).astype(np.float32)
+    cdf[-1] = 1.0
+    return e[:-1].astype(np.float32), np.diff(e).astype(np.float32), cdf
+
+
+@dataclass
+class SourceSpec:
+    pos_type: str = "point"
+    pos_param: tuple = (0.0, 0.0, 0.0, 0.0)
+    pos_rot: np.ndarray = field(default_factory=lambda: np.eye(3))  # source frame -> phantom frame
+    pos_trans: tuple = (0.0, 0.0, 0.0)
+    dir_type: str = "iso"
+    dir_rot: np.ndarray = field(default_factory=lambda: np.eye(3))
+    theta: tuple = (0.0, np.pi)
+    phi: tuple = (0.0, 2 * np.pi)
+    momentum: tuple = (0.0, 0.0, 1.0)
+    focus: tuple = (0.0, 0.0, 0.0)  # phantom frame
+    ene_type: str = "mono"
+    energy: float = 0.1405  # mono / range min (MeV)
+    energy_max: float = 0.0
+    energy_value: Optional[np.ndarray] = None
+    energy_width: Optional[np.ndarray] = None
+    energy_cdf: Optional[np.ndarray] = None
+    weight: float = 1.0
+    phsp: Optional[list] = None  # 8 float32 arrays x y z dx dy dz E w (source frame)
+    phsp_start: int = 0
+    phsp_iso: bool = False
+
+    def max_energy(self):
+        if self.pos_type == "phsp":
+            return float(np.max(self.phsp[6])) if len(self.phsp[6]) else 0.0
+        if self.ene_type == "mono":
+            return float(self.energy)
+        if self.ene_type == "range":
+            return float(self.energy_max)
+        if self.ene_type == "spectrum_histogram":
+            return float(self.energy_value[-1] + self.energy_width[-1])
+        return float(np.max(self.energy_value))
+
+    def desc(self):
+        d = A.SourceDesc()
+        d.pos_type = A.POS[self.pos_type]
+        d.dir_type = A.DIR[self.dir_type]
+        d.ene_type = A.ENE[self.ene_type]
+        pp = list(self.pos_param) + [0.0] * (4 - len(self.pos_param))
+        d.pos_param = (C.c_float * 4)(*[float(x) for x in pp[:4]])
+        R = np.asarray(self.pos_rot, dtype=np.float64).reshape(3, 3)
+        d.pos_mat = (C.c_float * 12)(*[float(x) for x in R.ravel()], *[float(x) for x in self.pos_trans])
+        D = np.asarray(self.dir_rot, dtype=np.float64).reshape(3, 3)
+        d.dir_mat = (C.c_float * 9)(*[float(x) for x in D.ravel()])
+        d.cos_theta_min = float(np.cos(self.theta[0]))
+        d.cos_theta_max = float(np.cos(self.theta[1]))
+        d.phi_min = float(self.phi[0])
+        d.phi_max = float(self.phi[1])
+        d.phi_full = int(abs((self.phi[1] - self.phi[0]) - 2 * np.pi) < 1e-12)
+        d.momentum = _f3(self.momentum)
+        d.focus = _f3(self.focus)
+        d.energy_mono = float(self.energy)
+        d.energy_max = float(self.energy_max)
+        d.weight = float(self.weight)
+        if self.ene_type in ("spectrum_discrete", "spectrum_histogram"):
+            self._ev = np.ascontiguousarray(self.energy_value, dtype=np.float32)
+            self._ec = np.ascontiguousarray(self.energy_cdf, dtype=np.float32)
+            d.n_energy = len(self._ev)
+            d.energy_value = A.fptr(self._ev)
+            d.energy_cdf = A.fptr(self._ec)
+            if self.ene_type == "spectrum_histogram":
+                self._ew = np.ascontiguousarray(self.energy_width, dtype=np.float32)
+                d.energy_width = A.fptr(self._ew)
+        if self.pos_type == "phsp":
+            self._ph = [np.ascontiguousarray(a, dtype=np.float32) for a in self.phsp]
+            d.phsp_n = len(self._ph[0])
+            d.phsp_start = int(self.phsp_start)
+            d.phsp_iso = int(bool(self.phsp_iso))
+            d.phsp = (A.FP * 8)(*[A.fptr(a) for a in self._ph])
+        return d
+
+
+@dataclass
+class TallySpec:
+    size: tuple  # (nx, ny, nz)
+    spacing: tuple = (1.0, 1.0, 1.0)
+    corner: tuple = (0.0, 0.0, 0.0)  # phantom frame
+    aligned: bool = True
+    edep2: bool = True
+    dose: bool = False  # per-deposit dose (needed only when not aligned)
+    dose2: bool = False
+    counts: bool = False
+    record_face: int = -1
+    record_capacity: int = 0
+    detector_label_min: int = -1
+
+    def flags(self):
+        f = 0
+        if self.edep2:
+            f |= A.TALLY_EDEP2
+        if self.dose:
+            f |= A.TALLY_DOSE
+        if self.dose2:
+            f |= A.TALLY_DOSE2
+        if self.counts:
+            f |= A.TALLY_COUNTS
+        if self.aligned:
+            f |= A.TALLY_ALIGNED
+        return f
+
+    @property
+    def n_voxels(self):
+        return int(np.prod(self.size))
+
+    def desc(self):
+        d = A.TallyDesc()
+        d.nx, d.ny, d.nz = [int(x) for x in self.size]
+        d.flags = self.flags()
+        d.corner = _f3(self.corner)
+        d.spacing = _f3(self.spacing)
+        d.record_face = int(self.record_face)
+        d.record_capacity = int(self.record_capacity)
+        d.detector_label_min = int(self.detector_label_min)
+        return d
+
+
+def physics_desc(tables):
+    """C descriptor for an opengate_amd.physics.xs.PhysicsTables (keeps arrays alive on the desc)."""
+    d = A.PhysicsDesc()
+    d.n_materials = tables.n_materials
+    d.n_buckets = tables.n_buckets
+    d.n_x_nodes = len(tables.ray_x)
+    keep = []
+
+    def f(a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        keep.append(a)
+        return A.fptr(a)
+
+    d.density = f(tables.density)
+    d.mu_total = f(tables.mu_total)
+    d.mu_majorant = f(tables.mu_majorant)
+    d.p_compton = f(tables.p_compton)
+    d.p_rayleigh = f(tables.p_rayleigh)
+    d.mu_en = f(tables.mu_en)
+    d.ray_x = f(tables.ray_x)
+    d.ray_cdf = f(tables.ray_cdf)
+    d.incoh_s = f(tables.incoh_s)
+    d.e_cut = float(tables.e_cut)
+    d._keep = keep
+    return d
+
+
+class GhError(RuntimeError):
+    pass
+
+
+class GpuEngine:
+    """Owns one gh_ctx on one device."""
+
+    def __init__(self, device=0):
+        self.lib = A.load_library()
+        h = C.c_void_p()
+        rc = self.lib.gh_create(int(device), C.byref(h))
+        if rc != A.GH_OK:
+            raise GhError(f"gh_create(device={device}) failed with code {rc} (no usable HIP device?)")
+        self.ctx = h
+        self.sims = []
+        self.tables = None
+
+    def _check(self, rc, what):
+        if rc != A.GH_OK:
+            msg = self.lib.gh_last_error(self.ctx).decode()
+            raise GhError(f"{what}: {msg} (code {rc})")
+
+    def close(self):
+        if self.ctx:
+            self.lib.gh_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        n_cu, lds = C.c_int32(), C.c_int32()
+        name = C.create_string_buffer(64)
+        self._check(self.lib.gh_device_info(self.ctx, C.byref(n_cu), C.byref(lds), name, 64), "gh_device_info")
+        return {"n_cu": n_cu.value, "lds_bytes": lds.value, "arch": name.value.decode()}
+
+    def set_physics(self, tables):
+        self._pd = physics_desc(tables)
+        self._check(self.lib.gh_set_physics(self.ctx, C.byref(self._pd)), "gh_set_physics")
+        self.tables = tables
+
+    def clear(self):
+        self._check(self.lib.gh_clear_simulations(self.ctx), "gh_clear_simulations")
+        self.sims = []
+
+    def add_simulation(self, phantom: PhantomSpec, source: SourceSpec, tally: TallySpec):
+        if self.tables is not None and source.max_energy() >= self.tables.e_max:
+            raise GhError(f"source energy {source.max_energy()} MeV beyond the physics tables ({self.tables.e_max} MeV)")
+        pd, sd, td = phantom.desc(), source.desc(), tally.desc()
+        sid = C.c_int32()
+        self._check(self.lib.gh_add_simulation(self.ctx, C.byref(pd), C.byref(sd), C.byref(td), C.byref(sid)),
+                    "gh_add_simulation")
+        self.sims.append((phantom, source, tally))
+        return sid.value
+
+    def reset_tally(self, sim_id):
+        self._check(self.lib.gh_reset_tally(self.ctx, int(sim_id)), "gh_reset_tally")
+
+    def run(self, n_histories, seed, first_history=None, block=256, grid=0, refill_interval=16, chunk=4096):
+        n = np.ascontiguousarray(np.atleast_1d(n_histories), dtype=np.uint64)
+        if len(n) != len(self.sims):
+            raise GhError("one history count per simulation is required")
+        f = np.zeros_like(n) if first_history is None else np.ascontiguousarray(np.atleast_1d(first_history), dtype=np.uint64)
+        lp = A.Launch(int(block), int(grid), int(refill_interval), int(chunk))
+        out = A.Counters()
+        self._check(self.lib.gh_run(self.ctx, n.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                    f.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_uint64(int(seed)),
+                                    C.byref(lp), C.byref(out)), "gh_run")
+        return out.as_dict()
+
+    def get_tally(self, sim_id=0):
+        _, _, tal = self.sims[sim_id]
+        nz_ny_nx = tuple(reversed([int(x) for x in tal.size]))
+        res = {"edep": np.zeros(nz_ny_nx)}
+        if tal.edep2:
+            res["edep2"] = np.zeros(nz_ny_nx)
+        if tal.dose:
+            res["dose"] = np.zeros(nz_ny_nx)
+        if tal.dose2:
+            res["dose2"] = np.zeros(nz_ny_nx)
+        if tal.counts:
+            res["counts"] = np.zeros(nz_ny_nx)
+        self._check(self.lib.gh_get_tally(self.ctx, int(sim_id), A.dptr(res["edep"]), A.dptr(res.get("edep2")),
+                                          A.dptr(res.get("dose")), A.dptr(res.get("dose2")),
+                                          A.dptr(res.get("counts"))), "gh_get_tally")
+        return res
+
+    def get_phsp(self, sim_id=0):
+        _, _, tal = self.sims[sim_id]
+        n = C.c_uint64()
+        self._check(self.lib.gh_get_phsp(self.ctx, int(sim_id), (A.FP * 8)(), 0, C.byref(n)), "gh_get_phsp")
+        m = min(int(n.value), int(tal.record_capacity))
+        arrs = [np.zeros(m, dtype=np.float32) for _ in range(8)]
+        self._check(self.lib.gh_get_phsp(self.ctx, int(sim_id), (A.FP * 8)(*[A.fptr(a) for a in arrs]), m,
+                                         C.byref(n)), "gh_get_phsp")
+        return arrs, int(n.value)
+
+    def get_event_edep(self, sim_id, n):
+        out = np.zeros(int(n), dtype=np.float32)
+        self._check(self.lib.gh_get_event_edep(self.ctx, int(sim_id), A.fptr(out), int(n)), "gh_get_event_edep")
+        return out
+
+    def hu_to_labels(self, hu, bins, labels_sorted):
+        hu = np.ascontiguousarray(hu, dtype=np.int16)
+        bins = np.ascontiguousarray(bins, dtype=np.float64)
+        lut = np.ascontiguousarray(labels_sorted, dtype=np.uint16)
+        if len(lut) != len(bins) + 1:
+            raise GhError("labels_sorted must have len(bins)+1 entries")
+        out = np.zeros(hu.shape, dtype=np.uint16)
+        self._check(self.lib.gh_hu_to_labels(self.ctx, hu.ctypes.data_as(C.POINTER(C.c_int16)), hu.size,
+                                             A.dptr(bins), len(bins), lut.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                             out.ctypes.data_as(C.POINTER(C.c_uint16))), "gh_hu_to_labels")
+        return out

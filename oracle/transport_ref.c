@@ -1,0 +1,807 @@
+/* transport_ref.c -- CPU restatement of the photon transport + voxel dose path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the oracle the GPU library is checked
+ * against (and the CPU baseline that bench.py times). Nothing in the product links,
+ * loads or calls it.
+ *
+ * It restates, history by history, the algorithm the HIP kernel runs: Philox4x32-10
+ * streams keyed per history, source sampling (GateGenericSource.cpp:312-664,
+ * GateSPSEneDistribution.cpp:16-159, GatePhaseSpaceSource.cpp:142-228), Woodcock
+ * tracking through the voxel labels (the material lookup of
+ * GateImageNestedParameterisation::ComputeMaterial, GateImageNestedParameterisation.cpp:47-75),
+ * Compton / Rayleigh / photoelectric interactions with local deposition of electron
+ * energy, and the dose tally of GateDoseActor::SteppingAction / ScoreSquaredValue
+ * (GateDoseActor.cpp:180-243, 359-397): edep = sum of deposits, edep^2 = sum over
+ * histories of the squared per-history voxel sum.
+ *
+ * Numerical contract shared with the kernel: FP32 arithmetic, compiled with
+ * -ffp-contract=off, every fused multiply-add spelled fmaf, correctly rounded / and
+ * sqrtf, log and sin/cos from the series below. With identical inputs each history
+ * then takes the same path on both sides, so integer counters agree exactly and FP64
+ * tallies agree up to the order of the FP64 additions.
+ *
+ * The squared tally is computed two ways: the 4-slot per-history cache the kernel
+ * uses (FIFO eviction of the oldest voxel when a history touches a fifth voxel) and,
+ * in `exact_edep2`, the exact per-history voxel sums (no slot limit).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/gatehip.h"
+
+/* ------------------------------------------------------------------ Philox */
+void ref_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    uint64_t a = (uint64_t)0xD2511F53u * (uint64_t)x0;
+    uint64_t b = (uint64_t)0xCD9E8D57u * (uint64_t)x2;
+    uint32_t n0 = (uint32_t)(b >> 32) ^ x1 ^ k0;
+    uint32_t n1 = (uint32_t)b;
+    uint32_t n2 = (uint32_t)(a >> 32) ^ x3 ^ k1;
+    uint32_t n3 = (uint32_t)a;
+    x0 = n0; x1 = n1; x2 = n2; x3 = n3;
+  }
+  out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+typedef struct {
+  uint32_t key[2];
+  uint32_t ctr[4];
+  uint32_t buf[4];
+  int pos; /* next word of buf to hand out; 4 = empty */
+} stream_t;
+
+static void stream_open(stream_t *s, uint64_t seed, uint64_t history, uint32_t sim) {
+  s->key[0] = (uint32_t)seed;
+  s->key[1] = (uint32_t)(seed >> 32);
+  s->ctr[0] = (uint32_t)history;
+  s->ctr[1] = (uint32_t)(history >> 32);
+  s->ctr[2] = sim;
+  s->ctr[3] = 0;
+  s->pos = 4;
+}
+
+static uint32_t stream_word(stream_t *s) {
+  if (s->pos == 4) {
+    ref_philox4x32_10(s->ctr, s->key, s->buf);
+    s->ctr[3]++;
+    s->pos = 0;
+  }
+  return s->buf[s->pos++];
+}
+
+static float uni(stream_t *s) { return (float)((stream_word(s) >> 8) | 1u) * 0x1p-24f; }
+
+/* words [0, n) of the stream of one history, for the RNG parity test */
+void ref_stream_words(uint64_t seed, uint64_t history, uint32_t sim, uint32_t *out, int n) {
+  stream_t s;
+  stream_open(&s, seed, history, sim);
+  for (int i = 0; i < n; ++i) out[i] = stream_word(&s);
+}
+
+/* ------------------------------------------------------------------ math */
+static uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+float ref_log(float x) {
+  uint32_t b = fbits(x);
+  int e = (int)(b >> 23) - 127;
+  uint32_t m = (b & 0x007FFFFFu) | 0x3F800000u;
+  if (m > 0x3FB504F3u) {
+    m -= 0x00800000u;
+    e += 1;
+  }
+  float f = bitsf(m) - 1.0f;
+  float s = f / (2.0f + f);
+  float z = s * s;
+  float p = fmaf(z, 0x1.c71c72p-3f, 0x1.24924ap-2f);
+  p = fmaf(z, p, 0x1.99999ap-2f);
+  p = fmaf(z, p, 0x1.555556p-1f);
+  p = p * z;
+  float l1 = fmaf(s, p, s + s);
+  float ef = (float)e;
+  return fmaf(ef, 0x1.62ep-1f, fmaf(ef, 0x1.0bfbe8p-15f, l1));
+}
+
+void ref_sincos(float x, float *sn, float *cs) {
+  float k = rintf(x * 0x1.45f306p-1f);
+  int q = (int)k;
+  float r = fmaf(-k, 0x1.921fp+0f, x);
+  r = fmaf(-k, 0x1.6a88p-17f, r);
+  r = fmaf(-k, 0x1.0b4612p-34f, r);
+  float r2 = r * r;
+  float t = fmaf(r2, 0x1.71de3ap-19f, -0x1.a01a02p-13f);
+  t = fmaf(r2, t, 0x1.111112p-7f);
+  t = fmaf(r2, t, -0x1.555556p-3f);
+  float sp = fmaf(r * r2, t, r);
+  float c = fmaf(r2, -0x1.27e4fcp-22f, 0x1.a01a02p-16f);
+  c = fmaf(r2, c, -0x1.6c16c2p-10f);
+  c = fmaf(r2, c, 0x1.555556p-5f);
+  float cp = fmaf(r2 * r2, c, fmaf(-0.5f, r2, 1.0f));
+  switch (q & 3) {
+    case 0: *sn = sp; *cs = cp; break;
+    case 1: *sn = cp; *cs = -sp; break;
+    case 2: *sn = -sp; *cs = -cp; break;
+    default: *sn = -cp; *cs = sp; break;
+  }
+}
+
+static void unit(float v[3]) {
+  float n2 = fmaf(v[2], v[2], fmaf(v[1], v[1], v[0] * v[0]));
+  float inv = 1.0f / sqrtf(n2);
+  v[0] = v[0] * inv;
+  v[1] = v[1] * inv;
+  v[2] = v[2] * inv;
+}
+
+#define CAP 1000
+#define MEC2 0x1.05a1a8p-1f
+#define KPM 0x1.fac5e8p+8f
+
+static void disc_azimuth(stream_t *s, float *cp, float *sp) {
+  float a, b, r2;
+  int tries = 0;
+  for (;;) {
+    a = fmaf(2.0f, uni(s), -1.0f);
+    b = fmaf(2.0f, uni(s), -1.0f);
+    r2 = fmaf(b, b, a * a);
+    ++tries;
+    if (!(r2 > 1.0f || r2 == 0.0f) || tries >= CAP) break;
+  }
+  float inv = 1.0f / r2;
+  *cp = fmaf(a, a, -(b * b)) * inv;
+  *sp = ((a + a) * b) * inv;
+}
+
+static void scatter_dir(float d[3], float c, float cp, float sp) {
+  float st = sqrtf(fmaxf(fmaf(-c, c, 1.0f), 0.0f));
+  float n[3];
+  if (fabsf(d[2]) < 0.99999f) {
+    float tmp = sqrtf(fmaf(-d[2], d[2], 1.0f));
+    float f = st / tmp;
+    n[0] = fmaf(c, d[0], f * fmaf(d[0] * d[2], cp, -(d[1] * sp)));
+    n[1] = fmaf(c, d[1], f * fmaf(d[1] * d[2], cp, d[0] * sp));
+    n[2] = fmaf(c, d[2], -((st * cp) * tmp));
+  } else {
+    n[0] = st * cp;
+    n[1] = st * sp;
+    n[2] = (d[2] > 0.0f) ? c : -c;
+  }
+  unit(n);
+  d[0] = n[0]; d[1] = n[1]; d[2] = n[2];
+}
+
+/* last i in [0, n-2] with a[i] <= x */
+static int floor_index(const float *a, int n, float x) {
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+/* first k with cdf[k] > u */
+static int first_above(const float *cdf, int n, float u) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+static float grid_value(const float *xs, const float *ys, int n, float x) {
+  int i = floor_index(xs, n, x);
+  float f = (x - xs[i]) / (xs[i + 1] - xs[i]);
+  f = fminf(fmaxf(f, 0.0f), 1.0f);
+  return fmaf(f, ys[i + 1] - ys[i], ys[i]);
+}
+
+/* ------------------------------------------------------------------ physics view */
+typedef struct {
+  const gh_physics_desc *d;
+  int nb1;
+  float inv_rho[256];
+} phys_t;
+
+typedef struct {
+  int bucket;
+  float frac, maj, inv_maj;
+} eidx_t;
+
+static void energy_index(const phys_t *P, float E, eidx_t *x) {
+  uint32_t b = fbits(E);
+  x->bucket = (int)(b >> 17) - 7488;
+  x->frac = (float)(b & 0x1FFFFu) * 0x1p-17f;
+  const float *m = P->d->mu_majorant;
+  x->maj = fmaf(x->frac, m[x->bucket + 1] - m[x->bucket], m[x->bucket]);
+  x->inv_maj = 1.0f / x->maj;
+}
+
+static float node_interp(const float *row, const eidx_t *x) {
+  return fmaf(x->frac, row[x->bucket + 1] - row[x->bucket], row[x->bucket]);
+}
+
+/* Kahn's method for Klein-Nishina, then acceptance with S(q)/Z */
+static void compton_sample(stream_t *s, const phys_t *P, int m, float E, float *q_out, float *c_out,
+                           uint64_t *rej) {
+  const gh_physics_desc *d = P->d;
+  float alpha = E / MEC2;
+  float a2 = alpha + alpha;
+  float onep = 1.0f + a2;
+  float thr = onep / (9.0f + a2);
+  float kk = E * KPM;
+  float two_k2 = (2.0f * kk) * kk;
+  const float *srow = d->incoh_s + (size_t)m * d->n_x_nodes;
+  float q = 1.0f, c = 1.0f;
+  int it;
+  for (it = 0; it < CAP; ++it) {
+    float r1 = uni(s), r2 = uni(s), r3 = uni(s);
+    int ok;
+    if (r1 <= thr) {
+      q = fmaf(a2, r2, 1.0f);
+      float iq = 1.0f / q;
+      ok = r3 <= 4.0f * (iq - iq * iq);
+    } else {
+      q = onep / fmaf(a2, r2, 1.0f);
+      float cc = 1.0f - (q - 1.0f) / alpha;
+      ok = r3 <= 0.5f * fmaf(cc, cc, 1.0f / q);
+    }
+    if (!ok) { (*rej)++; continue; }
+    c = 1.0f - (q - 1.0f) / alpha;
+    float x = two_k2 * (1.0f - c);
+    float sv = grid_value(d->ray_x, srow, d->n_x_nodes, x);
+    if (uni(s) <= sv) break;
+    (*rej)++;
+  }
+  if (it == CAP) { q = 1.0f; c = 1.0f; }
+  *q_out = q;
+  *c_out = fmaxf(c, -1.0f);
+}
+
+static float rayleigh_sample(stream_t *s, const phys_t *P, int m, float E) {
+  const gh_physics_desc *d = P->d;
+  float kk = E * KPM;
+  float two_k2 = (2.0f * kk) * kk;
+  float xmax = two_k2 + two_k2;
+  const float *G = d->ray_cdf + (size_t)m * d->n_x_nodes;
+  const float *X = d->ray_x;
+  float gmax = grid_value(X, G, d->n_x_nodes, xmax);
+  float c = 1.0f;
+  for (int it = 0; it < CAP; ++it) {
+    float t = uni(s) * gmax;
+    int j = floor_index(G, d->n_x_nodes, t);
+    float dg = G[j + 1] - G[j];
+    float x = (dg > 0.0f) ? fmaf((t - G[j]) / dg, X[j + 1] - X[j], X[j]) : X[j];
+    c = fmaxf(1.0f - x / two_k2, -1.0f);
+    if (uni(s) <= 0.5f * fmaf(c, c, 1.0f)) break;
+  }
+  return c;
+}
+
+/* ------------------------------------------------------------------ tally */
+typedef struct {
+  double *edep, *edep2, *dose, *dose2, *counts;
+  double *exact_edep2;   /* exact per-history squared sums, may be NULL */
+  /* per-history scratch for the exact squared tally */
+  int *hv;
+  double *hs;
+  int hn, hcap;
+} tally_t;
+
+typedef struct {
+  int v[4];
+  float s[4], dsum[4];
+  uint32_t c[4];
+  int n;
+} slots_t;
+
+typedef struct {
+  uint64_t c[12];
+  double e[4];
+} stats_t;
+
+enum { K_HIST = 0, K_STEPS, K_DEP, K_COMPT, K_RAYL, K_PHOTO, K_ESC, K_WORLD, K_CUT, K_EVICT, K_REC, K_REJ };
+enum { EN_DEP = 0, EN_OUT, EN_ESC, EN_WORLD };
+
+static void write_slot(tally_t *t, int v, float s, float ds, uint32_t c, stats_t *st) {
+  t->edep[v] += (double)s;
+  if (t->edep2) t->edep2[v] += (double)s * (double)s;
+  if (t->counts) t->counts[v] += (double)c;
+  if (t->dose) t->dose[v] += (double)ds;
+  if (t->dose2) t->dose2[v] += (double)ds * (double)ds;
+  st->c[K_DEP]++;
+  st->e[EN_DEP] += (double)s;
+}
+
+static void exact_add(tally_t *t, int v, float x) {
+  if (!t->exact_edep2) return;
+  for (int i = 0; i < t->hn; ++i)
+    if (t->hv[i] == v) { t->hs[i] += (double)x; return; }
+  if (t->hn == t->hcap) {
+    t->hcap = t->hcap ? 2 * t->hcap : 16;
+    t->hv = (int *)realloc(t->hv, t->hcap * sizeof(int));
+    t->hs = (double *)realloc(t->hs, t->hcap * sizeof(double));
+  }
+  t->hv[t->hn] = v;
+  t->hs[t->hn] = (double)x;
+  t->hn++;
+}
+
+static void history_end(tally_t *t, slots_t *sl, stats_t *st) {
+  for (int k = 0; k < sl->n; ++k) write_slot(t, sl->v[k], sl->s[k], sl->dsum[k], sl->c[k], st);
+  sl->n = 0;
+  if (t->exact_edep2) {
+    for (int i = 0; i < t->hn; ++i) t->exact_edep2[t->hv[i]] += t->hs[i] * t->hs[i];
+    t->hn = 0;
+  }
+}
+
+static void score(tally_t *t, slots_t *sl, int v, float x, float dx, stats_t *st) {
+  if (v < 0) { st->e[EN_OUT] += (double)x; return; }
+  exact_add(t, v, x);
+  for (int k = 0; k < sl->n; ++k) {
+    if (sl->v[k] == v) {
+      sl->s[k] = sl->s[k] + x;
+      sl->dsum[k] = sl->dsum[k] + dx;
+      sl->c[k] += 1u;
+      return;
+    }
+  }
+  if (sl->n == 4) {
+    write_slot(t, sl->v[0], sl->s[0], sl->dsum[0], sl->c[0], st);
+    for (int k = 0; k < 3; ++k) {
+      sl->v[k] = sl->v[k + 1]; sl->s[k] = sl->s[k + 1]; sl->dsum[k] = sl->dsum[k + 1]; sl->c[k] = sl->c[k + 1];
+    }
+    sl->n = 3;
+    st->c[K_EVICT]++;
+  }
+  sl->v[sl->n] = v; sl->s[sl->n] = x; sl->dsum[sl->n] = dx; sl->c[sl->n] = 1u;
+  sl->n++;
+}
+
+/* ------------------------------------------------------------------ geometry */
+typedef struct {
+  const gh_phantom_desc *ph;
+  const gh_source_desc *src;
+  const gh_tally_desc *tal;
+  float inv_s[3], L[3], tinv[3], d_ct, d_phi;
+  int aligned;
+} sim_t;
+
+static int voxel_of(const sim_t *S, const float p[3]) {
+  int ix = (int)floorf(p[0] * S->inv_s[0]);
+  int iy = (int)floorf(p[1] * S->inv_s[1]);
+  int iz = (int)floorf(p[2] * S->inv_s[2]);
+  if ((unsigned)ix >= (unsigned)S->ph->nx || (unsigned)iy >= (unsigned)S->ph->ny || (unsigned)iz >= (unsigned)S->ph->nz)
+    return -1;
+  return ix + S->ph->nx * (iy + S->ph->ny * iz);
+}
+
+static int tally_index(const sim_t *S, int vox, const float p[3]) {
+  if (S->aligned) return vox;
+  const gh_tally_desc *t = S->tal;
+  int ix = (int)floorf((p[0] - t->corner[0]) * S->tinv[0]);
+  int iy = (int)floorf((p[1] - t->corner[1]) * S->tinv[1]);
+  int iz = (int)floorf((p[2] - t->corner[2]) * S->tinv[2]);
+  if ((unsigned)ix >= (unsigned)t->nx || (unsigned)iy >= (unsigned)t->ny || (unsigned)iz >= (unsigned)t->nz) return -1;
+  return ix + t->nx * (iy + t->ny * iz);
+}
+
+static void box_hit(const sim_t *S, const float p[3], const float d[3], float *tnear, float *tfar, int *face) {
+  *tnear = -INFINITY;
+  *tfar = INFINITY;
+  *face = -1;
+  for (int i = 0; i < 3; ++i) {
+    if (d[i] != 0.0f) {
+      float inv = 1.0f / d[i];
+      float t0 = (-p[i]) * inv;
+      float t1 = (S->L[i] - p[i]) * inv;
+      float lo = fminf(t0, t1), hi = fmaxf(t0, t1);
+      if (lo > *tnear) *tnear = lo;
+      if (hi < *tfar) { *tfar = hi; *face = 2 * i + (d[i] > 0.0f ? 1 : 0); }
+    } else if (p[i] < 0.0f || p[i] > S->L[i]) {
+      *tnear = INFINITY;
+    }
+  }
+}
+
+static float world_distance(const sim_t *S, const float p[3], const float d[3]) {
+  float t = INFINITY;
+  for (int i = 0; i < 3; ++i) {
+    if (d[i] > 0.0f) t = fminf(t, (S->ph->world_hi[i] - p[i]) / d[i]);
+    else if (d[i] < 0.0f) t = fminf(t, (S->ph->world_lo[i] - p[i]) / d[i]);
+  }
+  return fmaxf(t, 0.0f);
+}
+
+/* ------------------------------------------------------------------ source */
+typedef struct {
+  float p[3], d[3], E, w;
+} photon_t;
+
+static void emit(const sim_t *S, stream_t *rs, uint64_t h, photon_t *ph) {
+  const gh_source_desc *src = S->src;
+  float lp[3] = {0.0f, 0.0f, 0.0f}, ld[3] = {0.0f, 0.0f, 1.0f};
+  float E = src->energy_mono, w = src->weight;
+  if (src->pos_type == GH_POS_PHSP) {
+    int64_t row = (src->phsp_start + (int64_t)h) % src->phsp_n;
+    for (int k = 0; k < 3; ++k) {
+      lp[k] = src->phsp[k][row];
+      ld[k] = src->phsp[3 + k][row];
+    }
+    E = src->phsp[6][row];
+    w = src->phsp[7][row];
+  } else {
+    if (src->pos_type == GH_POS_BOX) {
+      for (int k = 0; k < 3; ++k) lp[k] = (uni(rs) - 0.5f) * src->pos_param[k];
+    } else if (src->pos_type == GH_POS_SPHERE || src->pos_type == GH_POS_ELLIPSOID) {
+      float a, b, c, r2;
+      int tries = 0;
+      do {
+        a = fmaf(2.0f, uni(rs), -1.0f);
+        b = fmaf(2.0f, uni(rs), -1.0f);
+        c = fmaf(2.0f, uni(rs), -1.0f);
+        r2 = fmaf(c, c, fmaf(b, b, a * a));
+      } while (r2 > 1.0f && ++tries < CAP);
+      int ell = src->pos_type == GH_POS_ELLIPSOID;
+      lp[0] = src->pos_param[0] * a;
+      lp[1] = src->pos_param[ell ? 1 : 0] * b;
+      lp[2] = src->pos_param[ell ? 2 : 0] * c;
+    } else if (src->pos_type == GH_POS_DISC || src->pos_type == GH_POS_CYLINDER) {
+      float a, b, r2;
+      int tries = 0;
+      do {
+        a = fmaf(2.0f, uni(rs), -1.0f);
+        b = fmaf(2.0f, uni(rs), -1.0f);
+        r2 = fmaf(b, b, a * a);
+      } while (r2 > 1.0f && ++tries < CAP);
+      lp[0] = src->pos_param[0] * a;
+      lp[1] = src->pos_param[0] * b;
+      if (src->pos_type == GH_POS_CYLINDER) lp[2] = fmaf(2.0f, uni(rs), -1.0f) * src->pos_param[1];
+    }
+    if (src->dir_type == GH_DIR_ISO) {
+      float ct = fmaf(-uni(rs), S->d_ct, src->cos_theta_min);
+      float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+      float cp, sp;
+      if (src->phi_full) {
+        disc_azimuth(rs, &cp, &sp);
+      } else {
+        float phi = fmaf(uni(rs), S->d_phi, src->phi_min);
+        ref_sincos(phi, &sp, &cp);
+      }
+      ld[0] = -(st * cp);
+      ld[1] = -(st * sp);
+      ld[2] = -ct;
+    } else if (src->dir_type == GH_DIR_MOMENTUM) {
+      ld[0] = src->momentum[0]; ld[1] = src->momentum[1]; ld[2] = src->momentum[2];
+    }
+    if (src->ene_type == GH_ENE_LINES) {
+      E = src->energy_value[first_above(src->energy_cdf, src->n_energy, uni(rs))];
+    } else if (src->ene_type == GH_ENE_HISTOGRAM) {
+      int k = first_above(src->energy_cdf, src->n_energy, uni(rs));
+      E = fmaf(uni(rs), src->energy_width[k], src->energy_value[k]);
+    } else if (src->ene_type == GH_ENE_RANGE) {
+      E = fmaf(uni(rs), src->energy_max - src->energy_mono, src->energy_mono);
+    }
+  }
+  const float *M = src->pos_mat;
+  for (int i = 0; i < 3; ++i) ph->p[i] = fmaf(M[3 * i], lp[0], fmaf(M[3 * i + 1], lp[1], fmaf(M[3 * i + 2], lp[2], M[9 + i])));
+  if (src->pos_type == GH_POS_PHSP && src->phsp_iso) {
+    float ct = fmaf(2.0f, uni(rs), -1.0f);
+    float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+    float cp, sp;
+    disc_azimuth(rs, &cp, &sp);
+    ph->d[0] = st * cp; ph->d[1] = st * sp; ph->d[2] = ct;
+  } else if (src->dir_type == GH_DIR_FOCUSED && src->pos_type != GH_POS_PHSP) {
+    for (int i = 0; i < 3; ++i) ph->d[i] = src->focus[i] - ph->p[i];
+    unit(ph->d);
+  } else {
+    const float *D = src->dir_mat;
+    for (int i = 0; i < 3; ++i) ph->d[i] = fmaf(D[3 * i], ld[0], fmaf(D[3 * i + 1], ld[1], D[3 * i + 2] * ld[2]));
+    unit(ph->d);
+  }
+  ph->E = E;
+  ph->w = w;
+}
+
+/* ------------------------------------------------------------------ one history */
+typedef struct {
+  float *rec[8];
+  uint64_t rec_cap, rec_n;
+  float *event_edep;
+} outputs_t;
+
+static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t first, uint32_t sim_id, uint64_t h,
+                        tally_t *t, stats_t *st, outputs_t *o) {
+  const gh_physics_desc *d = P->d;
+  stream_t rs;
+  stream_open(&rs, seed, first + h, sim_id);
+  photon_t ph;
+  emit(S, &rs, h, &ph);
+  st->c[K_HIST]++;
+  slots_t sl;
+  sl.n = 0;
+  t->hn = 0;
+  float ev = 0.0f;
+  const int det = S->tal->detector_label_min;
+  int in_box = voxel_of(S, ph.p) >= 0;
+  if (ph.E < d->e_cut) {
+    if (in_box) {
+      int vox = voxel_of(S, ph.p);
+      int lab = S->ph->labels[vox];
+      float x = ph.E * ph.w;
+      score(t, &sl, tally_index(S, vox, ph.p), x, x * P->inv_rho[lab], st);
+      if (det >= 0 && lab >= det) ev += x;
+      history_end(t, &sl, st);
+    } else {
+      st->e[EN_WORLD] += (double)(ph.E * ph.w);
+    }
+    st->c[K_CUT]++;
+    if (o->event_edep) o->event_edep[h] = ev;
+    return;
+  }
+  eidx_t ex;
+  energy_index(P, ph.E, &ex);
+  const int wm = S->ph->world_material;
+  for (;;) {
+    if (in_box) {
+      float step = -ref_log(uni(&rs)) * ex.inv_maj;
+      float q[3];
+      for (int i = 0; i < 3; ++i) q[i] = fmaf(step, ph.d[i], ph.p[i]);
+      int vox = voxel_of(S, q);
+      if (vox >= 0) {
+        ph.p[0] = q[0]; ph.p[1] = q[1]; ph.p[2] = q[2];
+        st->c[K_STEPS]++;
+        int lab = S->ph->labels[vox];
+        float mu = node_interp(d->mu_total + (size_t)lab * P->nb1, &ex);
+        if (!(uni(&rs) * ex.maj < mu)) continue;
+        float r = uni(&rs);
+        float pc = node_interp(d->p_compton + (size_t)lab * P->nb1, &ex);
+        float pr = node_interp(d->p_rayleigh + (size_t)lab * P->nb1, &ex);
+        int tv = tally_index(S, vox, ph.p);
+        float irho = P->inv_rho[lab];
+        int is_det = det >= 0 && lab >= det;
+        if (r < pc) {
+          st->c[K_COMPT]++;
+          float qq, c;
+          compton_sample(&rs, P, lab, ph.E, &qq, &c, &st->c[K_REJ]);
+          float e1 = ph.E / qq;
+          float x = (ph.E - e1) * ph.w;
+          score(t, &sl, tv, x, x * irho, st);
+          if (is_det) ev += x;
+          float cp, sp;
+          disc_azimuth(&rs, &cp, &sp);
+          scatter_dir(ph.d, c, cp, sp);
+          ph.E = e1;
+          if (ph.E < d->e_cut) {
+            float x2 = ph.E * ph.w;
+            score(t, &sl, tv, x2, x2 * irho, st);
+            if (is_det) ev += x2;
+            st->c[K_CUT]++;
+            break;
+          }
+          energy_index(P, ph.E, &ex);
+        } else if (r < pr) {
+          st->c[K_RAYL]++;
+          float c = rayleigh_sample(&rs, P, lab, ph.E);
+          float cp, sp;
+          disc_azimuth(&rs, &cp, &sp);
+          scatter_dir(ph.d, c, cp, sp);
+        } else {
+          st->c[K_PHOTO]++;
+          float x = ph.E * ph.w;
+          score(t, &sl, tv, x, x * irho, st);
+          if (is_det) ev += x;
+          break;
+        }
+      } else {
+        float tn, tf;
+        int face;
+        box_hit(S, ph.p, ph.d, &tn, &tf, &face);
+        float tt = fmaxf(tf, 0.0f);
+        float e[3];
+        for (int i = 0; i < 3; ++i) e[i] = fmaf(tt, ph.d[i], ph.p[i]);
+        if (S->tal->record_face >= 0 && face == S->tal->record_face) {
+          if (o->rec[0] && o->rec_n < o->rec_cap) {
+            uint64_t k = o->rec_n;
+            for (int i = 0; i < 3; ++i) { o->rec[i][k] = e[i]; o->rec[3 + i][k] = ph.d[i]; }
+            o->rec[6][k] = ph.E;
+            o->rec[7][k] = ph.w;
+          }
+          o->rec_n++;
+          st->c[K_REC]++;
+          st->e[EN_ESC] += (double)(ph.E * ph.w);
+          break;
+        }
+        if (wm < 0) {
+          st->c[K_ESC]++;
+          st->e[EN_ESC] += (double)(ph.E * ph.w);
+          break;
+        }
+        ph.p[0] = e[0]; ph.p[1] = e[1]; ph.p[2] = e[2];
+        in_box = 0;
+      }
+    } else {
+      float tn, tf;
+      int face;
+      box_hit(S, ph.p, ph.d, &tn, &tf, &face);
+      int hit = (tn < tf) && (tf > 1e-3f);
+      float tin = fmaxf(tn, 0.0f);
+      if (wm < 0) {
+        if (!hit) {
+          st->c[K_ESC]++;
+          st->e[EN_ESC] += (double)(ph.E * ph.w);
+          break;
+        }
+        for (int i = 0; i < 3; ++i) ph.p[i] = fmaf(tin, ph.d[i], ph.p[i]);
+        in_box = 1;
+        continue;
+      }
+      float muw = node_interp(d->mu_total + (size_t)wm * P->nb1, &ex);
+      float tw = world_distance(S, ph.p, ph.d);
+      float dist = (muw > 0.0f) ? (-ref_log(uni(&rs)) / muw) : INFINITY;
+      if (hit && dist >= tin && tin < tw) {
+        for (int i = 0; i < 3; ++i) ph.p[i] = fmaf(tin, ph.d[i], ph.p[i]);
+        in_box = 1;
+        continue;
+      }
+      if (dist >= tw) {
+        st->c[K_ESC]++;
+        st->e[EN_ESC] += (double)(ph.E * ph.w);
+        break;
+      }
+      for (int i = 0; i < 3; ++i) ph.p[i] = fmaf(dist, ph.d[i], ph.p[i]);
+      st->c[K_WORLD]++;
+      float r = uni(&rs);
+      float pc = node_interp(d->p_compton + (size_t)wm * P->nb1, &ex);
+      float pr = node_interp(d->p_rayleigh + (size_t)wm * P->nb1, &ex);
+      if (r < pc) {
+        float qq, c;
+        compton_sample(&rs, P, wm, ph.E, &qq, &c, &st->c[K_REJ]);
+        float e1 = ph.E / qq;
+        st->e[EN_WORLD] += (double)((ph.E - e1) * ph.w);
+        float cp, sp;
+        disc_azimuth(&rs, &cp, &sp);
+        scatter_dir(ph.d, c, cp, sp);
+        ph.E = e1;
+        if (ph.E < d->e_cut) {
+          st->e[EN_WORLD] += (double)(ph.E * ph.w);
+          st->c[K_CUT]++;
+          break;
+        }
+        energy_index(P, ph.E, &ex);
+      } else if (r < pr) {
+        float c = rayleigh_sample(&rs, P, wm, ph.E);
+        float cp, sp;
+        disc_azimuth(&rs, &cp, &sp);
+        scatter_dir(ph.d, c, cp, sp);
+      } else {
+        st->e[EN_WORLD] += (double)(ph.E * ph.w);
+        break;
+      }
+    }
+  }
+  history_end(t, &sl, st);
+  if (o->event_edep) o->event_edep[h] = ev;
+}
+
+/* ------------------------------------------------------------------ driver */
+typedef struct {
+  const sim_t *S;
+  const phys_t *P;
+  uint64_t seed, first, h0, h1;
+  uint32_t sim_id;
+  size_t ntal;
+  int flags, want_exact;
+  tally_t t;
+  stats_t st;
+  outputs_t o;
+} work_t;
+
+static void *worker(void *arg) {
+  work_t *w = (work_t *)arg;
+  for (uint64_t h = w->h0; h < w->h1; ++h) run_history(w->S, w->P, w->seed, w->first, w->sim_id, h, &w->t, &w->st, &w->o);
+  return NULL;
+}
+
+/* Runs histories [0, n) of one simulation (sim_id keys the streams) on n_threads
+ * threads and adds into the caller's tallies (edep required; others may be NULL).
+ * Recording of exit-face photons and per-history detector energy need n_threads = 1.
+ * counters[12] and energies[4] are added to. Returns 0 on success. */
+int ref_run(const gh_physics_desc *phys, const gh_phantom_desc *ph, const gh_source_desc *src,
+            const gh_tally_desc *tal, uint32_t sim_id, uint64_t n, uint64_t first, uint64_t seed, int n_threads,
+            double *edep, double *edep2, double *dose, double *dose2, double *counts, double *exact_edep2,
+            float *const rec[8], uint64_t rec_cap, uint64_t *rec_n, float *event_edep, uint64_t counters[12],
+            double energies[4]) {
+  if (!phys || !ph || !src || !tal || !edep || n_threads < 1) return 1;
+  if ((rec && rec[0]) || event_edep) n_threads = 1;
+  phys_t P;
+  P.d = phys;
+  P.nb1 = phys->n_buckets + 1;
+  for (int i = 0; i < phys->n_materials && i < 256; ++i) P.inv_rho[i] = phys->density[i] > 0.0f ? 1.0f / phys->density[i] : 0.0f;
+  sim_t S;
+  S.ph = ph;
+  S.src = src;
+  S.tal = tal;
+  int nn[3] = {ph->nx, ph->ny, ph->nz};
+  for (int i = 0; i < 3; ++i) {
+    S.inv_s[i] = 1.0f / ph->spacing[i];
+    S.L[i] = (float)((double)nn[i] * (double)ph->spacing[i]);
+    S.tinv[i] = 1.0f / tal->spacing[i];
+  }
+  S.d_ct = src->cos_theta_min - src->cos_theta_max;
+  S.d_phi = src->phi_max - src->phi_min;
+  S.aligned = (tal->flags & GH_TALLY_ALIGNED) != 0;
+  size_t ntal = (size_t)tal->nx * tal->ny * tal->nz;
+  work_t *W = (work_t *)calloc((size_t)n_threads, sizeof(work_t));
+  pthread_t *T = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));
+  for (int k = 0; k < n_threads; ++k) {
+    work_t *w = &W[k];
+    w->S = &S;
+    w->P = &P;
+    w->seed = seed;
+    w->first = first;
+    w->sim_id = sim_id;
+    w->h0 = n * (uint64_t)k / (uint64_t)n_threads;
+    w->h1 = n * (uint64_t)(k + 1) / (uint64_t)n_threads;
+    if (n_threads == 1) {
+      w->t.edep = edep; w->t.edep2 = edep2; w->t.dose = dose; w->t.dose2 = dose2; w->t.counts = counts;
+      w->t.exact_edep2 = exact_edep2;
+      if (rec) for (int i = 0; i < 8; ++i) w->o.rec[i] = rec[i];
+      w->o.rec_cap = rec_cap;
+      w->o.event_edep = event_edep;
+    } else {
+      w->t.edep = (double *)calloc(ntal, sizeof(double));
+      w->t.edep2 = edep2 ? (double *)calloc(ntal, sizeof(double)) : NULL;
+      w->t.dose = dose ? (double *)calloc(ntal, sizeof(double)) : NULL;
+      w->t.dose2 = dose2 ? (double *)calloc(ntal, sizeof(double)) : NULL;
+      w->t.counts = counts ? (double *)calloc(ntal, sizeof(double)) : NULL;
+      w->t.exact_edep2 = exact_edep2 ? (double *)calloc(ntal, sizeof(double)) : NULL;
+    }
+  }
+  if (n_threads == 1) {
+    worker(&W[0]);
+  } else {
+    for (int k = 0; k < n_threads; ++k) pthread_create(&T[k], NULL, worker, &W[k]);
+    for (int k = 0; k < n_threads; ++k) pthread_join(T[k], NULL);
+  }
+  for (int k = 0; k < n_threads; ++k) {
+    work_t *w = &W[k];
+    if (n_threads > 1) {
+      double *dst[6] = {edep, edep2, dose, dose2, counts, exact_edep2};
+      double *srcs[6] = {w->t.edep, w->t.edep2, w->t.dose, w->t.dose2, w->t.counts, w->t.exact_edep2};
+      for (int a = 0; a < 6; ++a) {
+        if (!dst[a] || !srcs[a]) continue;
+        for (size_t i = 0; i < ntal; ++i) dst[a][i] += srcs[a][i];
+        free(srcs[a]);
+      }
+    }
+    free(w->t.hv);
+    free(w->t.hs);
+    if (counters) for (int i = 0; i < 12; ++i) counters[i] += w->st.c[i];
+    if (energies) for (int i = 0; i < 4; ++i) energies[i] += w->st.e[i];
+    if (rec_n) *rec_n += w->o.rec_n;
+  }
+  free(W);
+  free(T);
+  return 0;
+}
+
+/* HU -> labels: out[i] = lut[#{k : bins[k] <= hu[i]}] (numpy.digitize, right=False) */
+void ref_hu_to_labels(const int16_t *hu, int64_t n, const double *bins, int n_bins, const uint16_t *lut, uint16_t *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    int k = 0;
+    while (k < n_bins && bins[k] <= (double)hu[i]) ++k;
+    out[i] = lut[k];
+  }
+}

@@ -1,0 +1,181 @@
+"""GPU transport vs the CPU oracle on identical inputs (through the C-ABI).
+
+Every history follows the same FP32 path on both sides, so all integer counters
+(histories, Woodcock steps, deposits, interactions by process, escapes, ...) must be
+bit-exact, and the FP64 tallies must agree up to the order of FP64 additions.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from opengate_amd.transport import PhantomSpec, SourceSpec, TallySpec, spectrum_histogram, spectrum_lines
+
+from helpers import aligned_tally, centre, small_phantom, tables
+
+pytestmark = pytest.mark.gpu
+
+INT_KEYS = ["histories", "steps", "deposits", "compton", "rayleigh", "photo", "escaped", "world_interactions",
+            "cut_kills", "sq_evictions", "recorded", "kn_rejections"]
+E_KEYS = ["e_deposited", "e_outside_tally", "e_escaped", "e_world"]
+
+
+def compare(gpu_c, ref_c, gpu_t, ref_t, tol=1e-9):
+    for k in INT_KEYS:
+        assert gpu_c[k] == ref_c[k], (k, gpu_c[k], ref_c[k])
+    for k in E_KEYS:
+        assert abs(gpu_c[k] - ref_c[k]) <= tol * max(1.0, abs(ref_c[k])), (k, gpu_c[k], ref_c[k])
+    for k, ref in ref_t.items():
+        if k in ("exact_edep2", "phsp", "event_edep"):
+            continue
+        g = gpu_t[k]
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.max(np.abs(g - ref)) <= tol * scale, (k, np.max(np.abs(g - ref)), scale)
+
+
+def run_both(engine, T, ph, src, tal, n, seed, **launch):
+    engine.clear()
+    engine.set_physics(T)
+    sid = engine.add_simulation(ph, src, tal)
+    print(f"[gpu case] n={n} seed={seed} src={src.pos_type}/{src.dir_type}/{src.ene_type} "
+          f"phantom={ph.shape_xyz} launch={launch}", flush=True)
+    c = engine.run([n], seed, **launch)
+    g = engine.get_tally(sid)
+    r, rc = O.run(T, ph, src, tal, n, seed)
+    return c, g, rc, r
+
+
+def test_point_source_water_bone(engine):
+    T = tables()
+    ph = small_phantom(32, 2.0)
+    src = SourceSpec(pos_trans=centre(ph), energy=0.1405)
+    tal = aligned_tally(ph, counts=True)
+    c, g, rc, r = run_both(engine, T, ph, src, tal, 200_000, 1234)
+    compare(c, rc, g, r)
+    assert c["histories"] == 200_000
+
+
+def test_box_source_lines_and_launch_shapes(engine):
+    T = tables(e_max=0.21)
+    ph = small_phantom(24, 3.0)
+    ev, ew, cdf = spectrum_lines([0.11295, 0.20837], [0.062, 0.1038])
+    src = SourceSpec(pos_type="box", pos_param=(20.0, 30.0, 10.0), pos_trans=centre(ph), ene_type="spectrum_discrete",
+                     energy_value=ev, energy_cdf=cdf)
+    tal = aligned_tally(ph)
+    for launch in ({"block": 256, "refill_interval": 16, "chunk": 4096},
+                   {"block": 64, "grid": 7, "refill_interval": 1, "chunk": 100},
+                   {"block": 128, "grid": 301, "refill_interval": 64, "chunk": 333}):
+        c, g, rc, r = run_both(engine, T, ph, src, tal, 150_000, 77, **launch)
+        compare(c, rc, g, r)
+
+
+def test_cone_from_outside_air_world_histogram_nonaligned(engine):
+    T = tables(("G4_WATER", "G4_BONE_CORTICAL_ICRP", "G4_AIR"), 0.125)
+    ph = small_phantom(32, 2.0)
+    ph.world_material = 2
+    ph.world_lo = (-400.0, -400.0, -400.0)
+    ph.world_hi = (464.0, 464.0, 464.0)
+    e = np.arange(10, 121) * 1e-3
+    mid = 0.5 * (e[1:] + e[:-1])
+    w = np.clip(0.120 - mid, 0, None) / mid
+    ev, ew, cdf = spectrum_histogram(e, w)
+    cx, cy, cz = centre(ph)
+    # cone of half-angle 15 deg from 300 mm above, pointing to -z (G4 iso convention)
+    src = SourceSpec(pos_trans=(cx, cy, cz + 300.0), theta=(0.0, np.deg2rad(15.0)), ene_type="spectrum_histogram",
+                     energy_value=ev, energy_width=ew, energy_cdf=cdf)
+    tal = TallySpec(size=(20, 20, 20), spacing=(2.5, 2.5, 2.5), corner=(7.0, 7.0, 7.0), aligned=False,
+                    dose=True, dose2=True, counts=True)
+    c, g, rc, r = run_both(engine, T, ph, src, tal, 100_000, 99)
+    compare(c, rc, g, r)
+    assert c["world_interactions"] > 0
+
+
+def test_restricted_phi_and_sphere_source(engine):
+    T = tables()
+    ph = small_phantom(20, 2.5)
+    src = SourceSpec(pos_type="sphere", pos_param=(8.0,), pos_trans=centre(ph), theta=(0.3, 2.0),
+                     phi=(0.2, 4.0), energy=0.1)
+    tal = aligned_tally(ph)
+    c, g, rc, r = run_both(engine, T, ph, src, tal, 100_000, 5)
+    compare(c, rc, g, r)
+
+
+def test_phase_space_source_iso_and_file_directions(engine):
+    T = tables()
+    ph = small_phantom(24, 2.0)
+    rng = np.random.default_rng(3)
+    n = 5000
+    cx, cy, cz = centre(ph)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    arrs = [np.full(n, cx, np.float32) + rng.uniform(-5, 5, n).astype(np.float32),
+            np.full(n, cy, np.float32), np.full(n, cz, np.float32),
+            d[:, 0].astype(np.float32), d[:, 1].astype(np.float32), d[:, 2].astype(np.float32),
+            rng.uniform(0.02, 0.14, n).astype(np.float32), rng.uniform(0.5, 1.5, n).astype(np.float32)]
+    for iso in (False, True):
+        src = SourceSpec(pos_type="phsp", phsp=arrs, phsp_start=17, phsp_iso=iso)
+        tal = aligned_tally(ph)
+        c, g, rc, r = run_both(engine, T, ph, src, tal, 20_000, 11)
+        compare(c, rc, g, r)
+
+
+def test_batched_launch_equals_separate_runs(engine):
+    T = tables(e_max=0.21)
+    specs = []
+    for k, (n, sp) in enumerate([(16, 3.0), (24, 2.0), (20, 1.5)]):
+        ph = small_phantom(n, sp)
+        src = SourceSpec(pos_type="box", pos_param=(10.0, 10.0, 10.0), pos_trans=centre(ph),
+                         energy=[0.1405, 0.2, 0.06][k])
+        specs.append((ph, src, aligned_tally(ph)))
+    engine.clear()
+    engine.set_physics(T)
+    for s in specs:
+        engine.add_simulation(*s)
+    hist = [30_000, 50_000, 7_000]
+    print(f"[gpu case] batched 3 sims {hist}", flush=True)
+    c = engine.run(hist, 2024, first_history=[0, 1000, 5])
+    tot = {k: 0 for k in INT_KEYS}
+    for sid, ((ph, src, tal), n, first) in enumerate(zip(specs, hist, [0, 1000, 5])):
+        r, rc = O.run(T, ph, src, tal, n, 2024, first=first, sim_id=sid)
+        g = engine.get_tally(sid)
+        for k in ("edep", "edep2"):
+            assert np.max(np.abs(g[k] - r[k])) <= 1e-9 * r[k].max()
+        for k in INT_KEYS:
+            tot[k] += rc[k]
+    for k in INT_KEYS:
+        assert c[k] == tot[k], k
+
+
+def test_recorder_and_detector_event_energy(engine):
+    T = tables()
+    ph = small_phantom(20, 2.0)
+    src = SourceSpec(pos_trans=centre(ph), energy=0.1405)
+    tal = aligned_tally(ph, record_face=5, record_capacity=200_000, detector_label_min=1)
+    engine.clear()
+    engine.set_physics(T)
+    sid = engine.add_simulation(ph, src, tal)
+    n = 50_000
+    print("[gpu case] recorder + detector", flush=True)
+    c = engine.run([n], 8)
+    arrs, nrec = engine.get_phsp(sid)
+    ev = engine.get_event_edep(sid, n)
+    r, rc = O.run(T, ph, src, tal, n, 8, record=True, event=True)
+    assert nrec == rc["recorded_total"] == c["recorded"]
+    np.testing.assert_array_equal(ev, r["event_edep"])
+    # recorded rows: same multiset (GPU order depends on scheduling)
+    key = lambda a: np.lexsort([a[6], a[2], a[1], a[0]])
+    g = np.stack(arrs)[:, key(arrs)]
+    rr = np.stack([x[:nrec] for x in r["phsp"]])
+    rr = rr[:, key(rr)]
+    np.testing.assert_array_equal(g, rr)
+
+
+def test_hu_to_labels_bit_exact(engine):
+    rng = np.random.default_rng(0)
+    hu = rng.integers(-1100, 3100, size=(17, 33, 65)).astype(np.int16)
+    hu.ravel()[:4200] = np.arange(-1100, 3100)[:4200]
+    vm = [[-1050, -950, "G4_AIR"], [-950, -120, "G4_LUNG_ICRP"], [-120, 19, "G4_ADIPOSE_TISSUE_ICRP"],
+          [19, 80, "G4_WATER"], [120, 400.5, "G4_MUSCLE_SKELETAL_ICRP"], [400.5, 3000, "G4_BONE_CORTICAL_ICRP"]]
+    ref, lut = O.label_image(hu, "G4_AIR", vm)
+    bins, ls = O.label_bins(lut, vm)
+    out = engine.hu_to_labels(hu, bins, ls)
+    np.testing.assert_array_equal(out, ref)
