@@ -150,6 +150,11 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *phys);
 int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_desc *src,
                       const gh_tally_desc *tal, int32_t *sim_id);
 int gh_clear_simulations(gh_ctx *ctx);
+/* Replaces the source of a simulation (tallies are kept): several sources of one
+ * simulation are run one after the other into the same tally, as the per-event source
+ * selection of GateSourceManager::PrepareNextSource (GateSourceManager.cpp:313-386)
+ * sums them. */
+int gh_set_source(gh_ctx *ctx, int32_t sim_id, const gh_source_desc *src);
 int gh_reset_tally(gh_ctx *ctx, int32_t sim_id);
 
 /* Transports n_histories[s] histories of every simulation s in one persistent launch.

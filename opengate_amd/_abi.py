@@ -107,6 +107,7 @@ SIGNATURES = {
     "gh_add_simulation": (C.c_int, [C.c_void_p, C.POINTER(PhantomDesc), C.POINTER(SourceDesc),
                                     C.POINTER(TallyDesc), C.POINTER(C.c_int32)]),
     "gh_clear_simulations": (C.c_int, [C.c_void_p]),
+    "gh_set_source": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(SourceDesc)]),
     "gh_reset_tally": (C.c_int, [C.c_void_p, C.c_int32]),
     "gh_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint64,
                          C.POINTER(Launch), C.POINTER(Counters)]),

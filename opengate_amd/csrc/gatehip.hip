@@ -675,7 +675,8 @@ struct DevBuf {
 
 struct SimHost {
   SimDev dev;
-  std::vector<void *> owned;  // device allocations of this simulation
+  std::vector<void *> owned;         // device allocations of this simulation
+  std::vector<void *> source_owned;  // device allocations of its current source
   size_t n_tally = 0;
   int flags = 0;
   int det = -1;
@@ -728,6 +729,93 @@ void free_all(std::vector<void *> &v) {
 
 }  // namespace
 
+namespace {
+
+int validate_source(gh_ctx *ctx, const gh_source_desc *src) {
+  if (src->pos_type < GH_POS_POINT || src->pos_type > GH_POS_PHSP) FAIL(GH_EINVAL, "invalid pos_type");
+  if (src->dir_type < GH_DIR_ISO || src->dir_type > GH_DIR_FOCUSED) FAIL(GH_EINVAL, "invalid dir_type");
+  if (src->ene_type < GH_ENE_MONO || src->ene_type > GH_ENE_RANGE) FAIL(GH_EINVAL, "invalid ene_type");
+  if ((src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) &&
+      (src->n_energy <= 0 || !src->energy_value || !src->energy_cdf ||
+       (src->ene_type == GH_ENE_HISTOGRAM && !src->energy_width)))
+    FAIL(GH_EINVAL, "spectrum arrays missing");
+  if (src->pos_type == GH_POS_PHSP && (src->phsp_n <= 0 || src->phsp_start < 0))
+    FAIL(GH_EINVAL, "phase-space source needs rows");
+  if (src->pos_type == GH_POS_PHSP)
+    for (int k = 0; k < 8; ++k)
+      if (!src->phsp[k]) FAIL(GH_EINVAL, "phase-space array missing");
+  {
+    // every source energy must lie on the energy grid: (0, E_hi)
+    uint32_t hb = (uint32_t)(BUCKET_BASE + ctx->P.nb) << ESHIFT;
+    float e_hi;
+    std::memcpy(&e_hi, &hb, 4);
+    auto bad = [&](float e) { return !(e > 0.0f) || !(e < e_hi); };
+    bool out_of_grid = false;
+    if (src->pos_type == GH_POS_PHSP) {
+      for (int64_t i = 0; i < src->phsp_n && !out_of_grid; ++i) out_of_grid = bad(src->phsp[6][i]);
+    } else if (src->ene_type == GH_ENE_MONO) {
+      out_of_grid = bad(src->energy_mono);
+    } else if (src->ene_type == GH_ENE_RANGE) {
+      out_of_grid = bad(src->energy_mono) || bad(src->energy_max) || src->energy_max < src->energy_mono;
+    } else {
+      for (int i = 0; i < src->n_energy && !out_of_grid; ++i) {
+        const float top = src->ene_type == GH_ENE_HISTOGRAM ? src->energy_value[i] + src->energy_width[i]
+                                                            : src->energy_value[i];
+        out_of_grid = bad(src->energy_value[i]) || bad(top);
+        if (i > 0 && src->energy_cdf[i] < src->energy_cdf[i - 1]) out_of_grid = true;
+      }
+      if (!out_of_grid && !(src->energy_cdf[src->n_energy - 1] == 1.0f)) {
+        ctx->err = "gh_add_simulation: energy_cdf must end at exactly 1";
+        return GH_EINVAL;
+      }
+    }
+    if (out_of_grid) FAIL(GH_ERANGE, "gh_add_simulation: source energy outside (0, top of the physics tables)");
+  }
+  return GH_OK;
+}
+
+int upload_source(gh_ctx *ctx, const gh_source_desc *src, SimDev &d, std::vector<void *> &owned) {
+  int rc;
+  d.e_val = d.e_width = d.e_cdf = nullptr;
+  d.phsp_n = 0;
+  d.phsp_start = 0;
+  for (int k = 0; k < 8; ++k) d.phsp[k] = nullptr;
+  d.pos_type = src->pos_type; d.dir_type = src->dir_type; d.ene_type = src->ene_type;
+  d.n_energy = src->n_energy;
+  std::memcpy(d.pos_param, src->pos_param, sizeof(d.pos_param));
+  std::memcpy(d.pos_mat, src->pos_mat, sizeof(d.pos_mat));
+  std::memcpy(d.dir_mat, src->dir_mat, sizeof(d.dir_mat));
+  d.ct_min = src->cos_theta_min;
+  d.d_ct = src->cos_theta_min - src->cos_theta_max;
+  d.phi_min = src->phi_min;
+  d.d_phi = src->phi_max - src->phi_min;
+  d.phi_full = src->phi_full;
+  d.phsp_iso = src->phsp_iso;
+  std::memcpy(d.mom, src->momentum, sizeof(d.mom));
+  std::memcpy(d.focus, src->focus, sizeof(d.focus));
+  d.e_mono = src->energy_mono; d.e_max = src->energy_max; d.weight = src->weight;
+  float *t;
+  if (src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) {
+    if ((rc = dev_upload(ctx, src->energy_value, src->n_energy, &t, owned))) return rc;
+    d.e_val = t;
+    if ((rc = dev_upload(ctx, src->energy_cdf, src->n_energy, &t, owned))) return rc;
+    d.e_cdf = t;
+    if (src->ene_type == GH_ENE_HISTOGRAM) {
+      if ((rc = dev_upload(ctx, src->energy_width, src->n_energy, &t, owned))) return rc;
+      d.e_width = t;
+    }
+  }
+  if (src->pos_type == GH_POS_PHSP) {
+    d.phsp_n = src->phsp_n; d.phsp_start = src->phsp_start;
+    for (int k = 0; k < 8; ++k) {
+      if ((rc = dev_upload(ctx, src->phsp[k], (size_t)src->phsp_n, &t, owned))) return rc;
+      d.phsp[k] = t;
+    }
+  }
+  return GH_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int gh_abi_version(void) { return GH_ABI_VERSION; }
@@ -759,7 +847,7 @@ void gh_destroy(gh_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  for (auto &s : ctx->sims) free_all(s.owned);
+  for (auto &s : ctx->sims) { free_all(s.owned); free_all(s.source_owned); }
   free_all(ctx->phys_owned);
   if (ctx->d_sims) (void)hipFree(ctx->d_sims);
   if (ctx->d_chunk_off) (void)hipFree(ctx->d_chunk_off);
@@ -849,7 +937,7 @@ int gh_clear_simulations(gh_ctx *ctx) {
   if (!ctx) return GH_EINVAL;
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  for (auto &s : ctx->sims) free_all(s.owned);
+  for (auto &s : ctx->sims) { free_all(s.owned); free_all(s.source_owned); }
   ctx->sims.clear();
   return GH_OK;
 }
@@ -868,44 +956,9 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
   if (ph->world_material >= ctx->P.n_mat) FAIL(GH_EINVAL, "gh_add_simulation: world material out of range");
   for (int64_t i = 0; i < nvox; ++i)
     if (ph->labels[i] >= ctx->P.n_mat) FAIL(GH_EINVAL, "gh_add_simulation: label out of range of the material table");
-  if (src->pos_type < GH_POS_POINT || src->pos_type > GH_POS_PHSP) FAIL(GH_EINVAL, "invalid pos_type");
-  if (src->dir_type < GH_DIR_ISO || src->dir_type > GH_DIR_FOCUSED) FAIL(GH_EINVAL, "invalid dir_type");
-  if (src->ene_type < GH_ENE_MONO || src->ene_type > GH_ENE_RANGE) FAIL(GH_EINVAL, "invalid ene_type");
-  if ((src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) &&
-      (src->n_energy <= 0 || !src->energy_value || !src->energy_cdf ||
-       (src->ene_type == GH_ENE_HISTOGRAM && !src->energy_width)))
-    FAIL(GH_EINVAL, "spectrum arrays missing");
-  if (src->pos_type == GH_POS_PHSP && (src->phsp_n <= 0 || src->phsp_start < 0))
-    FAIL(GH_EINVAL, "phase-space source needs rows");
-  if (src->pos_type == GH_POS_PHSP)
-    for (int k = 0; k < 8; ++k)
-      if (!src->phsp[k]) FAIL(GH_EINVAL, "phase-space array missing");
   {
-    // every source energy must lie on the energy grid: (0, E_hi)
-    uint32_t hb = (uint32_t)(BUCKET_BASE + ctx->P.nb) << ESHIFT;
-    float e_hi;
-    std::memcpy(&e_hi, &hb, 4);
-    auto bad = [&](float e) { return !(e > 0.0f) || !(e < e_hi); };
-    bool out_of_grid = false;
-    if (src->pos_type == GH_POS_PHSP) {
-      for (int64_t i = 0; i < src->phsp_n && !out_of_grid; ++i) out_of_grid = bad(src->phsp[6][i]);
-    } else if (src->ene_type == GH_ENE_MONO) {
-      out_of_grid = bad(src->energy_mono);
-    } else if (src->ene_type == GH_ENE_RANGE) {
-      out_of_grid = bad(src->energy_mono) || bad(src->energy_max) || src->energy_max < src->energy_mono;
-    } else {
-      for (int i = 0; i < src->n_energy && !out_of_grid; ++i) {
-        const float top = src->ene_type == GH_ENE_HISTOGRAM ? src->energy_value[i] + src->energy_width[i]
-                                                            : src->energy_value[i];
-        out_of_grid = bad(src->energy_value[i]) || bad(top);
-        if (i > 0 && src->energy_cdf[i] < src->energy_cdf[i - 1]) out_of_grid = true;
-      }
-      if (!out_of_grid && !(src->energy_cdf[src->n_energy - 1] == 1.0f)) {
-        ctx->err = "gh_add_simulation: energy_cdf must end at exactly 1";
-        return GH_EINVAL;
-      }
-    }
-    if (out_of_grid) FAIL(GH_ERANGE, "gh_add_simulation: source energy outside (0, top of the physics tables)");
+    const int rc_src = validate_source(ctx, src);
+    if (rc_src) return rc_src;
   }
   if (tal->nx <= 0 || tal->ny <= 0 || tal->nz <= 0) FAIL(GH_EINVAL, "invalid tally size");
   if ((tal->flags & GH_TALLY_ALIGNED) && (tal->nx != ph->nx || tal->ny != ph->ny || tal->nz != ph->nz))
@@ -931,38 +984,8 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
     d.wlo[i] = ph->world_lo[i];
     d.whi[i] = ph->world_hi[i];
   }
-  d.pos_type = src->pos_type; d.dir_type = src->dir_type; d.ene_type = src->ene_type;
-  d.n_energy = src->n_energy;
-  std::memcpy(d.pos_param, src->pos_param, sizeof(d.pos_param));
-  std::memcpy(d.pos_mat, src->pos_mat, sizeof(d.pos_mat));
-  std::memcpy(d.dir_mat, src->dir_mat, sizeof(d.dir_mat));
-  d.ct_min = src->cos_theta_min;
-  d.d_ct = src->cos_theta_min - src->cos_theta_max;
-  d.phi_min = src->phi_min;
-  d.d_phi = src->phi_max - src->phi_min;
-  d.phi_full = src->phi_full;
-  d.phsp_iso = src->phsp_iso;
-  std::memcpy(d.mom, src->momentum, sizeof(d.mom));
-  std::memcpy(d.focus, src->focus, sizeof(d.focus));
-  d.e_mono = src->energy_mono; d.e_max = src->energy_max; d.weight = src->weight;
+  if ((rc = upload_source(ctx, src, d, h.source_owned))) { free_all(h.owned); free_all(h.source_owned); return rc; }
   float *t;
-  if (src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) {
-    if ((rc = dev_upload(ctx, src->energy_value, src->n_energy, &t, h.owned))) { free_all(h.owned); return rc; }
-    d.e_val = t;
-    if ((rc = dev_upload(ctx, src->energy_cdf, src->n_energy, &t, h.owned))) { free_all(h.owned); return rc; }
-    d.e_cdf = t;
-    if (src->ene_type == GH_ENE_HISTOGRAM) {
-      if ((rc = dev_upload(ctx, src->energy_width, src->n_energy, &t, h.owned))) { free_all(h.owned); return rc; }
-      d.e_width = t;
-    }
-  }
-  if (src->pos_type == GH_POS_PHSP) {
-    d.phsp_n = src->phsp_n; d.phsp_start = src->phsp_start;
-    for (int k = 0; k < 8; ++k) {
-      if ((rc = dev_upload(ctx, src->phsp[k], (size_t)src->phsp_n, &t, h.owned))) { free_all(h.owned); return rc; }
-      d.phsp[k] = t;
-    }
-  }
   // tally
   const size_t ntal = (size_t)tal->nx * tal->ny * tal->nz;
   h.n_tally = ntal;
@@ -1008,6 +1031,22 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
   ctx->sims.push_back(std::move(h));
   if (sim_id) *sim_id = (int32_t)ctx->sims.size() - 1;
   return gh_reset_tally(ctx, (int32_t)ctx->sims.size() - 1);
+}
+
+int gh_set_source(gh_ctx *ctx, int32_t sim_id, const gh_source_desc *src) {
+  if (!ctx) return GH_EINVAL;
+  if (sim_id < 0 || sim_id >= (int32_t)ctx->sims.size()) FAIL(GH_EINVAL, "gh_set_source: bad sim id");
+  if (!src) FAIL(GH_EINVAL, "gh_set_source: null descriptor");
+  int rc = validate_source(ctx, src);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  SimHost &h = ctx->sims[sim_id];
+  std::vector<void *> fresh;
+  if ((rc = upload_source(ctx, src, h.dev, fresh))) { free_all(fresh); return rc; }
+  free_all(h.source_owned);
+  h.source_owned = std::move(fresh);
+  return GH_OK;
 }
 
 int gh_reset_tally(gh_ctx *ctx, int32_t sim_id) {

@@ -1,0 +1,333 @@
+"""Run driver: turns a Simulation into C-ABI descriptors and runs it on the GPU.
+
+Plays the role of opengate/engines.py (SimulationEngine.run_engine :1554,
+initialize :1721, start_and_stop :1632) for the photon path:
+  * the phantom volume (one ImageVolume, or a homogeneous Box) becomes a label grid
+    in the phantom frame p_ph = R^T (p_world - t) + L/2 (box [0, L));
+  * materials of the labels (+ the world, unless vacuum) get FP32 tables;
+  * every source becomes a SourceSpec in that frame; sources run one after the
+    other into the same tally with disjoint history ranges of the RNG;
+  * the DoseActor grid becomes the tally, dose/uncertainty are derived in FP64 as
+    DoseActor.EndOfRunActionMasterThread and get_variance_or_uncertainty do.
+"""
+import time
+
+import numpy as np
+
+from .actors.dose import GY_FACTOR, DoseActor, PhaseSpaceActor, SimulationStatisticsActor, FACES
+from .geometry.volumes import BoxVolume, ImageVolume
+from .physics import xs
+from .sources.generic import GenericSource, PhaseSpaceSource
+from .transport import GpuEngine, PhantomSpec, SourceSpec, TallySpec, spectrum_histogram, spectrum_lines
+from .units import g4_units
+
+VACUUM = ("G4_Galactic", "Vacuum", "G4_VACUUM")
+
+
+def _rot(m):
+    return np.asarray(m, dtype=np.float64).reshape(3, 3)
+
+
+class SimulationEngine:
+    def __init__(self, sim):
+        self.sim = sim
+        self.engine = None
+
+    # ------------------------------------------------------------------ geometry
+    def phantom_volume(self):
+        vols = [v for n, v in self.sim.volume_manager.volumes.items() if n != "world"]
+        images = [v for v in vols if isinstance(v, ImageVolume)]
+        if len(images) > 1:
+            raise NotImplementedError("one ImageVolume per simulation")
+        if images:
+            other = [v for v in vols if v is not images[0] and images[0].mother != v.name]
+            if other:
+                raise NotImplementedError(f"volumes {[v.name for v in other]} besides the ImageVolume and its mothers")
+            return images[0]
+        if len(vols) == 1 and isinstance(vols[0], BoxVolume):
+            return vols[0]
+        if not vols:
+            raise NotImplementedError("a phantom volume (ImageVolume or Box) is required")
+        raise NotImplementedError("only one ImageVolume (with Box mothers) or one Box phantom")
+
+    def build_geometry(self, vol):
+        db = self.sim.volume_manager.material_database
+        if isinstance(vol, ImageVolume):
+            lab16 = vol.create_label_image(self.engine).array
+            lut = vol.material_to_label_lut
+            names = [None] * len(lut)
+            for m, k in lut.items():
+                names[k] = m
+            spacing = vol.spacing
+            size = vol.size_pix
+        else:
+            names = [vol.material]
+            spacing = np.asarray(vol.size, dtype=np.float64)
+            size = np.array([1, 1, 1])
+            lab16 = np.zeros((1, 1, 1), np.uint16)
+        if len(names) > 255:
+            raise NotImplementedError("more than 255 materials in one phantom")
+        mats = [db.get(n) for n in names]
+        world = self.sim.world
+        world_index = -1
+        if world.material not in VACUUM:
+            if world.material in names:
+                world_index = names.index(world.material)
+            else:
+                mats.append(db.get(world.material))
+                names.append(world.material)
+                world_index = len(mats) - 1
+        R, t = self.sim.volume_manager.global_transform(vol.name)
+        L = size * spacing
+        self.R, self.t, self.L, self.size, self.spacing = R, t, L, size, spacing
+        wlo = whi = (0.0, 0.0, 0.0)
+        if world_index >= 0:
+            if not np.allclose(R, np.eye(3)):
+                raise NotImplementedError("a rotated phantom needs a vacuum world (world.material = 'G4_Galactic')")
+            half = np.asarray(world.size, dtype=np.float64) / 2
+            wlo = tuple(-half - t + L / 2)
+            whi = tuple(half - t + L / 2)
+        ph = PhantomSpec(lab16.astype(np.uint8), tuple(spacing), world_index, wlo, whi)
+        return ph, mats, names
+
+    def to_phantom(self, p_world):
+        return self.R.T @ (np.asarray(p_world, dtype=np.float64) - self.t) + self.L / 2
+
+    # ------------------------------------------------------------------ sources
+    def source_spec(self, src, n):
+        vm = self.sim.volume_manager
+        Ra, ta = vm.global_transform(src.attached if isinstance(src.attached, str) else src.attached.name)
+        if isinstance(src, PhaseSpaceSource):
+            arrs = src.load_arrays()
+            Rp = _rot(src.position.rotation) if src.rotate_direction else np.eye(3)
+            tp = np.asarray(src.position.translation, dtype=np.float64) if src.translate_position else np.zeros(3)
+            Rpos = _rot(src.position.rotation) if src.translate_position else np.eye(3)
+            if src.global_flag:
+                Rg, tg = np.eye(3), np.zeros(3)
+            else:
+                Rg, tg = Ra, ta
+            pos_rot = self.R.T @ Rg @ Rpos
+            pos_trans = self.R.T @ (Rg @ tp + tg - self.t) + self.L / 2
+            dir_rot = self.R.T @ Rg @ Rp
+            return SourceSpec(pos_type="phsp", pos_rot=pos_rot, pos_trans=tuple(pos_trans), dir_rot=dir_rot,
+                              phsp=arrs, phsp_start=int(src.entry_start), phsp_iso=bool(src.isotropic_direction))
+        assert isinstance(src, GenericSource)
+        if src.particle != "gamma":
+            raise NotImplementedError(f"source '{src.name}': particle '{src.particle}' (photon path: gamma only)")
+        pos = src.position
+        Rl = _rot(pos.rotation)
+        Rg = Ra @ Rl
+        Tg = Ra @ np.asarray(pos.translation, dtype=np.float64) + ta
+        kind = pos.type
+        if kind == "point":
+            pp = (0.0, 0.0, 0.0, 0.0)
+        elif kind == "box":
+            pp = tuple(pos.size) + (0.0,)
+        elif kind == "sphere":
+            pp = (pos.radius, 0.0, 0.0, 0.0)
+        elif kind == "disc":
+            pp = (pos.radius, 0.0, 0.0, 0.0)
+        elif kind == "cylinder":
+            pp = (pos.radius, pos.dz if pos.dz is not None else 0.0, 0.0, 0.0)
+        elif kind == "ellipsoid":
+            pp = tuple(np.asarray(pos.size, dtype=np.float64) / 2) + (0.0,)
+        else:
+            raise NotImplementedError(f"source position type '{kind}'")
+        if pos.confine:
+            raise NotImplementedError("position.confine")
+        d = src.direction
+        rel = bool(src.direction_relative_to_attached_volume)
+        dir_rot = self.R.T @ (Rg if rel else np.eye(3))
+        spec = SourceSpec(pos_type=kind, pos_param=pp, pos_rot=self.R.T @ Rg,
+                          pos_trans=tuple(self.R.T @ (Tg - self.t) + self.L / 2), dir_rot=dir_rot,
+                          weight=1.0 if src.weight is None or src.weight < 0 else float(src.weight))
+        if d.type == "iso":
+            spec.dir_type = "iso"
+            spec.theta = (float(d.theta[0]), float(d.theta[1]))
+            spec.phi = (float(d.phi[0]), float(d.phi[1]))
+        elif d.type == "momentum":
+            spec.dir_type = "momentum"
+            spec.momentum = tuple(np.asarray(d.momentum, dtype=np.float64))
+        elif d.type == "focused":
+            spec.dir_type = "focused"
+            f = np.asarray(d.focus_point, dtype=np.float64)
+            if rel:
+                f = Rg @ (f - np.asarray(pos.translation, dtype=np.float64)) + Tg
+            spec.focus = tuple(self.to_phantom(f))
+        else:
+            raise NotImplementedError(f"direction type '{d.type}'")
+        if getattr(d, "angular_acceptance", None) and d.angular_acceptance.get("enabled"):
+            raise NotImplementedError("direction.angular_acceptance")
+        e = src.energy
+        if e.type == "mono":
+            spec.ene_type, spec.energy = "mono", float(e.mono)
+        elif e.type == "range":
+            spec.ene_type, spec.energy, spec.energy_max = "range", float(e.min_energy), float(e.max_energy)
+        elif e.type == "spectrum_discrete":
+            spec.ene_type = "spectrum_discrete"
+            spec.energy_value, _, spec.energy_cdf = spectrum_lines(e.spectrum_energies, e.spectrum_weights)
+        elif e.type == "spectrum_histogram":
+            if e.spectrum_histogram_interpolation not in (None, "None", "none"):
+                raise NotImplementedError("spectrum_histogram_interpolation")
+            spec.ene_type = "spectrum_histogram"
+            spec.energy_value, spec.energy_width, spec.energy_cdf = spectrum_histogram(
+                e.spectrum_energy_bin_edges, e.spectrum_weights)
+        else:
+            raise NotImplementedError(f"energy type '{e.type}'")
+        if e.is_cdf:
+            raise NotImplementedError("energy.is_cdf")
+        return spec
+
+    # ------------------------------------------------------------------ tally
+    def tally_spec(self, vol, ph):
+        doses = [a for a in self.sim.actors.values() if isinstance(a, DoseActor)]
+        if len(doses) > 1:
+            raise NotImplementedError("one DoseActor per simulation")
+        recs = [a for a in self.sim.actors.values() if isinstance(a, PhaseSpaceActor)]
+        if len(recs) > 1:
+            raise NotImplementedError("one PhaseSpaceActor per simulation")
+        rec_face, rec_cap = -1, 0
+        if recs:
+            r = recs[0]
+            if r.attached_to_name != vol.name:
+                raise NotImplementedError("the PhaseSpaceActor records photons leaving the phantom volume")
+            if self.sim.world.material not in VACUUM:
+                raise NotImplementedError("exit-face recording needs a vacuum world")
+            rec_face, rec_cap = FACES[r.exit_face], int(r.max_photons)
+        self.dose = doses[0] if doses else None
+        if self.dose is None:
+            return TallySpec(size=tuple(self.size), spacing=tuple(self.spacing), edep2=False,
+                             record_face=rec_face, record_capacity=rec_cap)
+        a = self.dose
+        if a.attached_to_name != vol.name:
+            raise NotImplementedError("the DoseActor must be attached to the phantom volume")
+        if not np.allclose(_rot(a.rotation), np.eye(3)):
+            raise NotImplementedError("DoseActor.rotation other than identity")
+        size, spacing = a.resolved_grid(vol)
+        tr = np.asarray(a.translation, dtype=np.float64)
+        aligned = (np.array_equal(size, self.size) and np.allclose(spacing, self.spacing) and not tr.any())
+        corner = self.L / 2 + tr - size * spacing / 2
+        self.tally_grid = (size, spacing)
+        return TallySpec(size=tuple(int(x) for x in size), spacing=tuple(spacing), corner=tuple(corner),
+                         aligned=aligned, edep2=a.wants_squared(), dose=a.wants_dose() and not aligned,
+                         dose2=a.wants_dose() and a.wants_squared() and not aligned,
+                         counts=a.counts.active, record_face=rec_face, record_capacity=rec_cap)
+
+    # ------------------------------------------------------------------ run
+    def run(self):
+        sim = self.sim
+        t_init = time.time()
+        sim.physics_manager.check_supported()
+        seed = sim.resolve_seed()
+        self.engine = GpuEngine(sim.device)
+        vol = self.phantom_volume()
+        ph, mats, names = self.build_geometry(vol)
+        duration_s = sum((b - a) for a, b in sim.run_timing_intervals) / g4_units.s
+        runs = []
+        for s in sim.sources.values():
+            n = s.number_of_primaries(duration_s)
+            if n > 0:
+                runs.append((s, self.source_spec(s, n), n))
+        if not runs:
+            raise ValueError("no source with primaries (set source.n or source.activity)")
+        e_max = max(spec.max_energy() for _, spec, _ in runs)
+        e_cut = max(float(sim.physics_manager.photon_tracking_cut), 2.0 ** -10)
+        tables = xs.build_tables(mats, e_max * (1 + 1e-6), e_cut=e_cut)
+        tal = self.tally_spec(vol, ph)
+        self.engine.set_physics(tables)
+        sid = self.engine.add_simulation(ph, runs[0][1], tal)
+        t_init = time.time() - t_init
+
+        counters = None
+        first = 0
+        self.executed = []  # (SourceSpec, histories, first history) in launch order
+        self.inputs = (tables, ph, tal)
+        t0 = time.time()
+        for k, (src, spec, n) in enumerate(runs):
+            if k > 0:
+                self.engine.set_source(sid, spec)
+            todo = n
+            while todo > 0:
+                chunk = min(todo, 2**32 - 1)
+                if self.dose is not None and self.dose.uncertainty_goal is not None:
+                    chunk = self._goal_chunk(first, todo)
+                c = self.engine.run([chunk], seed, first_history=[first], **sim.launch)
+                self.executed.append((spec, chunk, first))
+                counters = c if counters is None else {key: counters[key] + c[key] for key in c}
+                first += chunk
+                todo -= chunk
+                if self.dose is not None and self.dose.uncertainty_goal is not None and self._goal_reached(first):
+                    todo = 0
+        duration = time.time() - t0
+        n_events = first
+        self.seed = seed
+        self.finish(vol, ph, tal, names, tables, counters, n_events, duration, t_init)
+        return counters
+
+    # uncertainty goal: GateDoseActor::EndOfEventAction / ComputeMeanUncertainty (GateDoseActor.cpp:245-330)
+    def _goal_chunk(self, done, todo):
+        a = self.dose
+        if done == 0:
+            return int(min(todo, max(1, a.uncertainty_first_check_after_n_events)))
+        return int(min(todo, max(1, self._next_check - done)))
+
+    def _goal_reached(self, n):
+        a = self.dose
+        t = self.engine.get_tally(0)
+        edep, edep2 = t["edep"], t["edep2"]
+        top = np.sort(edep.ravel())[-int(a.uncertainty_top_voxels_count):]
+        thresh = a.uncertainty_voxel_edep_threshold * top.mean()
+        sel = edep > thresh
+        from .actors.dose import variance_or_uncertainty
+
+        unc = variance_or_uncertainty(edep[sel], edep2[sel], n, "uncertainty")
+        u = float(unc.mean()) if unc.size else 1.0
+        self.goal_uncertainty = u
+        if u <= a.uncertainty_goal:
+            return True
+        self._next_check = int(n * (u / a.uncertainty_goal) ** 2 * a.uncertainty_overshoot_factor_N_events) + 1
+        return False
+
+    def finish(self, vol, ph, tal, names, tables, counters, n_events, duration, t_init):
+        sim = self.sim
+        t = self.engine.get_tally(0)
+        if self.dose is not None:
+            a = self.dose
+            size, spacing = self.tally_grid
+            vvox = float(np.prod(spacing))
+            origin = a.output_origin(size, spacing, vol)
+            edep = t["edep"]
+            edep2 = t.get("edep2")
+            dose = dose2 = density = None
+            if tal.aligned:
+                rho = tables.density.astype(np.float64)[ph.labels]
+                density = rho
+                if a.wants_dose():
+                    k = GY_FACTOR / vvox
+                    inv = np.divide(1.0, rho, out=np.zeros_like(rho), where=rho > 0)
+                    dose = edep * inv * k
+                    if edep2 is not None:
+                        dose2 = edep2 * inv * inv * k * k
+            elif a.wants_dose():
+                k = GY_FACTOR / vvox
+                dose = t["dose"] * k
+                if "dose2" in t:
+                    dose2 = t["dose2"] * k * k
+            if a.density.active and density is None:
+                raise NotImplementedError("density output needs the dose grid of the image")
+            a.store_results(size, spacing, origin, edep, edep2, dose, dose2, t.get("counts"), density, n_events)
+            a.write_outputs()
+        for r in sim.actors.values():
+            if isinstance(r, PhaseSpaceActor):
+                arrs, n = self.engine.get_phsp(0)
+                r.store(arrs, n, self.L / 2)
+                r.write_outputs()
+        for s in sim.actors.values():
+            if isinstance(s, SimulationStatisticsActor):
+                s.counts.update(runs=len(sim.run_timing_intervals), events=int(n_events),
+                                tracks=int(n_events + counters["compton"] + counters["photo"]),
+                                steps=int(counters["steps"]), duration=duration * g4_units.s,
+                                init=t_init * g4_units.s)
+                s.raw = dict(counters)
+                s.write_outputs()
+        self.engine.close()

@@ -69,8 +69,10 @@ def ct12_materials(db: MaterialDatabase):
         if name not in db:
             b = db.get(base)
             from .physics.elements import symbol_of
+            from .units import g4_units
 
-            db.add_material_weights(name, [symbol_of(z) for z in b.elements], list(b.fractions), rho)
+            db.add_material_weights(name, [symbol_of(z) for z in b.elements], list(b.fractions),
+                                    rho * g4_units.g_cm3)
         vm.append([lo, hi, name])
     return vm
 

@@ -91,11 +91,14 @@ class MaterialDatabase:
             self.materials[name] = _mk(name, rho, comp, vac)
 
     def add_material_weights(self, name, elements, weights, density):
-        """elements: symbols or names; weights: mass fractions; density in g/cm3."""
+        """elements: symbols or names; weights: mass fractions; density in Geant4 units
+        (g_cm3 = 6.24150907e18), as MaterialDatabase.add_material_weights of the reference."""
+        from ..units import g4_units
+
         comp = {}
         for e, w in zip(elements, weights):
             comp[e] = comp.get(e, 0.0) + float(w)
-        self.materials[name] = _mk(name, density, comp)
+        self.materials[name] = _mk(name, float(density) / g4_units.g_cm3, comp)
         return self.materials[name]
 
     def add_material(self, material: Material):

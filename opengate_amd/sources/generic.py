@@ -1,0 +1,115 @@
+"""GenericSource and PhaseSpaceSource with the reference's user parameters.
+
+GenericSource: opengate/sources/generic.py:24-95 (position / direction / energy
+parameter groups), :333-420 (source attributes); number of primaries `n` or
+`activity` (sources/base.py:23-80). PhaseSpaceSource: opengate/sources/phspsources.py
+:253-482 (keys, entry_start, global_flag, isotropic_direction, translate / rotate).
+The ROOT reader of the reference (uproot) is replaced by numpy .npz / .npy-dict files
+that hold the same branch names.
+"""
+import numpy as np
+
+from ..base import Box, GateObject
+from ..units import g4_units
+
+
+def _position_parameters():
+    return Box(type="point", radius=0.0, sigma_x=0.0, sigma_y=0.0, size=[0.0, 0.0, 0.0],
+               translation=[0.0, 0.0, 0.0], rotation=np.eye(3), confine=None, dz=None)
+
+
+def _direction_parameters():
+    return Box(type="iso", theta=[0.0, 180.0 * g4_units.deg], phi=[0.0, 360.0 * g4_units.deg],
+               momentum=[0.0, 0.0, 1.0], focus_point=[0.0, 0.0, 0.0], sigma=[0.0, 0.0],
+               angular_acceptance=Box(enabled=False), accolinearity_flag=False,
+               accolinearity_fwhm=0.5 * g4_units.deg, histogram_theta_weights=[],
+               histogram_theta_angles=[], histogram_phi_weights=[], histogram_phi_angles=[])
+
+
+def _energy_parameters():
+    return Box(type="mono", mono=0.0, sigma_gauss=0.0, is_cdf=False, min_energy=None, max_energy=None,
+               spectrum_type=None, spectrum_weights=[], spectrum_energies=[], histogram_weight=[],
+               histogram_energy=[], spectrum_energy_bin_edges=[], spectrum_histogram_interpolation=None)
+
+
+class SourceBase(GateObject):
+    runtime_attributes = ("simulation",)
+    user_info_defaults = {
+        "attached_to": ("world", {"doc": "volume the source is attached to"}),
+        "mother": (None, {"doc": "deprecated alias of attached_to"}),
+        "n": (0, {"doc": "number of primaries (per run)"}),
+        "activity": (0.0, {"doc": "activity [Bq in Geant4 units]; primaries = activity x run duration"}),
+        "half_life": (-1, {"doc": "not used on the photon path"}),
+        "start_time": (None, {"doc": "not used"}),
+        "end_time": (None, {"doc": "not used"}),
+    }
+
+    def __init__(self, name=None, **kw):
+        super().__init__(name, **kw)
+        self.simulation = None
+
+    def number_of_primaries(self, duration_s):
+        if self.n and int(self.n) > 0:
+            return int(self.n)
+        if self.activity and self.activity > 0:
+            return int(round(self.activity / g4_units.Bq * duration_s))
+        return 0
+
+    @property
+    def attached(self):
+        return self.mother if self.mother else self.attached_to
+
+
+class GenericSource(SourceBase):
+    user_info_defaults = {
+        "particle": ("gamma", {"doc": "only 'gamma' on the photon path"}),
+        "weight": (-1, {"doc": "history weight (-1: 1)"}),
+        "weight_sigma": (-1, {"doc": "not supported"}),
+        "direction_relative_to_attached_volume": (False, {"doc": "rotate directions with the attached volume"}),
+        "position": (_position_parameters(), {"doc": "position parameters"}),
+        "direction": (_direction_parameters(), {"doc": "direction parameters"}),
+        "energy": (_energy_parameters(), {"doc": "energy parameters"}),
+        "polarization": ([], {"doc": "not supported"}),
+    }
+    source_type = "GenericSource"
+
+
+class PhaseSpaceSource(SourceBase):
+    user_info_defaults = {
+        "phsp_file": (None, {"doc": ".npz file (or dict of arrays) with the branches below"}),
+        "position_key": ("PrePositionLocal", {"doc": "prefix of the _X/_Y/_Z position branches"}),
+        "position_key_x": (None, {}), "position_key_y": (None, {}), "position_key_z": (None, {}),
+        "direction_key": ("PreDirectionLocal", {"doc": "prefix of the direction branches"}),
+        "direction_key_x": (None, {}), "direction_key_y": (None, {}), "direction_key_z": (None, {}),
+        "energy_key": ("KineticEnergy", {}),
+        "weight_key": ("Weight", {}),
+        "PDGCode_key": ("PDGCode", {}),
+        "particle": ("gamma", {}),
+        "global_flag": (False, {"doc": "positions/directions are global (no attached-volume transform)"}),
+        "isotropic_direction": (False, {"doc": "ignore the stored direction, emit isotropically"}),
+        "translate_position": (False, {}),
+        "rotate_direction": (False, {}),
+        "position": (Box(translation=[0.0, 0.0, 0.0], rotation=np.eye(3)), {}),
+        "entry_start": (0, {"doc": "first entry used"}),
+        "batch_size": (10000, {"doc": "kept for compatibility"}),
+        "generate_until_next_primary": (False, {}),
+        "primary_lower_energy_threshold": (0.0, {}),
+        "primary_PDGCode": (0, {}),
+        "verbose_batch": (False, {}),
+    }
+    source_type = "PhaseSpaceSource"
+
+    def load_arrays(self):
+        """8 float32 arrays (x, y, z, dx, dy, dz, E, w) of the phase space file."""
+        f = self.phsp_file
+        data = dict(f) if isinstance(f, dict) else dict(np.load(f, allow_pickle=False))
+
+        def key(prefix, axis, override):
+            return override if override else f"{prefix}_{axis}"
+
+        cols = [data[key(self.position_key, a, getattr(self, f"position_key_{a.lower()}"))] for a in "XYZ"]
+        cols += [data[key(self.direction_key, a, getattr(self, f"direction_key_{a.lower()}"))] for a in "XYZ"]
+        cols.append(data[self.energy_key])
+        w = data.get(self.weight_key)
+        cols.append(w if w is not None else np.ones(len(cols[0])))
+        return [np.ascontiguousarray(c, dtype=np.float32) for c in cols]

@@ -271,6 +271,14 @@ class GpuEngine:
         self.sims.append((phantom, source, tally))
         return sid.value
 
+    def set_source(self, sim_id, source: SourceSpec):
+        if self.tables is not None and source.max_energy() >= self.tables.e_max:
+            raise GhError(f"source energy {source.max_energy()} MeV beyond the physics tables ({self.tables.e_max} MeV)")
+        sd = source.desc()
+        self._check(self.lib.gh_set_source(self.ctx, int(sim_id), C.byref(sd)), "gh_set_source")
+        ph, _, tal = self.sims[sim_id]
+        self.sims[sim_id] = (ph, source, tal)
+
     def reset_tally(self, sim_id):
         self._check(self.lib.gh_reset_tally(self.ctx, int(sim_id)), "gh_reset_tally")
 

@@ -1,0 +1,55 @@
+"""Benchmark workloads as GPU-engine inputs (phantom, source, tally, tables).
+
+ct256 (the CT-scale workload): 256x256x128 int16 HU phantom at 1 mm mapped to 12
+materials by HU intervals on the GPU, world of air (3 m cube), 120 kVp Kramers
+spectrum (2.5 mm Al, 1 keV bins) in a cone of 15 deg half angle from 600 mm above
+the phantom centre, dose on the phantom grid (edep, edep^2).
+
+water64: 64^3 water with three cortical-bone spheres at 2 mm, 140.5 keV point source
+at the centre, vacuum world, dose on the phantom grid.
+"""
+import numpy as np
+
+from . import phantoms as P
+from .physics import xs
+from .physics.materials import MaterialDatabase
+from .transport import PhantomSpec, SourceSpec, TallySpec, spectrum_histogram
+
+CT256_DESC = ("ct256: 256x256x128 HU phantom, 1 mm voxels, 12 materials + air world; 120 kVp Kramers "
+              "spectrum (2.5 mm Al, 1 keV bins), 15 deg half-angle cone from 600 mm; dose on the 256x256x128 grid")
+WATER64_DESC = "water64: 64^3 water + 3 bone spheres, 2 mm; 140.5 keV point source at the centre; dose 64^3"
+
+
+def ct256(engine, seed=2):
+    """Returns (tables, phantom, source, tally, info). Labels are computed on the GPU."""
+    db = MaterialDatabase()
+    vm = P.ct12_materials(db)
+    hu = P.ct256_hu(seed)
+    names = ["CT_Air"] + [r[2] for r in vm if r[2] != "CT_Air"]
+    lut = {n: i for i, n in enumerate(names)}
+    rows = sorted(vm, key=lambda r: r[0])
+    bins = np.array([r[0] for r in rows] + [rows[-1][1]], dtype=np.float64)
+    labels_sorted = np.array([0] + [lut[r[2]] for r in rows] + [0], dtype=np.uint16)
+    lab = engine.hu_to_labels(hu, bins, labels_sorted)
+    mats = [db.get(n) for n in names] + [db.get("G4_AIR")]
+    tables = xs.build_tables(mats, 0.1201)
+    world = len(mats) - 1
+    L = np.array([256.0, 256.0, 128.0])
+    half_w = 1500.0
+    ph = PhantomSpec(lab.astype(np.uint8), (1.0, 1.0, 1.0), world, tuple(-half_w + L / 2), tuple(half_w + L / 2))
+    e, w = P.kramers_spectrum(db=db)
+    ev, ew, cdf = spectrum_histogram(e, w)
+    src = SourceSpec(pos_trans=(128.0, 128.0, 64.0 + 600.0), theta=(0.0, np.deg2rad(15.0)),
+                     ene_type="spectrum_histogram", energy_value=ev, energy_width=ew, energy_cdf=cdf)
+    tal = TallySpec(size=(256, 256, 128), spacing=(1.0, 1.0, 1.0))
+    return tables, ph, src, tal, {"desc": CT256_DESC, "hu": hu, "names": names}
+
+
+def water64(seed=1):
+    db = MaterialDatabase()
+    lab, spheres = P.water64_labels(seed)
+    tables = xs.build_tables([db.get("G4_WATER"), db.get("G4_BONE_CORTICAL_ICRP")], 0.1406)
+    ph = PhantomSpec(lab, (2.0, 2.0, 2.0))
+    src = SourceSpec(pos_trans=(64.0, 64.0, 64.0), energy=0.1405)
+    tal = TallySpec(size=(64, 64, 64), spacing=(2.0, 2.0, 2.0))
+    return tables, ph, src, tal, {"desc": WATER64_DESC, "spheres": spheres}

@@ -56,6 +56,12 @@ def lib():
                               C.POINTER(A.TallyDesc), C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int,
                               DP, DP, DP, DP, DP, DP, C.POINTER(A.FP * 8), C.c_uint64, C.POINTER(C.c_uint64),
                               A.FP, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+        L.ref_sample_compton.argtypes = [C.POINTER(A.PhysicsDesc), C.c_int, C.c_float, C.c_uint64, C.c_uint64,
+                                         A.FP, A.FP]
+        L.ref_sample_rayleigh.argtypes = [C.POINTER(A.PhysicsDesc), C.c_int, C.c_float, C.c_uint64, C.c_uint64,
+                                          A.FP]
+        L.ref_mu_at.restype = C.c_float
+        L.ref_mu_at.argtypes = [C.POINTER(A.PhysicsDesc), C.c_int, C.c_float]
         L.ref_hu_to_labels.argtypes = [C.POINTER(C.c_int16), C.c_int64, DP, C.c_int, C.POINTER(C.c_uint16),
                                        C.POINTER(C.c_uint16)]
         _lib = L
@@ -133,6 +139,34 @@ def run(tables, phantom, source, tally, n, seed, first=0, sim_id=0, threads=1, e
         counters[name] = float(en[i])
     counters["recorded_total"] = int(rec_n.value)
     return out, counters
+
+
+def sample_compton(tables, m, E, n, seed=1):
+    from opengate_amd import _abi as A
+    from opengate_amd.transport import physics_desc
+
+    pd = physics_desc(tables)
+    q = np.zeros(n, np.float32)
+    c = np.zeros(n, np.float32)
+    lib().ref_sample_compton(C.byref(pd), m, E, n, seed, A.fptr(q), A.fptr(c))
+    return q, c
+
+
+def sample_rayleigh(tables, m, E, n, seed=1):
+    from opengate_amd import _abi as A
+    from opengate_amd.transport import physics_desc
+
+    pd = physics_desc(tables)
+    c = np.zeros(n, np.float32)
+    lib().ref_sample_rayleigh(C.byref(pd), m, E, n, seed, A.fptr(c))
+    return c
+
+
+def mu_at(tables, m, E):
+    from opengate_amd.transport import physics_desc
+
+    pd = physics_desc(tables)
+    return lib().ref_mu_at(C.byref(pd), m, float(E))
 
 
 def hu_to_labels_c(hu, bins, lut):

@@ -805,3 +805,39 @@ void ref_hu_to_labels(const int16_t *hu, int64_t n, const double *bins, int n_bi
     out[i] = lut[k];
   }
 }
+
+/* Sampler hooks for the distribution tests: n draws of the Compton (q = E/E', cos)
+ * and Rayleigh (cos) samplers for material m at energy E, stream (seed, history i). */
+void ref_sample_compton(const gh_physics_desc *phys, int m, float E, uint64_t n, uint64_t seed, float *q_out,
+                        float *c_out) {
+  phys_t P;
+  P.d = phys;
+  P.nb1 = phys->n_buckets + 1;
+  for (uint64_t i = 0; i < n; ++i) {
+    stream_t s;
+    stream_open(&s, seed, i, 0);
+    uint64_t rej = 0;
+    compton_sample(&s, &P, m, E, &q_out[i], &c_out[i], &rej);
+  }
+}
+
+void ref_sample_rayleigh(const gh_physics_desc *phys, int m, float E, uint64_t n, uint64_t seed, float *c_out) {
+  phys_t P;
+  P.d = phys;
+  P.nb1 = phys->n_buckets + 1;
+  for (uint64_t i = 0; i < n; ++i) {
+    stream_t s;
+    stream_open(&s, seed, i, 0);
+    c_out[i] = rayleigh_sample(&s, &P, m, E);
+  }
+}
+
+/* interpolated table value at E (mu_total row of material m), as the transport sees it */
+float ref_mu_at(const gh_physics_desc *phys, int m, float E) {
+  phys_t P;
+  P.d = phys;
+  P.nb1 = phys->n_buckets + 1;
+  eidx_t x;
+  energy_index(&P, E, &x);
+  return node_interp(phys->mu_total + (size_t)m * P.nb1, &x);
+}

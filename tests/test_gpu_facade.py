@@ -1,0 +1,121 @@
+"""The drop-in API on the GPU: a reference-style user script (ImageVolume + HU table,
+GenericSource, DoseActor, SimulationStatisticsActor) checked against the CPU oracle
+run on the very descriptors the facade handed to the C-ABI."""
+import numpy as np
+import pytest
+
+import opengate_amd as gate
+from oracle import oracle as O
+from opengate_amd import phantoms as P
+from opengate_amd.actors.dose import GY_FACTOR
+
+pytestmark = pytest.mark.gpu
+
+
+def water_bone_image():
+    lab, _ = P.water64_labels(seed=1, n=32, spacing=2.0)
+    hu = np.where(lab == 1, 1200, 0).astype(np.int16)
+    return gate.Image(hu, (2.0, 2.0, 2.0), origin=(-31.0, -31.0, -31.0))
+
+
+def build_sim(tmp_path, seed=42, two_sources=False):
+    u = gate.g4_units
+    sim = gate.Simulation()
+    sim.output_dir = str(tmp_path)
+    sim.random_seed = seed
+    sim.world.material = "G4_Galactic"
+    ct = sim.add_volume("Image", "ct")
+    ct.image = water_bone_image()
+    ct.material = "G4_AIR"
+    ct.voxel_materials = [[-1000, 500, "G4_WATER"], [500, 3000, "G4_BONE_CORTICAL_ICRP"]]
+    ct.translation = [5 * u.mm, -3 * u.mm, 2 * u.mm]
+    src = sim.add_source("GenericSource", "tc99m")
+    src.particle = "gamma"
+    src.position.type = "sphere"
+    src.position.radius = 4 * u.mm
+    src.position.translation = [5 * u.mm, -3 * u.mm, 2 * u.mm]
+    src.direction.type = "iso"
+    src.energy.type = "mono"
+    src.energy.mono = 140.5 * u.keV
+    src.n = 40_000
+    if two_sources:
+        s2 = sim.add_source("GenericSource", "lu")
+        s2.position.type = "box"
+        s2.position.size = [10 * u.mm, 10 * u.mm, 10 * u.mm]
+        s2.energy.type = "spectrum_discrete"
+        s2.energy.spectrum_energies = [112.95 * u.keV, 208.37 * u.keV]
+        s2.energy.spectrum_weights = [0.062, 0.1038]
+        s2.n = 30_000
+    dose = sim.add_actor("DoseActor", "dose")
+    dose.attached_to = "ct"
+    dose.size = "like_image_volume"
+    dose.spacing = "like_image_volume"
+    dose.edep_uncertainty.active = True
+    dose.dose.active = True
+    dose.dose_uncertainty.active = True
+    dose.output_filename = "out.mhd"
+    stats = sim.add_actor("SimulationStatisticsActor", "stats")
+    return sim, dose, stats
+
+
+@pytest.mark.parametrize("two_sources", [False, True])
+def test_user_script_matches_oracle(tmp_path, two_sources):
+    sim, dose, stats = build_sim(tmp_path, two_sources=two_sources)
+    print(f"[gpu case] facade two_sources={two_sources}", flush=True)
+    sim.run()
+    eng = sim.engine
+    tables, ph, tal = eng.inputs
+    edep = np.zeros_like(dose.edep.image.array)
+    edep2 = np.zeros_like(edep)
+    n_tot = 0
+    for spec, n, first in eng.executed:
+        r, _ = O.run(tables, ph, spec, tal, n, eng.seed, first=first)
+        edep += r["edep"]
+        edep2 += r["edep2"]
+        n_tot += n
+    assert stats.counts["events"] == n_tot == dose.number_of_samples
+    g = dose.edep.image.array
+    assert np.max(np.abs(g - edep)) <= 1e-9 * edep.max()
+    assert np.max(np.abs(dose.edep_squared.image.array - edep2)) <= 1e-9 * edep2.max()
+    # dose = edep / rho / V in Gy; uncertainty = the reference formula
+    rho = tables.density.astype(np.float64)[ph.labels]
+    exp_dose = np.divide(edep, rho, out=np.zeros_like(edep), where=rho > 0) * GY_FACTOR / 8.0
+    assert np.allclose(dose.dose.image.array, exp_dose, rtol=1e-9, atol=1e-30)
+    np.testing.assert_allclose(dose.edep_uncertainty.image.array, O.uncertainty(edep, edep2, n_tot),
+                               rtol=1e-6, atol=1e-12)
+    # files written with the reference suffixes, origin of the local frame
+    img = gate.read_image(tmp_path / "out_edep.mhd")
+    np.testing.assert_allclose(img.origin, -32 * 2.0 / 2 + 1.0 + np.array([0.0, 0.0, 0.0]))
+    assert (tmp_path / "out_dose_uncertainty.mhd").exists()
+    assert stats.pps > 0
+
+
+def test_ct256_labels_and_transport_parity(engine):
+    from opengate_amd import workloads
+
+    print("[gpu case] ct256 labels + 100k histories", flush=True)
+    tables, ph, src, tal, info = workloads.ct256(engine)
+    vm = P.ct12_materials(__import__("opengate_amd.physics.materials", fromlist=["x"]).MaterialDatabase())
+    ref, lut = O.label_image(info["hu"], "CT_Air", vm)
+    remap = np.array([info["names"].index(m) for m, _ in sorted(lut.items(), key=lambda kv: kv[1])])
+    np.testing.assert_array_equal(ph.labels, remap[ref])
+    engine.clear()
+    engine.set_physics(tables)
+    engine.add_simulation(ph, src, tal)
+    c = engine.run([100_000], 5)
+    g = engine.get_tally(0)
+    r, rc = O.run(tables, ph, src, tal, 100_000, 5)
+    for k in ("histories", "steps", "deposits", "compton", "rayleigh", "photo", "escaped", "world_interactions"):
+        assert c[k] == rc[k], k
+    assert np.max(np.abs(g["edep"] - r["edep"])) <= 1e-9 * r["edep"].max()
+
+
+def test_uncertainty_goal_stops_early(tmp_path):
+    sim, dose, stats = build_sim(tmp_path)
+    sim.sources["tc99m"].n = 2_000_000
+    dose.uncertainty_goal = 0.05
+    dose.uncertainty_first_check_after_n_events = 20_000
+    print("[gpu case] uncertainty goal", flush=True)
+    sim.run()
+    assert stats.counts["events"] < 2_000_000
+    assert sim.engine.goal_uncertainty <= 0.05
