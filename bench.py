@@ -64,7 +64,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories", type=float, default=1e9)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--refill", type=int, default=8)
+    ap.add_argument("--refill", type=int, default=64)
+    ap.add_argument("--slow-min", type=int, default=16)
+    ap.add_argument("--refill-min", type=int, default=16)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy x5 (reference kernel for rocprofv3 FETCH/WRITE_SIZE)")
@@ -92,7 +94,8 @@ def main():
     eng.set_physics(tables)
     eng.add_simulation(ph, src, tal)
     n = int(args.histories)
-    launch = {"refill_interval": args.refill, "grid": args.grid}
+    launch = {"refill_interval": args.refill, "grid": args.grid, "slow_min": args.slow_min,
+              "refill_min": args.refill_min}
     try:
         import torch
 

@@ -126,10 +126,12 @@ typedef struct gh_tally_desc {
 } gh_tally_desc;
 
 typedef struct gh_launch {
-  int32_t block;             /* threads per workgroup (multiple of 64) */
+  int32_t block;             /* threads per workgroup (multiple of 64, <= 256) */
   int32_t grid;              /* workgroups; 0 = automatic */
-  int32_t refill_interval;   /* Woodcock iterations between wave refills */
+  int32_t refill_interval;   /* max step-phase iterations (2 Woodcock steps each) per loop */
   int32_t chunk;             /* histories per work-queue chunk */
+  int32_t slow_min;          /* parked lanes (collision / exit / world / dead) that end a step phase */
+  int32_t refill_min;        /* dead lanes that trigger a refill (clamped to slow_min) */
 } gh_launch;
 
 typedef struct gh_counters {

@@ -61,7 +61,8 @@ class TallyDesc(C.Structure):
 
 
 class Launch(C.Structure):
-    _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("refill_interval", C.c_int32), ("chunk", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("refill_interval", C.c_int32), ("chunk", C.c_int32),
+                ("slow_min", C.c_int32), ("refill_min", C.c_int32)]
 
 
 class Counters(C.Structure):

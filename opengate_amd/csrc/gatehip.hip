@@ -74,7 +74,9 @@ struct Launch {
   uint32_t chunk;
   unsigned long long *next_chunk;
   uint64_t seed;
-  int refill;
+  int max_iter;   // step-phase iterations (2 tentative steps each) between slow phases
+  int slow_min;   // lanes waiting for slow work that end a step phase
+  int refill_min; // dead lanes that trigger a refill (<= slow_min)
   Phys P;
   unsigned long long *counters;   // [12]
   double *energies;               // [4]
@@ -131,20 +133,25 @@ __device__ __forceinline__ void ray_box(float px, float py, float pz, float dx, 
   slab(pz, dz, Lz, 2, tnear, tfar, face);
 }
 
-__device__ __forceinline__ float world_axis(float p, float d, float lo, float hi, float t) {
-  if (d > 0.0f) return fminf(t, (hi - p) / d);
-  if (d < 0.0f) return fminf(t, (lo - p) / d);
+__device__ __forceinline__ float world_axis(float p, float d, float inv, float lo, float hi, float t) {
+  if (d > 0.0f) return fminf(t, (hi - p) * inv);
+  if (d < 0.0f) return fminf(t, (lo - p) * inv);
   return t;
 }
 
-// distance to the world box boundary from inside
+// distance to the world box boundary from inside (iu, iv, iw = 1/direction)
 __device__ __forceinline__ float world_exit(const SimDev &s, float px, float py, float pz, float dx,
                                             float dy, float dz) {
   float t = INFINITY;
-  t = world_axis(px, dx, s.wlo[0], s.whi[0], t);
-  t = world_axis(py, dy, s.wlo[1], s.whi[1], t);
-  t = world_axis(pz, dz, s.wlo[2], s.whi[2], t);
+  t = world_axis(px, dx, 1.0f / dx, s.wlo[0], s.whi[0], t);
+  t = world_axis(py, dy, 1.0f / dy, s.wlo[1], s.whi[1], t);
+  t = world_axis(pz, dz, 1.0f / dz, s.wlo[2], s.whi[2], t);
   return fmaxf(t, 0.0f);
+}
+
+// FP64 add to global memory with the hardware atomic (no return value)
+__device__ __forceinline__ void gatomic_add(double *p, double v) {
+  (void)__builtin_amdgcn_global_atomic_fadd_f64((__attribute__((address_space(1))) double *)p, v);
 }
 
 // --------------------------------------------------------------------- photon
@@ -176,7 +183,6 @@ enum { E_DEP = 0, E_OUT, E_ESC, E_WORLD, N_ENERGIES };
 
 struct LaneStats {
   uint32_t steps, rej;
-  double e_dep;
   unsigned long long *cnt;  // LDS [N_COUNTERS]
   double *en;               // LDS [N_ENERGIES]
   __device__ __forceinline__ void inc(int k) {
@@ -189,13 +195,13 @@ struct LaneStats {
 
 __device__ __forceinline__ void flush_slot(const SimDev &s, int vox, float sum, float dsum, uint32_t cnt,
                                            LaneStats &st) {
-  unsafeAtomicAdd(s.edep + vox, (double)sum);
-  if (s.edep2) unsafeAtomicAdd(s.edep2 + vox, (double)sum * (double)sum);
-  if (s.counts) unsafeAtomicAdd(s.counts + vox, (double)cnt);
-  if (s.dose) unsafeAtomicAdd(s.dose + vox, (double)dsum);
-  if (s.dose2) unsafeAtomicAdd(s.dose2 + vox, (double)dsum * (double)dsum);
+  gatomic_add(s.edep + vox, (double)sum);
+  if (s.edep2) gatomic_add(s.edep2 + vox, (double)sum * (double)sum);
+  if (s.counts) gatomic_add(s.counts + vox, (double)cnt);
+  if (s.dose) gatomic_add(s.dose + vox, (double)dsum);
+  if (s.dose2) gatomic_add(s.dose2 + vox, (double)dsum * (double)dsum);
   st.inc(C_DEPOSITS);
-  st.e_dep += (double)sum;
+  st.add_e(E_DEP, (double)sum);
 }
 
 __device__ __forceinline__ void sq_flush_all(const SimDev &s, SqCache &c, LaneStats &st) {
@@ -350,21 +356,78 @@ __device__ __forceinline__ bool inside_box(const Hot &h, float x, float y, float
 }
 
 // ---------------------------------------------------------------------- kernel
-template <bool BATCHED, bool LDS_TABLES>
-__global__ void __launch_bounds__(256) transport_kernel(Launch L) {
+// lane modes
+enum { M_DEAD = 0, M_STEP = 1, M_COLLIDE = 2, M_EXIT = 3, M_WORLD = 4 };
+
+// u01 of a raw Philox word (odd multiples of 2^-24)
+__device__ __forceinline__ float u01w(uint32_t w) { return (float)((w >> 8) | 1u) * 0x1p-24f; }
+
+// Persistent transport kernel. The loop of a wave has three phases, each entered by
+// all 64 lanes together (the conditions are ballots, so they are wave-uniform):
+//   refill  dead lanes get new histories (source sampling) once at least refill_min
+//           lanes are dead, or when no lane can step;
+//   step    every lane in M_STEP takes up to two Woodcock steps per iteration from one
+//           Philox block of its step stream; a real collision (M_COLLIDE) or a box exit
+//           (M_EXIT) parks the lane; the phase ends when slow_min lanes are parked;
+//   slow    parked lanes process their collision, exit or world event together, using
+//           the event stream of their history.
+// Batching the slow work this way keeps the wave converged: a collision in one lane no
+// longer stalls 63 lanes that only want to step.
+// LDS layout (floats) of the staged tables; STAGE 1 stages mu + majorant, STAGE 2 all
+struct LdsLayout {
+  int mu, maj, pcpr, sinc, rcdf, rx, total;
+};
+__host__ __device__ inline LdsLayout lds_layout(int n_mat, int nb1, int nx, int stage) {
+  LdsLayout o;
+  o.mu = 0;
+  o.maj = n_mat * nb1;
+  int end = o.maj + nb1;
+  o.pcpr = (end + 1) & ~1;
+  o.sinc = o.pcpr + 2 * n_mat * nb1;
+  o.rcdf = o.sinc + n_mat * nx;
+  o.rx = o.rcdf + n_mat * nx;
+  o.total = stage >= 2 ? o.rx + nx : end;
+  if (stage == 0) o.total = 0;
+  return o;
+}
+
+template <bool BATCHED, int STAGE, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
   extern __shared__ float lds[];
   const Phys &P = L.P;
   const int NB1 = P.nb + 1;
-  const float *mu_tab = P.mu;
-  const float *maj_tab = P.maj;
-  if (LDS_TABLES) {
+  Tabs T;
+  T.mu = P.mu; T.maj = P.maj; T.pcpr = P.pcpr; T.rx = P.rx; T.rcdf = P.rcdf; T.sinc = P.sinc;
+  if (STAGE >= 1) {
+    const LdsLayout o = lds_layout(P.n_mat, NB1, P.nx, STAGE);
     const int n_mu = P.n_mat * NB1;
-    for (int i = threadIdx.x; i < n_mu; i += blockDim.x) lds[i] = P.mu[i];
-    for (int i = threadIdx.x; i < NB1; i += blockDim.x) lds[n_mu + i] = P.maj[i];
-    __syncthreads();
-    mu_tab = lds;
-    maj_tab = lds + n_mu;
+    for (int i = threadIdx.x; i < n_mu; i += BLOCK) lds[o.mu + i] = P.mu[i];
+    for (int i = threadIdx.x; i < NB1; i += BLOCK) lds[o.maj + i] = P.maj[i];
+    T.mu = lds + o.mu;
+    T.maj = lds + o.maj;
+    if (STAGE >= 2) {
+      for (int i = threadIdx.x; i < 2 * n_mu; i += BLOCK) lds[o.pcpr + i] = P.pcpr[i];
+      const int n_x = P.n_mat * P.nx;
+      for (int i = threadIdx.x; i < n_x; i += BLOCK) {
+        lds[o.sinc + i] = P.sinc[i];
+        lds[o.rcdf + i] = P.rcdf[i];
+      }
+      for (int i = threadIdx.x; i < P.nx; i += BLOCK) lds[o.rx + i] = P.rx[i];
+      T.pcpr = lds + o.pcpr;
+      T.sinc = lds + o.sinc;
+      T.rcdf = lds + o.rcdf;
+      T.rx = lds + o.rx;
+    }
   }
+  const float *mu_tab = T.mu;
+  const float *maj_tab = T.maj;
+  __shared__ unsigned long long s_cnt[N_COUNTERS];
+  __shared__ double s_en[N_ENERGIES];
+  if (threadIdx.x < N_COUNTERS) s_cnt[threadIdx.x] = 0ull;
+  if (threadIdx.x < N_ENERGIES) s_en[threadIdx.x] = 0.0;
+  __syncthreads();
+  LaneStats st;
+  st.steps = 0u; st.rej = 0u; st.cnt = s_cnt; st.en = s_en;
 
   Hot hot;
   if (!BATCHED) hot.load(L.sims[0]);
@@ -376,175 +439,164 @@ __global__ void __launch_bounds__(256) transport_kernel(Launch L) {
   ec.b = 0; ec.frac = 0.0f; ec.maj = 1.0f; ec.inv_maj = 1.0f;
   Rng rng;
   rng_init(rng, 0, 0, 0);
+  uint32_t sblock = 0;           // step-stream block counter of the current history
   SqCache sq = {};
   float ev_edep = 0.0f;
-  __shared__ unsigned long long s_cnt[N_COUNTERS];
-  __shared__ double s_en[N_ENERGIES];
-  if (threadIdx.x < N_COUNTERS) s_cnt[threadIdx.x] = 0ull;
-  if (threadIdx.x < N_ENERGIES) s_en[threadIdx.x] = 0.0;
-  __syncthreads();
-  LaneStats st;
-  st.steps = 0u; st.rej = 0u; st.e_dep = 0.0; st.cnt = s_cnt; st.en = s_en;
-  bool alive = false;
+  int mode = M_DEAD;
+  int cvox = 0, clab = 0;        // voxel and label of a pending collision
 
   // wave-uniform work state
   int cur_sim = 0;
   uint64_t cur_next = 0, cur_end = 0;
   bool have_work = true;
+  const uint32_t slow_min = (uint32_t)L.slow_min;
+  const uint32_t refill_min = (uint32_t)L.refill_min;
 
   for (;;) {
     // ------------------------------------------------------------- refill
-    uint64_t dead = __ballot(!alive);
-    while (dead != 0ull && have_work) {
-      if (cur_next >= cur_end) {
-        unsigned long long c = 0;
-        if (lane_id() == 0) c = atomicAdd(L.next_chunk, 1ull);
-        c = __shfl(c, 0, 64);
-        if (c >= L.total_chunks) { have_work = false; break; }
-        int s = 0;
-        while (s + 1 < L.n_sims && L.chunk_offsets[s + 1] <= c) ++s;
-        cur_sim = s;
-        const uint64_t local = (c - L.chunk_offsets[s]) * (uint64_t)L.chunk;
-        cur_next = local;
-        const uint64_t e = local + L.chunk;
-        cur_end = e < L.sims[s].n_hist ? e : L.sims[s].n_hist;
-        continue;
-      }
-      const uint64_t avail = cur_end - cur_next;
-      const uint32_t nd = (uint32_t)__popcll(dead);
-      const uint32_t take = (uint32_t)(avail < nd ? avail : nd);
-      if (!alive) {
-        const uint32_t rank = mbcnt(dead);
-        if (rank < take) {
-          const SimDev &s = L.sims[BATCHED ? cur_sim : 0];
-          if (BATCHED) hot.load(s);
-          ph.sim = BATCHED ? cur_sim : 0;
-          ph.hist = (uint32_t)(cur_next + rank);
-          rng_init(rng, L.seed, s.first_hist + ph.hist, (uint32_t)ph.sim);
-          sample_source(s, rng, ph.hist, ph);
-          st.inc(C_HIST);
-          ev_edep = 0.0f;
-          sq.n = 0;
-          ph.mode = inside_box(hot, ph.x, ph.y, ph.z) ? MODE_BOX : MODE_WORLD;
-          if (ph.E < P.e_cut) {
-            // below the tracking cut from the start: deposit locally (inside) and stop
-            if (ph.mode == MODE_BOX) {
-              const int ix = (int)floorf(ph.x * hot.isx), iy = (int)floorf(ph.y * hot.isy),
-                        iz = (int)floorf(ph.z * hot.isz);
-              const int vox = ix + hot.nx * (iy + hot.ny * iz);
-              const int lab = hot.labels[vox];
-              const float x = ph.E * ph.wt;
-              deposit(s, sq, tally_voxel(s, hot, vox, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
-              if (lab >= s.det_label_min && s.det_label_min >= 0) ev_edep += x;
-              sq_flush_all(s, sq, st);
-            } else {
-              st.add_e(E_WORLD, (double)(ph.E * ph.wt));
-            }
-            st.inc(C_CUT);
-            if (s.event_edep) s.event_edep[ph.hist] = ev_edep;
-          } else {
-            ecache_set(ec, ph.E, maj_tab);
-            alive = true;
+    {
+      uint64_t dead = __ballot(mode == M_DEAD);
+      const uint32_t n_step = (uint32_t)__popcll(__ballot(mode == M_STEP));
+      if (have_work && dead != 0ull && ((uint32_t)__popcll(dead) >= refill_min || n_step == 0u)) {
+        while (dead != 0ull && have_work) {
+          if (cur_next >= cur_end) {
+            unsigned long long c = 0;
+            if (lane_id() == 0) c = atomicAdd(L.next_chunk, 1ull);
+            c = __shfl(c, 0, 64);
+            if (c >= L.total_chunks) { have_work = false; break; }
+            int s = 0;
+            while (s + 1 < L.n_sims && L.chunk_offsets[s + 1] <= c) ++s;
+            cur_sim = s;
+            const uint64_t local = (c - L.chunk_offsets[s]) * (uint64_t)L.chunk;
+            cur_next = local;
+            const uint64_t e = local + L.chunk;
+            cur_end = e < L.sims[s].n_hist ? e : L.sims[s].n_hist;
+            continue;
           }
-        }
-      }
-      cur_next += take;
-      dead = __ballot(!alive);
-    }
-    if (!have_work && __ballot(alive) == 0ull) break;
-
-    // ------------------------------------------------------------- transport
-    for (int it = 0; it < L.refill; ++it) {
-      if (!alive) break;
-      const SimDev &s = L.sims[BATCHED ? ph.sim : 0];
-      bool kill = false;
-      if (ph.mode == MODE_BOX) {
-        // ---- Woodcock tentative step
-        const float d = -glog(u01(rng)) * ec.inv_maj;
-        const float nx = fmaf(d, ph.u, ph.x), ny = fmaf(d, ph.v, ph.y), nz = fmaf(d, ph.w, ph.z);
-        const int ix = (int)floorf(nx * hot.isx), iy = (int)floorf(ny * hot.isy), iz = (int)floorf(nz * hot.isz);
-        if ((unsigned)ix < (unsigned)hot.nx && (unsigned)iy < (unsigned)hot.ny && (unsigned)iz < (unsigned)hot.nz) {
-          ph.x = nx; ph.y = ny; ph.z = nz;
-          st.steps++;
-          const int vox = ix + hot.nx * (iy + hot.ny * iz);
-          const int lab = hot.labels[vox];
-          const float mu = lerp_node(mu_tab + lab * NB1, ec);
-          if (u01(rng) * ec.maj < mu) {
-            // ---- real interaction
-            const float r = u01(rng);
-            const float pc = lerp_node(P.pc + lab * NB1, ec);
-            const float pr = lerp_node(P.pr + lab * NB1, ec);
-            const int tv = tally_voxel(s, hot, vox, ph.x, ph.y, ph.z);
-            const float irho = P.inv_rho[lab];
-            const bool det = s.det_label_min >= 0 && lab >= s.det_label_min;
-            if (r < pc) {
-              st.inc(C_COMPTON);
-              float q, c;
-              sample_compton(rng, P, lab, ph.E, q, c, st.rej);
-              const float e1 = ph.E / q;
-              const float x = (ph.E - e1) * ph.wt;
-              deposit(s, sq, tv, x, x * irho, st);
-              if (det) ev_edep += x;
-              float cp, sp;
-              azimuth(rng, cp, sp);
-              rotate(ph.u, ph.v, ph.w, c, cp, sp);
-              ph.E = e1;
+          const uint64_t avail = cur_end - cur_next;
+          const uint32_t nd = (uint32_t)__popcll(dead);
+          const uint32_t take = (uint32_t)(avail < nd ? avail : nd);
+          if (mode == M_DEAD) {
+            const uint32_t rank = mbcnt(dead);
+            if (rank < take) {
+              const SimDev &s = L.sims[BATCHED ? cur_sim : 0];
+              if (BATCHED) hot.load(s);
+              ph.sim = BATCHED ? cur_sim : 0;
+              ph.hist = (uint32_t)(cur_next + rank);
+              rng_init(rng, L.seed, s.first_hist + ph.hist, (uint32_t)ph.sim);
+              sblock = 0u;
+              sample_source(s, rng, ph.hist, ph);
+              st.inc(C_HIST);
+              ev_edep = 0.0f;
+              sq.n = 0;
+              const bool inside = inside_box(hot, ph.x, ph.y, ph.z);
               if (ph.E < P.e_cut) {
-                const float x2 = ph.E * ph.wt;
-                deposit(s, sq, tv, x2, x2 * irho, st);
-                if (det) ev_edep += x2;
+                // below the tracking cut from the start: deposit locally (inside) and stop
+                if (inside) {
+                  const int ix = (int)floorf(ph.x * hot.isx), iy = (int)floorf(ph.y * hot.isy),
+                            iz = (int)floorf(ph.z * hot.isz);
+                  const int vox = ix + hot.nx * (iy + hot.ny * iz);
+                  const int lab = hot.labels[vox];
+                  const float x = ph.E * ph.wt;
+                  deposit(s, sq, tally_voxel(s, hot, vox, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
+                  if (lab >= s.det_label_min && s.det_label_min >= 0) ev_edep += x;
+                  sq_flush_all(s, sq, st);
+                } else {
+                  st.add_e(E_WORLD, (double)(ph.E * ph.wt));
+                }
                 st.inc(C_CUT);
-                kill = true;
+                if (s.event_edep) s.event_edep[ph.hist] = ev_edep;
               } else {
                 ecache_set(ec, ph.E, maj_tab);
+                mode = inside ? M_STEP : M_WORLD;
               }
-            } else if (r < pr) {
-              st.inc(C_RAYLEIGH);
-              const float c = sample_rayleigh(rng, P, lab, ph.E);
-              float cp, sp;
-              azimuth(rng, cp, sp);
-              rotate(ph.u, ph.v, ph.w, c, cp, sp);
-            } else {
-              st.inc(C_PHOTO);
-              const float x = ph.E * ph.wt;
-              deposit(s, sq, tv, x, x * irho, st);
-              if (det) ev_edep += x;
-              kill = true;
             }
           }
-        } else {
-          // ---- leaves the phantom box: exit point from the last position inside
-          float tn, tf;
-          int face;
-          ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
-          const float t = fmaxf(tf, 0.0f);
-          const float ex = fmaf(t, ph.u, ph.x), ey = fmaf(t, ph.v, ph.y), ez = fmaf(t, ph.w, ph.z);
-          if (s.rec_face >= 0 && face == s.rec_face) {
-            const uint64_t m = __ballot(1);
-            const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-            unsigned long long base = 0;
-            if (lane_id() == leader) base = atomicAdd(s.rec_count, (unsigned long long)__popcll(m));
-            base = __shfl(base, leader, 64);
-            const uint64_t slot = base + mbcnt(m);
-            if (slot < s.rec_cap) {
-              s.rec[0][slot] = ex; s.rec[1][slot] = ey; s.rec[2][slot] = ez;
-              s.rec[3][slot] = ph.u; s.rec[4][slot] = ph.v; s.rec[5][slot] = ph.w;
-              s.rec[6][slot] = ph.E; s.rec[7][slot] = ph.wt;
+          cur_next += take;
+          dead = __ballot(mode == M_DEAD);
+        }
+      }
+      if (!have_work && __ballot(mode != M_DEAD) == 0ull) break;
+    }
+
+    // ------------------------------------------------------------- step phase
+    for (int it = 0; it < L.max_iter; ++it) {
+      const uint32_t n_step = (uint32_t)__popcll(__ballot(mode == M_STEP));
+      const uint32_t n_slow = (uint32_t)__popcll(
+          __ballot(mode == M_COLLIDE || mode == M_EXIT || mode == M_WORLD || (have_work && mode == M_DEAD)));
+      if (n_step == 0u || n_slow >= slow_min) break;
+      if (mode == M_STEP) {
+        const uint64_t gid = L.sims[BATCHED ? ph.sim : 0].first_hist + ph.hist;
+        uint32_t w0 = (uint32_t)gid, w1 = (uint32_t)(gid >> 32), w2 = (uint32_t)ph.sim, w3 = 0x80000000u | sblock;
+        philox10(w0, w1, w2, w3, rng.k0, rng.k1);
+        sblock++;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (mode == M_STEP) {
+            const float d = -glog(u01w(k == 0 ? w0 : w2)) * ec.inv_maj;
+            const float nx = fmaf(d, ph.u, ph.x), ny = fmaf(d, ph.v, ph.y), nz = fmaf(d, ph.w, ph.z);
+            const int ix = (int)floorf(nx * hot.isx), iy = (int)floorf(ny * hot.isy),
+                      iz = (int)floorf(nz * hot.isz);
+            if ((unsigned)ix < (unsigned)hot.nx && (unsigned)iy < (unsigned)hot.ny &&
+                (unsigned)iz < (unsigned)hot.nz) {
+              ph.x = nx; ph.y = ny; ph.z = nz;
+              st.steps++;
+              const int vox = ix + hot.nx * (iy + hot.ny * iz);
+              const int lab = hot.labels[vox];
+              const float mu = lerp_node(mu_tab + lab * NB1, ec);
+              if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
+                mode = M_COLLIDE;
+                cvox = vox;
+                clab = lab;
+              }
+            } else {
+              mode = M_EXIT;
             }
-            st.inc(C_RECORDED);
-            st.add_e(E_ESC, (double)(ph.E * ph.wt));
-            kill = true;
-          } else if (hot.world_mat < 0) {
-            st.inc(C_ESCAPED);
-            st.add_e(E_ESC, (double)(ph.E * ph.wt));
-            kill = true;
-          } else {
-            ph.x = ex; ph.y = ey; ph.z = ez;
-            ph.mode = MODE_WORLD;
           }
         }
-      } else {
-        // ---- world outside the phantom box (homogeneous material or vacuum)
+      }
+    }
+
+    // ------------------------------------------------------------- slow phase
+    // order: box exit -> world event -> collision. A world interaction becomes a
+    // collision with the world material and no tally voxel (cvox = -1), so box and
+    // world collisions share one copy of the interaction code.
+    if (mode == M_COLLIDE || mode == M_EXIT || mode == M_WORLD) {
+      const SimDev &s = L.sims[BATCHED ? ph.sim : 0];
+      bool kill = false;
+      if (mode == M_EXIT) {
+        // leaves the phantom box: exit point from the last position inside
+        float tn, tf;
+        int face;
+        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
+        const float t = fmaxf(tf, 0.0f);
+        const float ex = fmaf(t, ph.u, ph.x), ey = fmaf(t, ph.v, ph.y), ez = fmaf(t, ph.w, ph.z);
+        if (s.rec_face >= 0 && face == s.rec_face) {
+          const uint64_t m = __ballot(1);
+          const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+          unsigned long long base = 0;
+          if (lane_id() == leader) base = atomicAdd(s.rec_count, (unsigned long long)__popcll(m));
+          base = __shfl(base, leader, 64);
+          const uint64_t slot = base + mbcnt(m);
+          if (slot < s.rec_cap) {
+            s.rec[0][slot] = ex; s.rec[1][slot] = ey; s.rec[2][slot] = ez;
+            s.rec[3][slot] = ph.u; s.rec[4][slot] = ph.v; s.rec[5][slot] = ph.w;
+            s.rec[6][slot] = ph.E; s.rec[7][slot] = ph.wt;
+          }
+          st.inc(C_RECORDED);
+          st.add_e(E_ESC, (double)(ph.E * ph.wt));
+          kill = true;
+        } else if (hot.world_mat < 0) {
+          st.inc(C_ESCAPED);
+          st.add_e(E_ESC, (double)(ph.E * ph.wt));
+          kill = true;
+        } else {
+          ph.x = ex; ph.y = ey; ph.z = ez;
+          mode = M_WORLD;
+        }
+      }
+      if (mode == M_WORLD && !kill) {
+        // world outside the phantom box (homogeneous material or vacuum)
         float tn, tf;
         int face;
         ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
@@ -553,7 +605,7 @@ __global__ void __launch_bounds__(256) transport_kernel(Launch L) {
         if (hot.world_mat < 0) {
           if (hit) {
             ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
-            ph.mode = MODE_BOX;
+            mode = M_STEP;
           } else {
             st.inc(C_ESCAPED);
             st.add_e(E_ESC, (double)(ph.E * ph.wt));
@@ -566,7 +618,7 @@ __global__ void __launch_bounds__(256) transport_kernel(Launch L) {
           const float d = (muw > 0.0f) ? (-glog(u01(rng)) / muw) : INFINITY;
           if (hit && d >= tin && tin < tw) {
             ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
-            ph.mode = MODE_BOX;
+            mode = M_STEP;
           } else if (d >= tw) {
             st.inc(C_ESCAPED);
             st.add_e(E_ESC, (double)(ph.E * ph.wt));
@@ -574,41 +626,70 @@ __global__ void __launch_bounds__(256) transport_kernel(Launch L) {
           } else {
             ph.x = fmaf(d, ph.u, ph.x); ph.y = fmaf(d, ph.v, ph.y); ph.z = fmaf(d, ph.w, ph.z);
             st.inc(C_WORLD);
-            const float r = u01(rng);
-            const float pc = lerp_node(P.pc + wm * NB1, ec);
-            const float pr = lerp_node(P.pr + wm * NB1, ec);
-            if (r < pc) {
-              float q, c;
-              sample_compton(rng, P, wm, ph.E, q, c, st.rej);
-              const float e1 = ph.E / q;
-              st.add_e(E_WORLD, (double)((ph.E - e1) * ph.wt));
-              float cp, sp;
-              azimuth(rng, cp, sp);
-              rotate(ph.u, ph.v, ph.w, c, cp, sp);
-              ph.E = e1;
-              if (ph.E < P.e_cut) {
-                st.add_e(E_WORLD, (double)(ph.E * ph.wt));
-                st.inc(C_CUT);
-                kill = true;
-              } else {
-                ecache_set(ec, ph.E, maj_tab);
-              }
-            } else if (r < pr) {
-              const float c = sample_rayleigh(rng, P, wm, ph.E);
-              float cp, sp;
-              azimuth(rng, cp, sp);
-              rotate(ph.u, ph.v, ph.w, c, cp, sp);
-            } else {
-              st.add_e(E_WORLD, (double)(ph.E * ph.wt));
-              kill = true;
-            }
+            mode = M_COLLIDE;
+            clab = wm;
+            cvox = -1;
           }
         }
+      }
+      if (mode == M_COLLIDE && !kill) {
+        const bool in_world = cvox < 0;
+        const int lab = clab;
+        const float r = u01(rng);
+        float pc, pr;
+        branching(T, NB1, lab, ec, pc, pr);
+        const int tv = in_world ? -1 : tally_voxel(s, hot, cvox, ph.x, ph.y, ph.z);
+        const float irho = P.inv_rho[lab];
+        const bool det = !in_world && s.det_label_min >= 0 && lab >= s.det_label_min;
+        mode = in_world ? M_WORLD : M_STEP;
+        float x_dep = 0.0f;   // energy given to the medium at this collision
+        if (r < pc) {
+          if (!in_world) st.inc(C_COMPTON);
+          float q, c;
+          sample_compton(rng, T, P.nx, lab, ph.E, q, c, st.rej);
+          const float e1 = ph.E / q;
+          x_dep = (ph.E - e1) * ph.wt;
+          float cp, sp;
+          azimuth(rng, cp, sp);
+          rotate(ph.u, ph.v, ph.w, c, cp, sp);
+          ph.E = e1;
+        } else if (r < pr) {
+          if (!in_world) st.inc(C_RAYLEIGH);
+          const float c = sample_rayleigh(rng, T, P.nx, lab, ph.E);
+          float cp, sp;
+          azimuth(rng, cp, sp);
+          rotate(ph.u, ph.v, ph.w, c, cp, sp);
+        } else {
+          if (!in_world) st.inc(C_PHOTO);
+          x_dep = ph.E * ph.wt;
+          kill = true;
+        }
+        // local deposition: electron energy, and the photon itself below the tracking cut
+        float x_cut = 0.0f;
+        if (!kill && ph.E < P.e_cut) {
+          x_cut = ph.E * ph.wt;
+          st.inc(C_CUT);
+          kill = true;
+        }
+        if (in_world) {
+          if (x_dep != 0.0f) st.add_e(E_WORLD, (double)x_dep);
+          if (x_cut != 0.0f) st.add_e(E_WORLD, (double)x_cut);
+        } else {
+          if (x_dep != 0.0f) {
+            deposit(s, sq, tv, x_dep, x_dep * irho, st);
+            if (det) ev_edep += x_dep;
+          }
+          if (x_cut != 0.0f) {
+            deposit(s, sq, tv, x_cut, x_cut * irho, st);
+            if (det) ev_edep += x_cut;
+          }
+        }
+        if (!kill && r < pc) ecache_set(ec, ph.E, maj_tab);
       }
       if (kill) {
         sq_flush_all(s, sq, st);
         if (s.event_edep) s.event_edep[ph.hist] = ev_edep;
-        alive = false;
+        mode = M_DEAD;
       }
     }
   }
@@ -617,11 +698,9 @@ __global__ void __launch_bounds__(256) transport_kernel(Launch L) {
   {
     const uint64_t steps = wave_sum<uint64_t>((uint64_t)st.steps);
     const uint64_t rej = wave_sum<uint64_t>((uint64_t)st.rej);
-    const double edep = wave_sum<double>(st.e_dep);
     if (lane_id() == 0) {
       atomicAdd(s_cnt + C_STEPS, (unsigned long long)steps);
       atomicAdd(s_cnt + C_REJ, (unsigned long long)rej);
-      unsafeAtomicAdd(s_en + E_DEP, edep);
     }
   }
   __syncthreads();
@@ -696,7 +775,7 @@ struct gh_ctx {
   bool have_phys = false;
   Phys P{};
   std::vector<void *> phys_owned;
-  size_t lds_bytes = 0;
+  size_t lds1_bytes = 0, lds2_bytes = 0;
   // simulations
   std::vector<SimHost> sims;
   // run scratch
@@ -706,6 +785,8 @@ struct gh_ctx {
   unsigned long long *d_counters = nullptr;
   double *d_energies = nullptr;
   size_t d_sims_cap = 0;
+  gh_launch last_launch{};
+  int last_stage = 0;
 };
 
 namespace {
@@ -895,8 +976,12 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
       }
     }
     if (ph->ray_x[0] != 0.0f) FAIL(GH_EINVAL, "gh_set_physics: ray_x[0] must be 0");
-    for (size_t i = 1; i < NX; ++i)
-      if (!(ph->ray_x[i] > ph->ray_x[i - 1])) FAIL(GH_EINVAL, "gh_set_physics: ray_x must be ascending");
+    for (size_t i = 1; i < NX; ++i) {
+      uint32_t b;
+      std::memcpy(&b, &ph->ray_x[i], 4);
+      if (b != ((uint32_t)(X_BUCKET_BASE + (int)i - 1) << XSHIFT))
+        FAIL(GH_EINVAL, "gh_set_physics: ray_x must be the q^2 bucket grid (node i >= 1 has bits (920+i-1)<<20)");
+    }
     for (size_t m = 0; m < M; ++m)
       for (size_t i = 1; i < NX; ++i)
         if (!(ph->ray_cdf[m * NX + i] >= ph->ray_cdf[m * NX + i - 1]))
@@ -916,8 +1001,12 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
   int rc;
   if ((rc = dev_upload(ctx, ph->mu_total, M * NB1, &t, ctx->phys_owned))) return rc; P.mu = t;
   if ((rc = dev_upload(ctx, ph->mu_majorant, NB1, &t, ctx->phys_owned))) return rc; P.maj = t;
-  if ((rc = dev_upload(ctx, ph->p_compton, M * NB1, &t, ctx->phys_owned))) return rc; P.pc = t;
-  if ((rc = dev_upload(ctx, ph->p_rayleigh, M * NB1, &t, ctx->phys_owned))) return rc; P.pr = t;
+  std::vector<float> pcpr(2 * M * NB1);
+  for (size_t i = 0; i < M * NB1; ++i) {
+    pcpr[2 * i] = ph->p_compton[i];
+    pcpr[2 * i + 1] = ph->p_rayleigh[i];
+  }
+  if ((rc = dev_upload(ctx, (const float *)pcpr.data(), 2 * M * NB1, &t, ctx->phys_owned))) return rc; P.pcpr = t;
   if ((rc = dev_upload(ctx, muen, M * NB1, &t, ctx->phys_owned))) return rc; P.muen = t;
   if ((rc = dev_upload(ctx, ph->ray_x, NX, &t, ctx->phys_owned))) return rc; P.rx = t;
   if ((rc = dev_upload(ctx, ph->ray_cdf, M * NX, &t, ctx->phys_owned))) return rc; P.rcdf = t;
@@ -928,7 +1017,8 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
   P.nx = (int)NX;
   P.e_cut = ph->e_cut;
   ctx->P = P;
-  ctx->lds_bytes = (M * NB1 + NB1) * sizeof(float);
+  ctx->lds1_bytes = (size_t)lds_layout((int)M, (int)NB1, (int)NX, 1).total * sizeof(float);
+  ctx->lds2_bytes = (size_t)lds_layout((int)M, (int)NB1, (int)NX, 2).total * sizeof(float);
   ctx->have_phys = true;
   return GH_OK;
 }
@@ -1070,15 +1160,16 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   if (!n_histories) FAIL(GH_EINVAL, "gh_run: n_histories is null");
   HIPCHK(hipSetDevice(ctx->device));
   const int ns = (int)ctx->sims.size();
-  gh_launch lp{256, 0, 16, 4096};
+  gh_launch lp{256, 0, 64, 4096, 16, 16};
   if (launch) {
     if (launch->block > 0) lp.block = launch->block;
     if (launch->grid > 0) lp.grid = launch->grid;
     if (launch->refill_interval > 0) lp.refill_interval = launch->refill_interval;
     if (launch->chunk > 0) lp.chunk = launch->chunk;
+    if (launch->slow_min > 0) lp.slow_min = launch->slow_min;
+    if (launch->refill_min > 0) lp.refill_min = launch->refill_min;
   }
-  if (lp.block % 64 || lp.block > 256) FAIL(GH_EINVAL, "gh_run: block must be a multiple of 64 and <= 256");
-  if (lp.grid <= 0) lp.grid = std::max(1, ctx->n_cu) * 8;
+
   std::vector<uint64_t> offs(ns + 1, 0);
   std::vector<SimDev> devs(ns);
   for (int s = 0; s < ns; ++s) {
@@ -1125,21 +1216,58 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   L.chunk = (uint32_t)lp.chunk;
   L.next_chunk = ctx->d_next;
   L.seed = seed;
-  L.refill = lp.refill_interval;
+  L.max_iter = lp.refill_interval;
+  L.slow_min = std::max(1, std::min(64, lp.slow_min));
+  L.refill_min = std::max(1, std::min(L.slow_min, lp.refill_min));
   L.P = ctx->P;
   L.counters = ctx->d_counters;
   L.energies = ctx->d_energies;
-  const bool use_lds = ctx->lds_bytes <= 60 * 1024;
-  const size_t shmem = use_lds ? ctx->lds_bytes : 0;
+  // staging level and block: all tables in LDS shared by one 1024-thread workgroup per CU
+  // when they fit, else mu + majorant in LDS with 256-thread workgroups, else none.
+  const size_t budget_big = 150 * 1024, budget_small = 36 * 1024;
+  int stage, big;
+  if (launch && launch->block > 0) {
+    big = lp.block > 256;
+    if (big) stage = ctx->lds2_bytes <= budget_big ? 2 : (ctx->lds1_bytes <= budget_big ? 1 : 0);
+    else stage = ctx->lds1_bytes <= budget_small ? 1 : 0;
+    if (big && stage == 0) FAIL(GH_ERANGE, "gh_run: tables too large for a 1024-thread launch; use block <= 256");
+  } else if (ns == 1 && ctx->lds2_bytes <= budget_big) {
+    // (the batched kernel needs more registers than a 1024-thread workgroup allows)
+    stage = 2; big = 1;
+  } else if (ctx->lds1_bytes <= budget_small) {
+    stage = 1; big = 0;
+  } else if (ns == 1 && ctx->lds1_bytes <= budget_big) {
+    stage = 1; big = 1;
+  } else {
+    stage = 0; big = 0;
+  }
+  if (big) lp.block = 1024;
+  if (lp.block % 64 || lp.block > 1024 || (!big && lp.block > 256))
+    FAIL(GH_EINVAL, "gh_run: block must be a multiple of 64, <= 256, or 1024");
+  const size_t shmem = stage == 2 ? ctx->lds2_bytes : (stage == 1 ? ctx->lds1_bytes : 0);
+  const bool batched = ns > 1;
+  const void *fn = nullptr;
+#define GH_PICK(B, S, BL) (const void *)&transport_kernel<B, S, BL>
+  if (!batched) {
+    fn = big ? (stage == 2 ? GH_PICK(false, 2, 1024) : GH_PICK(false, 1, 1024))
+             : (stage == 1 ? GH_PICK(false, 1, 256) : GH_PICK(false, 0, 256));
+  } else {
+    fn = big ? (stage == 2 ? GH_PICK(true, 2, 1024) : GH_PICK(true, 1, 1024))
+             : (stage == 1 ? GH_PICK(true, 1, 256) : GH_PICK(true, 0, 256));
+  }
+#undef GH_PICK
+  if (shmem > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  if (lp.grid <= 0) {
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, lp.block, shmem));
+    lp.grid = std::max(1, ctx->n_cu) * std::max(1, per_cu);
+  }
+  ctx->last_launch = lp;
+  ctx->last_stage = stage;
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
   if (L.total_chunks > 0) {
-    if (ns == 1) {
-      if (use_lds) hipLaunchKernelGGL((transport_kernel<false, true>), dim3(lp.grid), dim3(lp.block), shmem, ctx->stream, L);
-      else hipLaunchKernelGGL((transport_kernel<false, false>), dim3(lp.grid), dim3(lp.block), 0, ctx->stream, L);
-    } else {
-      if (use_lds) hipLaunchKernelGGL((transport_kernel<true, true>), dim3(lp.grid), dim3(lp.block), shmem, ctx->stream, L);
-      else hipLaunchKernelGGL((transport_kernel<true, false>), dim3(lp.grid), dim3(lp.block), 0, ctx->stream, L);
-    }
+    void *args[] = {&L};
+    HIPCHK(hipLaunchKernel(fn, dim3(lp.grid), dim3(lp.block), args, shmem, ctx->stream));
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));

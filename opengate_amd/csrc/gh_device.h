@@ -19,6 +19,11 @@ constexpr int KBITS = 6;
 constexpr int ESHIFT = 23 - KBITS;                    // 17
 constexpr uint32_t EMASK = (1u << ESHIFT) - 1u;
 constexpr int BUCKET_BASE = ((127 - 10) << 23) >> ESHIFT;  // 7488: E_LO = 2^-10 MeV
+// q^2 grid of the form-factor tables: node 0 at 0, node j >= 1 at the float with bits
+// (X_BUCKET_BASE + j - 1) << XSHIFT (8 nodes per octave from 2^-12 A^-2)
+constexpr int XSHIFT = 20;
+constexpr uint32_t XMASK = (1u << XSHIFT) - 1u;
+constexpr int X_BUCKET_BASE = ((127 - 12) << 23) >> XSHIFT;  // 920
 constexpr float MEC2 = 0x1.05a1a8p-1f;                // 0.51099895 MeV
 constexpr float KPM = 0x1.fac5e8p+8f;                 // photon wave number per MeV [1/Angstrom]
 constexpr float EXIT_EPS = 1e-3f;                     // shortest in-box chord that counts as a hit [mm]
@@ -196,21 +201,28 @@ GH_DEV float lerp_node(const float *row, const ECache &c) {
   return fmaf(c.frac, v1 - v0, v0);
 }
 
-// value of a table on the q^2 grid at x (linear, clamped to the grid)
-GH_DEV float xgrid_lerp(const float *xs, const float *ys, int n, float x) {
-  const int i = bsearch_le(xs, n, x);
-  const float x0 = xs[i], x1 = xs[i + 1];
-  float f = (x - x0) / (x1 - x0);
-  f = fminf(fmaxf(f, 0.0f), 1.0f);
-  const float y0 = ys[i], y1 = ys[i + 1];
+// value of a table on the q^2 grid at x >= 0: the interval comes from the bits of x
+// (no search), the position inside it from the low mantissa bits; clamped at the top
+GH_DEV float qgrid_value(const float *ys, int n, float x) {
+  int j;
+  float f;
+  if (x < 0x1p-12f) {
+    j = 0;
+    f = x * 0x1p12f;
+  } else {
+    const uint32_t bits = __float_as_uint(x);
+    j = (int)(bits >> XSHIFT) - X_BUCKET_BASE + 1;
+    f = (float)(bits & XMASK) * 0x1p-20f;
+  }
+  if (j > n - 2) { j = n - 2; f = 1.0f; }
+  const float y0 = ys[j], y1 = ys[j + 1];
   return fmaf(f, y1 - y0, y0);
 }
 
 struct Phys {
   const float *mu;       // [M][NB+1] (global copy)
   const float *maj;      // [NB+1]
-  const float *pc;       // [M][NB+1]
-  const float *pr;       // [M][NB+1]
+  const float *pcpr;     // [M][NB+1][2] (sigma_C/mu, (sigma_C+sigma_R)/mu) interleaved
   const float *muen;     // [M][NB+1]
   const float *rx;       // [NX]
   const float *rcdf;     // [M][NX]
@@ -220,9 +232,22 @@ struct Phys {
   float e_cut;
 };
 
+// Table pointers a kernel reads through (LDS copies or the global arrays)
+struct Tabs {
+  const float *mu, *maj, *pcpr, *rx, *rcdf, *sinc;
+};
+
+// branching ratios at a collision: (sigma_C/mu, (sigma_C+sigma_R)/mu) interpolated at E
+GH_DEV void branching(const Tabs &T, int nb1, int lab, const ECache &c, float &pc, float &pr) {
+  const float2 a = *(const float2 *)(T.pcpr + 2 * (lab * nb1 + c.b));
+  const float2 b = *(const float2 *)(T.pcpr + 2 * (lab * nb1 + c.b + 1));
+  pc = fmaf(c.frac, b.x - a.x, a.x);
+  pr = fmaf(c.frac, b.y - a.y, a.y);
+}
+
 // Klein-Nishina by Kahn's rejection method (H. Kahn, "Applications of Monte Carlo",
 // RM-1237-AEC, 1956) followed by acceptance with S(q)/Z. Returns q = E/E' and cos(theta).
-GH_DEV void sample_compton(Rng &g, const Phys &P, int lab, float E, float &q_out, float &c_out,
+GH_DEV void sample_compton(Rng &g, const Tabs &T, int nx, int lab, float E, float &q_out, float &c_out,
                            uint32_t &rejections) {
   const float alpha = E / MEC2;
   const float a2 = alpha + alpha;
@@ -230,7 +255,7 @@ GH_DEV void sample_compton(Rng &g, const Phys &P, int lab, float E, float &q_out
   const float thr = onep / (9.0f + a2);
   const float kk = E * KPM;
   const float two_k2 = (2.0f * kk) * kk;
-  const float *srow = P.sinc + (size_t)lab * P.nx;
+  const float *srow = T.sinc + lab * nx;
   float q = 1.0f, c = 1.0f;
   int it = 0;
   for (; it < MAX_REJECT; ++it) {
@@ -248,7 +273,7 @@ GH_DEV void sample_compton(Rng &g, const Phys &P, int lab, float E, float &q_out
     if (!acc) { rejections++; continue; }
     c = 1.0f - (q - 1.0f) / alpha;
     const float x = two_k2 * (1.0f - c);
-    const float s = xgrid_lerp(P.rx, srow, P.nx, x);
+    const float s = qgrid_value(srow, nx, x);
     if (u01(g) <= s) break;
     rejections++;
   }
@@ -259,19 +284,19 @@ GH_DEV void sample_compton(Rng &g, const Phys &P, int lab, float E, float &q_out
 
 // Rayleigh: q^2 from the tabulated int F^2 (inverse by linear interpolation), then
 // acceptance with (1 + cos^2)/2.
-GH_DEV float sample_rayleigh(Rng &g, const Phys &P, int lab, float E) {
+GH_DEV float sample_rayleigh(Rng &g, const Tabs &T, int nx, int lab, float E) {
   const float kk = E * KPM;
   const float two_k2 = (2.0f * kk) * kk;
   const float xmax = two_k2 + two_k2;
-  const float *G = P.rcdf + (size_t)lab * P.nx;
-  const float gmax = xgrid_lerp(P.rx, G, P.nx, xmax);
+  const float *G = T.rcdf + lab * nx;
+  const float gmax = qgrid_value(G, nx, xmax);
   float c = 1.0f;
   for (int it = 0; it < MAX_REJECT; ++it) {
     const float t = u01(g) * gmax;
-    const int j = bsearch_le(G, P.nx, t);
+    const int j = bsearch_le(G, nx, t);
     const float g0 = G[j], g1 = G[j + 1];
     const float dg = g1 - g0;
-    const float x0 = P.rx[j], x1 = P.rx[j + 1];
+    const float x0 = T.rx[j], x1 = T.rx[j + 1];
     const float x = (dg > 0.0f) ? fmaf((t - g0) / dg, x1 - x0, x0) : x0;
     c = fmaxf(1.0f - x / two_k2, -1.0f);
     if (u01(g) <= 0.5f * fmaf(c, c, 1.0f)) break;

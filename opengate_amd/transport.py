@@ -282,12 +282,13 @@ class GpuEngine:
     def reset_tally(self, sim_id):
         self._check(self.lib.gh_reset_tally(self.ctx, int(sim_id)), "gh_reset_tally")
 
-    def run(self, n_histories, seed, first_history=None, block=256, grid=0, refill_interval=16, chunk=4096):
+    def run(self, n_histories, seed, first_history=None, block=256, grid=0, refill_interval=64, chunk=4096,
+            slow_min=16, refill_min=16):
         n = np.ascontiguousarray(np.atleast_1d(n_histories), dtype=np.uint64)
         if len(n) != len(self.sims):
             raise GhError("one history count per simulation is required")
         f = np.zeros_like(n) if first_history is None else np.ascontiguousarray(np.atleast_1d(first_history), dtype=np.uint64)
-        lp = A.Launch(int(block), int(grid), int(refill_interval), int(chunk))
+        lp = A.Launch(int(block), int(grid), int(refill_interval), int(chunk), int(slow_min), int(refill_min))
         out = A.Counters()
         self._check(self.lib.gh_run(self.ctx, n.ctypes.data_as(C.POINTER(C.c_uint64)),
                                     f.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_uint64(int(seed)),

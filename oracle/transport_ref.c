@@ -199,11 +199,26 @@ static int first_above(const float *cdf, int n, float u) {
   return lo;
 }
 
-static float grid_value(const float *xs, const float *ys, int n, float x) {
-  int i = floor_index(xs, n, x);
-  float f = (x - xs[i]) / (xs[i + 1] - xs[i]);
-  f = fminf(fmaxf(f, 0.0f), 1.0f);
-  return fmaf(f, ys[i + 1] - ys[i], ys[i]);
+/* value of a table on the q^2 grid at x >= 0 (node 0 at 0, node j >= 1 at the float
+ * with bits (920 + j - 1) << 20): interval from the bits of x, position from the low
+ * 20 mantissa bits, clamped at the top node */
+static float qgrid(const float *ys, int n, float x) {
+  int j;
+  float f;
+  if (x < 0x1p-12f) {
+    j = 0;
+    f = x * 0x1p12f;
+  } else {
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    j = (int)(b >> 20) - 920 + 1;
+    f = (float)(b & 0xFFFFFu) * 0x1p-20f;
+  }
+  if (j > n - 2) {
+    j = n - 2;
+    f = 1.0f;
+  }
+  return fmaf(f, ys[j + 1] - ys[j], ys[j]);
 }
 
 /* ------------------------------------------------------------------ physics view */
@@ -259,7 +274,7 @@ static void compton_sample(stream_t *s, const phys_t *P, int m, float E, float *
     if (!ok) { (*rej)++; continue; }
     c = 1.0f - (q - 1.0f) / alpha;
     float x = two_k2 * (1.0f - c);
-    float sv = grid_value(d->ray_x, srow, d->n_x_nodes, x);
+    float sv = qgrid(srow, d->n_x_nodes, x);
     if (uni(s) <= sv) break;
     (*rej)++;
   }
@@ -275,7 +290,7 @@ static float rayleigh_sample(stream_t *s, const phys_t *P, int m, float E) {
   float xmax = two_k2 + two_k2;
   const float *G = d->ray_cdf + (size_t)m * d->n_x_nodes;
   const float *X = d->ray_x;
-  float gmax = grid_value(X, G, d->n_x_nodes, xmax);
+  float gmax = qgrid(G, d->n_x_nodes, xmax);
   float c = 1.0f;
   for (int it = 0; it < CAP; ++it) {
     float t = uni(s) * gmax;
@@ -418,8 +433,9 @@ static void box_hit(const sim_t *S, const float p[3], const float d[3], float *t
 static float world_distance(const sim_t *S, const float p[3], const float d[3]) {
   float t = INFINITY;
   for (int i = 0; i < 3; ++i) {
-    if (d[i] > 0.0f) t = fminf(t, (S->ph->world_hi[i] - p[i]) / d[i]);
-    else if (d[i] < 0.0f) t = fminf(t, (S->ph->world_lo[i] - p[i]) / d[i]);
+    float inv = 1.0f / d[i];
+    if (d[i] > 0.0f) t = fminf(t, (S->ph->world_hi[i] - p[i]) * inv);
+    else if (d[i] < 0.0f) t = fminf(t, (S->ph->world_lo[i] - p[i]) * inv);
   }
   return fmaxf(t, 0.0f);
 }
@@ -553,18 +569,34 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
   eidx_t ex;
   energy_index(P, ph.E, &ex);
   const int wm = S->ph->world_material;
+  /* Woodcock steps draw from a separate "step stream" of the history: one Philox
+   * block (counter word 3 = 0x80000000 | step block) serves two tentative steps;
+   * its last two words are unused when the first step collides or leaves the box.
+   * Everything else (source, interactions, world) uses the event stream rs. */
+  const uint32_t hid = (uint32_t)(first + h), hid_hi = (uint32_t)((first + h) >> 32);
+  uint32_t step_block = 0;
   for (;;) {
     if (in_box) {
-      float step = -ref_log(uni(&rs)) * ex.inv_maj;
-      float q[3];
-      for (int i = 0; i < 3; ++i) q[i] = fmaf(step, ph.d[i], ph.p[i]);
-      int vox = voxel_of(S, q);
-      if (vox >= 0) {
+      uint32_t w[4];
+      const uint32_t sctr[4] = {hid, hid_hi, sim_id, 0x80000000u | step_block};
+      ref_philox4x32_10(sctr, rs.key, w);
+      step_block++;
+      int outcome = 0; /* 0: both steps virtual, 1: real collision, 2: leaves the box */
+      int vox = -1, lab = 0;
+      for (int k = 0; k < 2; ++k) {
+        float step = -ref_log((float)((w[2 * k] >> 8) | 1u) * 0x1p-24f) * ex.inv_maj;
+        float q[3];
+        for (int i = 0; i < 3; ++i) q[i] = fmaf(step, ph.d[i], ph.p[i]);
+        vox = voxel_of(S, q);
+        if (vox < 0) { outcome = 2; break; }
         ph.p[0] = q[0]; ph.p[1] = q[1]; ph.p[2] = q[2];
         st->c[K_STEPS]++;
-        int lab = S->ph->labels[vox];
+        lab = S->ph->labels[vox];
         float mu = node_interp(d->mu_total + (size_t)lab * P->nb1, &ex);
-        if (!(uni(&rs) * ex.maj < mu)) continue;
+        if ((float)((w[2 * k + 1] >> 8) | 1u) * 0x1p-24f * ex.maj < mu) { outcome = 1; break; }
+      }
+      if (outcome == 0) continue;
+      if (outcome == 1) {
         float r = uni(&rs);
         float pc = node_interp(d->p_compton + (size_t)lab * P->nb1, &ex);
         float pr = node_interp(d->p_rayleigh + (size_t)lab * P->nb1, &ex);
