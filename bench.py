@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--slow-min", type=int, default=16)
     ap.add_argument("--refill-min", type=int, default=16)
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--workload", default="ct256", choices=("ct256", "batch64"))
+    ap.add_argument("--autotune", action="store_true", help="pick launch parameters with gh_autotune first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy x5 (reference kernel for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -89,6 +91,8 @@ def main():
     from opengate_amd import workloads
     from opengate_amd.transport import GpuEngine
 
+    if args.workload == "batch64":
+        return bench_batch(args)
     eng = GpuEngine(0)
     tables, ph, src, tal, info = workloads.ct256(eng)
     eng.set_physics(tables)
@@ -96,6 +100,8 @@ def main():
     n = int(args.histories)
     launch = {"refill_interval": args.refill, "grid": args.grid, "slow_min": args.slow_min,
               "refill_min": args.refill_min}
+    if args.autotune:
+        launch, _ = eng.autotune([min(n, 20_000_000)])
     try:
         import torch
 
@@ -152,6 +158,47 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(tables, ph, src, tal)
+    eng.close()
+    print(json.dumps(line), flush=True)
+
+
+def bench_batch(args):
+    """64 simulations: one persistent batched launch vs one launch per simulation."""
+    from opengate_amd import workloads
+    from opengate_amd.transport import GpuEngine
+
+    tables, sims, info = workloads.batch64()
+    hist = [h for *_, h in sims]
+    total = sum(hist)
+    # plain form: each simulation alone in the single-simulation kernel
+    eng = GpuEngine(0)
+    eng.set_physics(tables)
+    plain_ms = 0.0
+    for ph, src, tal, h in sims:
+        eng.clear()
+        eng.add_simulation(ph, src, tal)
+        c = eng.run([h], 777)
+        plain_ms += c["kernel_ms"]
+    # batched form: all simulations in one persistent launch, launch parameters autotuned
+    eng.clear()
+    for ph, src, tal, _ in sims:
+        eng.add_simulation(ph, src, tal)
+    launch, _ = eng.autotune([max(1, h // 50) for h in hist])
+    eng.run([max(1, h // 50) for h in hist], 1, **launch)  # warm-up
+    for k in range(len(sims)):
+        eng.reset_tally(k)
+    t0 = time.perf_counter()
+    c = eng.run(hist, 777, **launch)
+    wall = time.perf_counter() - t0
+    line = {
+        "metric": METRIC, "value": total / wall, "unit": "photons/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seed-generated phantoms)",
+        "config": {"workload": info["desc"], "histories_total": total, "form": "batched persistent launch",
+                   "launch": launch, "plain_separate_launches_photons_per_s": total / (plain_ms * 1e-3),
+                   "batched_kernel_photons_per_s": total / (c["kernel_ms"] * 1e-3),
+                   "batched_over_plain": (plain_ms / c["kernel_ms"])},
+    }
     eng.close()
     print(json.dumps(line), flush=True)
 

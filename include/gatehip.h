@@ -165,6 +165,13 @@ int gh_reset_tally(gh_ctx *ctx, int32_t sim_id);
 int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_history,
            uint64_t seed, const gh_launch *launch, gh_counters *out);
 
+/* Autotuner (no reference counterpart: the reference has no launch shape to tune).
+ * Runs n_trial[s] histories per simulation (RNG seed `seed`, keep it different from the
+ * production seed) for each candidate launch (workgroup shape, slow-phase threshold,
+ * refill threshold, step iterations), keeps the fastest in *best (kernel ms in *best_ms)
+ * and restores every tally and recorder count to its state before the call. */
+int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *best, double *best_ms);
+
 /* Copies a tally to host. Any pointer may be NULL. counts/dose images only if enabled. */
 int gh_get_tally(gh_ctx *ctx, int32_t sim_id, double *edep, double *edep2, double *dose,
                  double *dose2, double *counts);
@@ -177,6 +184,19 @@ int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n);
  * (numpy.digitize with ascending bins, right=False). */
 int gh_hu_to_labels(gh_ctx *ctx, const int16_t *hu, int64_t n_voxels, const double *bins,
                     int32_t n_bins, const uint16_t *labels_sorted, uint16_t *out);
+
+/* ---- dense-layer GEMM for the learned models (no reference counterpart: the reference
+ * runs its GAN source and ARF networks through torch, gansources.py:512, arfactors.py:388).
+ * C[M,N] = A[M,K] * B[K,N] in bf16 with FP32 accumulation on the matrix cores;
+ * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]; C row pitch ldc. Device pointers,
+ * `stream` is a hipStream_t (NULL = default stream). Epilogue modes: */
+enum { GH_EPI_STORE = 0,      /* C32 = acc */
+       GH_EPI_BIAS_SILU = 1,  /* C32 = acc + bias (optional), C16 = silu(acc + bias) */
+       GH_EPI_BIAS = 2,       /* C32 = acc + bias and/or C16 = bf16(acc + bias) */
+       GH_EPI_DSILU = 3 };    /* C32 = acc * silu'(zprev) (optional), C16 = bf16(same) */
+int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
+                 int32_t M, int32_t N, int32_t K, const float *bias, const float *zprev, int64_t ldz,
+                 float *C32, void *C16, int64_t ldc, int32_t mode, void *stream);
 
 /* Device properties used by the autotuner and reports. */
 int gh_device_info(gh_ctx *ctx, int32_t *n_cu, int32_t *lds_bytes, char *name, int32_t name_len);

@@ -112,12 +112,17 @@ SIGNATURES = {
     "gh_reset_tally": (C.c_int, [C.c_void_p, C.c_int32]),
     "gh_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint64,
                          C.POINTER(Launch), C.POINTER(Counters)]),
+    "gh_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(Launch),
+                              C.POINTER(C.c_double)]),
     "gh_get_tally": (C.c_int, [C.c_void_p, C.c_int32, DP, DP, DP, DP, DP]),
     "gh_get_phsp": (C.c_int, [C.c_void_p, C.c_int32, FP * 8, C.c_uint64, C.POINTER(C.c_uint64)]),
     "gh_get_event_edep": (C.c_int, [C.c_void_p, C.c_int32, FP, C.c_uint64]),
     "gh_hu_to_labels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int16), C.c_int64, DP, C.c_int32,
                                   C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]),
     "gh_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
+    "gh_gemm_bf16": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_int32,
+                               C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                               C.c_int64, C.c_int32, C.c_void_p]),
 }
 
 _LIB = None
@@ -129,6 +134,13 @@ def load_library(path=None):
     if _LIB is not None and path is None:
         return _LIB
     p = Path(path) if path else LIB_PATH
+    # One HIP runtime per process: torch bundles its own libamdhip64.so.7 / HSA runtime.
+    # Loading torch first makes the dynamic linker resolve this library's libamdhip64.so.7
+    # to torch's copy (same SONAME), so tensors and this library share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not p.exists():
         raise RuntimeError(
             f"{p} not found: the MI355X transport library is not built. "

@@ -1288,6 +1288,72 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   return GH_OK;
 }
 
+int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *best, double *best_ms) {
+  if (!ctx) return GH_EINVAL;
+  if (!ctx->have_phys || ctx->sims.empty()) FAIL(GH_ESTATE, "gh_autotune: physics and simulations required");
+  if (!n_trial || !best) FAIL(GH_EINVAL, "gh_autotune: null argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  // snapshot every tally so that trial histories leave no trace
+  std::vector<std::pair<double *, double *>> saved;
+  std::vector<unsigned long long> rec_counts;
+  std::vector<void *> tmp;
+  for (auto &h : ctx->sims) {
+    double *arrs[5] = {h.dev.edep, h.dev.edep2, h.dev.dose, h.dev.dose2, h.dev.counts};
+    for (double *a : arrs) {
+      if (!a) continue;
+      void *b = nullptr;
+      HIPCHK(hipMalloc(&b, h.n_tally * sizeof(double)));
+      tmp.push_back(b);
+      HIPCHK(hipMemcpyAsync(b, a, h.n_tally * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      saved.push_back({a, (double *)b});
+    }
+    unsigned long long rc = 0;
+    if (h.dev.rec_count) HIPCHK(hipMemcpy(&rc, h.dev.rec_count, sizeof(rc), hipMemcpyDeviceToHost));
+    rec_counts.push_back(rc);
+  }
+  // candidates: workgroup shape (automatic staging / 256 threads) x slow-phase and
+  // refill thresholds x step iterations
+  struct Cand { int block, slow, refill, iter; };
+  std::vector<Cand> cands;
+  for (int block : {0, 256})
+    for (int slow : {8, 16, 32, 48, 56})
+      for (int iter : {16, 64})
+        cands.push_back({block, slow, slow == 56 ? 48 : slow, iter});
+  double best_t = 1e300;
+  gh_launch best_lp{};
+  int rc = GH_OK;
+  std::vector<uint64_t> first(ctx->sims.size(), 0);
+  for (const Cand &c : cands) {
+    gh_launch lp{c.block, 0, c.iter, 4096, c.slow, c.refill};
+    gh_counters out{};
+    rc = gh_run(ctx, n_trial, first.data(), seed, &lp, &out);
+    if (rc != GH_OK) break;
+    if (out.kernel_ms < best_t) {
+      best_t = out.kernel_ms;
+      best_lp = ctx->last_launch;
+      best_lp.block = c.block;
+      best_lp.grid = 0;
+    }
+  }
+  for (auto &pr : saved) {
+    size_t n = 0;
+    for (auto &h : ctx->sims)
+      for (double *a : {h.dev.edep, h.dev.edep2, h.dev.dose, h.dev.dose2, h.dev.counts})
+        if (a == pr.first) n = h.n_tally;
+    (void)hipMemcpyAsync(pr.first, pr.second, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+  }
+  for (size_t s = 0; s < ctx->sims.size(); ++s)
+    if (ctx->sims[s].dev.rec_count)
+      (void)hipMemcpyAsync(ctx->sims[s].dev.rec_count, &rec_counts[s], sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_all(tmp);
+  if (rc != GH_OK) return rc;
+  *best = best_lp;
+  if (best_ms) *best_ms = best_t;
+  return GH_OK;
+}
+
 int gh_get_tally(gh_ctx *ctx, int32_t sim_id, double *edep, double *edep2, double *dose, double *dose2,
                  double *counts) {
   if (!ctx) return GH_EINVAL;

@@ -1,0 +1,151 @@
+"""Dense MLP trained and evaluated with the library's BF16 MFMA GEMM (gh_gemm_bf16).
+
+Every matrix product of the forward and backward passes runs in the hand-written
+kernel (csrc/mlp.hip) with FP32 accumulation; bias, SiLU and the SiLU derivative are
+fused into its epilogues. PyTorch holds the tensors (device memory, the FP32 master
+weights, the Adam state and the loss), nothing more.
+
+Layer l: Z_l = X_l W_l + b_l (FP32), X_{l+1} = bf16(silu(Z_l)) for hidden layers,
+the last layer is linear. Backward: G_L = dLoss/dZ_L, dW_l = X_l^T G_l,
+db_l = sum_m G_l, G_{l-1} = (G_l W_l^T) * silu'(Z_{l-1}).
+"""
+import ctypes as C
+
+import numpy as np
+
+from .. import _abi as A
+
+EPI_STORE, EPI_BIAS_SILU, EPI_BIAS, EPI_DSILU = 0, 1, 2, 3
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def pad8(n):
+    return (n + 7) // 8 * 8
+
+
+class MLP:
+    """sizes = [in, hidden..., out]; inputs/outputs padded to multiples of 8 internally."""
+
+    def __init__(self, sizes, device="cuda", seed=0, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, weight_decay=0.0):
+        torch = _torch()
+        self.lib = A.load_library()
+        self.sizes = list(sizes)
+        self.dev = torch.device(device)
+        self.psizes = [pad8(s) for s in sizes]
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        self.W, self.b = [], []
+        for k, n, pk, pn in zip(self.sizes[:-1], self.sizes[1:], self.psizes[:-1], self.psizes[1:]):
+            w = torch.zeros(pk, pn)
+            w[:k, :n] = torch.randn(k, n, generator=g) * np.sqrt(2.0 / k)
+            self.W.append(w.to(self.dev))
+            self.b.append(torch.zeros(pn, device=self.dev))
+        self.lr, self.betas, self.eps, self.wd = lr, betas, eps, weight_decay
+        self.m = [torch.zeros_like(t) for t in self.W + self.b]
+        self.v = [torch.zeros_like(t) for t in self.W + self.b]
+        self.t = 0
+        self._refresh()
+
+    # ------------------------------------------------------------------ helpers
+    def _refresh(self):
+        self.W16 = [w.to(_torch().bfloat16).contiguous() for w in self.W]
+
+    def _stream(self):
+        return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+    def _gemm(self, a, sam, sak, b, sbk, sbn, M, N, K, bias=None, zprev=None, ldz=0, c32=None, c16=None, ldc=0,
+              mode=EPI_STORE):
+        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p()  # noqa: E731
+        rc = self.lib.gh_gemm_bf16(p(a), sam, sak, p(b), sbk, sbn, M, N, K, p(bias), p(zprev), ldz, p(c32), p(c16),
+                                   ldc, mode, self._stream())
+        if rc != 0:
+            raise RuntimeError(f"gh_gemm_bf16 failed ({rc})")
+
+    def n_params(self):
+        return sum(int(np.prod(s)) for s in zip(self.sizes[:-1], self.sizes[1:])) + sum(self.sizes[1:])
+
+    def flops_forward(self, batch):
+        return 2.0 * batch * sum(k * n for k, n in zip(self.sizes[:-1], self.sizes[1:]))
+
+    # ------------------------------------------------------------------ passes
+    def forward(self, x, keep=True):
+        """x: (M, in) float tensor on the device. Returns (M, out) FP32."""
+        torch = _torch()
+        M = x.shape[0]
+        X = torch.zeros(M, self.psizes[0], device=self.dev, dtype=torch.bfloat16)
+        X[:, : self.sizes[0]] = x.to(torch.bfloat16)
+        xs, zs = [X], []
+        L = len(self.W)
+        for l in range(L):
+            K, N = self.psizes[l], self.psizes[l + 1]
+            Z = torch.empty(M, N, device=self.dev, dtype=torch.float32)
+            if l < L - 1:
+                Y = torch.empty(M, N, device=self.dev, dtype=torch.bfloat16)
+                self._gemm(xs[-1], K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c32=Z, c16=Y, ldc=N,
+                           mode=EPI_BIAS_SILU)
+                xs.append(Y)
+            else:
+                self._gemm(xs[-1], K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c32=Z, ldc=N, mode=EPI_BIAS)
+            zs.append(Z)
+        if keep:
+            self._xs, self._zs = xs, zs
+        return zs[-1][:, : self.sizes[-1]]
+
+    def backward(self, grad_out):
+        """grad_out: dLoss/dOutput (M, out) FP32. Returns [dW...], [db...] (FP32, padded)."""
+        torch = _torch()
+        M = grad_out.shape[0]
+        L = len(self.W)
+        G32 = torch.zeros(M, self.psizes[-1], device=self.dev, dtype=torch.float32)
+        G32[:, : self.sizes[-1]] = grad_out
+        G16 = G32.to(torch.bfloat16)
+        dW, db = [None] * L, [None] * L
+        for l in range(L - 1, -1, -1):
+            K, N = self.psizes[l], self.psizes[l + 1]
+            dw = torch.empty(K, N, device=self.dev, dtype=torch.float32)
+            # dW = X^T G : (K x M) (M x N)
+            self._gemm(self._xs[l], 1, K, G16, N, 1, K, N, M, c32=dw, ldc=N, mode=EPI_STORE)
+            dW[l] = dw
+            db[l] = G32.sum(0)
+            if l > 0:
+                Gp32 = torch.empty(M, K, device=self.dev, dtype=torch.float32)
+                Gp16 = torch.empty(M, K, device=self.dev, dtype=torch.bfloat16)
+                # G_{l-1} = (G W^T) * silu'(Z_{l-1}) : (M x N) (N x K)
+                self._gemm(G16, N, 1, self.W16[l], 1, N, M, K, N, zprev=self._zs[l - 1], ldz=K, c32=Gp32, c16=Gp16,
+                           ldc=K, mode=EPI_DSILU)
+                G32, G16 = Gp32, Gp16
+        return dW, db
+
+    def step(self, dW, db):
+        """Adam update of the FP32 master weights (padding kept at zero)."""
+        torch = _torch()
+        self.t += 1
+        b1, b2 = self.betas
+        params = self.W + self.b
+        grads = dW + db
+        for i, (p, g) in enumerate(zip(params, grads)):
+            if self.wd and i < len(self.W):
+                g = g + self.wd * p
+            self.m[i].mul_(b1).add_(g, alpha=1 - b1)
+            self.v[i].mul_(b2).addcmul_(g, g, value=1 - b2)
+            mh = self.m[i] / (1 - b1**self.t)
+            vh = self.v[i] / (1 - b2**self.t)
+            p.sub_(self.lr * mh / (vh.sqrt() + self.eps))
+        for w, k, n in zip(self.W, self.sizes[:-1], self.sizes[1:]):
+            w[k:, :] = 0
+            w[:, n:] = 0
+        for bb, n in zip(self.b, self.sizes[1:]):
+            bb[n:] = 0
+        self._refresh()
+
+    def state_dict(self):
+        return {"sizes": self.sizes, "W": [w.cpu() for w in self.W], "b": [b.cpu() for b in self.b]}
+
+    def load_state_dict(self, d):
+        self.W = [w.to(self.dev) for w in d["W"]]
+        self.b = [b.to(self.dev) for b in d["b"]]
+        self._refresh()

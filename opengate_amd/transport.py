@@ -295,6 +295,20 @@ class GpuEngine:
                                     C.byref(lp), C.byref(out)), "gh_run")
         return out.as_dict()
 
+    def autotune(self, n_trial, seed=0x5EED_A070):
+        """Best launch parameters for the current simulations (tallies are left untouched).
+        Returns (kwargs for run(), kernel ms of the best trial)."""
+        n = np.ascontiguousarray(np.atleast_1d(n_trial), dtype=np.uint64)
+        if len(n) != len(self.sims):
+            raise GhError("one trial history count per simulation is required")
+        best = A.Launch()
+        ms = C.c_double()
+        self._check(self.lib.gh_autotune(self.ctx, n.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_uint64(int(seed)),
+                                         C.byref(best), C.byref(ms)), "gh_autotune")
+        kw = {"block": best.block, "grid": best.grid, "refill_interval": best.refill_interval, "chunk": best.chunk,
+              "slow_min": best.slow_min, "refill_min": best.refill_min}
+        return kw, ms.value
+
     def get_tally(self, sim_id=0):
         _, _, tal = self.sims[sim_id]
         nz_ny_nx = tuple(reversed([int(x) for x in tal.size]))

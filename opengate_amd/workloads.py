@@ -13,7 +13,7 @@ import numpy as np
 from . import phantoms as P
 from .physics import xs
 from .physics.materials import MaterialDatabase
-from .transport import PhantomSpec, SourceSpec, TallySpec, spectrum_histogram
+from .transport import PhantomSpec, SourceSpec, TallySpec, spectrum_histogram, spectrum_lines
 
 CT256_DESC = ("ct256: 256x256x128 HU phantom, 1 mm voxels, 12 materials + air world; 120 kVp Kramers "
               "spectrum (2.5 mm Al, 1 keV bins), 15 deg half-angle cone from 600 mm; dose on the 256x256x128 grid")
@@ -53,3 +53,27 @@ def water64(seed=1):
     src = SourceSpec(pos_trans=(64.0, 64.0, 64.0), energy=0.1405)
     tal = TallySpec(size=(64, 64, 64), spacing=(2.0, 2.0, 2.0))
     return tables, ph, src, tal, {"desc": WATER64_DESC, "spheres": spheres}
+
+
+BATCH64_DESC = ("batch64: 64 simulations, cubes of 64^3..192^3 voxels at 1.5-3 mm (water ellipsoid in air with "
+                "1-6 bone/lung/soft-tissue inserts), ellipsoid volume sources with the 140.511 keV line or the "
+                "112.95/208.37 keV Lu-177 lines, 1e6..1e8 histories each (log-uniform)")
+
+
+def batch64(seed=3, n_sims=64):
+    """Returns (tables, [(phantom, source, tally, histories), ...], info)."""
+    db = MaterialDatabase()
+    sims = P.batch_simulations(seed, n_sims)
+    tables = xs.build_tables([db.get(n) for n in P.BATCH_MATERIALS], 0.2090)
+    out = []
+    for s in sims:
+        n, sp = s["n"], s["spacing"]
+        L = n * sp
+        ph = PhantomSpec(s["labels"], (sp, sp, sp), world_material=-1)
+        ev, _, cdf = spectrum_lines(s["spectrum"]["energies"], s["spectrum"]["weights"])
+        src = SourceSpec(pos_type="ellipsoid", pos_param=tuple(s["source_half"]) + (0.0,),
+                         pos_trans=(L / 2, L / 2, L / 2), ene_type="spectrum_discrete", energy_value=ev,
+                         energy_cdf=cdf)
+        tal = TallySpec(size=(n, n, n), spacing=(sp, sp, sp))
+        out.append((ph, src, tal, s["histories"]))
+    return tables, out, {"desc": BATCH64_DESC}

@@ -1,0 +1,103 @@
+"""The BF16 MFMA GEMM and the MLP trained with it, against FP64 references (GPU)."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 70, 40), (257, 256, 256), (33, 8, 8), (96, 6, 3)])
+def test_gemm_modes_and_layouts(M, N, K):
+    torch = _torch()
+    from opengate_amd.learn.mlp import EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU, EPI_STORE, MLP
+
+    print(f"[gpu case] gemm M={M} N={N} K={K}", flush=True)
+    net = MLP([8, 8])  # only for the GEMM entry point
+    g = torch.Generator().manual_seed(M * 131 + N)
+    a = torch.randn(M, K, generator=g).to("cuda", torch.bfloat16)
+    bkn = torch.randn(K, N, generator=g).to("cuda", torch.bfloat16)
+    bias = torch.randn(N, generator=g).cuda()
+    zp = torch.randn(M, N, generator=g).cuda()
+    A64 = a.double().cpu().numpy()
+    B64 = bkn.double().cpu().numpy()
+    ref = A64 @ B64
+    tol = 2e-3 * (np.abs(A64) @ np.abs(B64)) + 1e-5
+    torch.cuda.synchronize()
+    # plain store, A row-major / B row-major (n contiguous)
+    c = torch.zeros(M, N, device="cuda")
+    net._gemm(a, K, 1, bkn, N, 1, M, N, K, c32=c, ldc=N, mode=EPI_STORE)
+    torch.cuda.synchronize()
+    assert np.all(np.abs(c.cpu().numpy() - ref) <= tol)
+    # transposed operands: A given as A^T (m contiguous), B given as B^T (k contiguous)
+    at = a.t().contiguous()
+    bt = bkn.t().contiguous()
+    c2 = torch.zeros(M, N, device="cuda")
+    net._gemm(at, 1, M, bt, 1, K, M, N, K, c32=c2, ldc=N, mode=EPI_STORE)
+    torch.cuda.synchronize()
+    assert np.all(np.abs(c2.cpu().numpy() - ref) <= tol)
+    # fused epilogues
+    z = torch.zeros(M, N, device="cuda")
+    y = torch.zeros(M, N, device="cuda", dtype=torch.bfloat16)
+    net._gemm(a, K, 1, bkn, N, 1, M, N, K, bias=bias, c32=z, c16=y, ldc=N, mode=EPI_BIAS_SILU)
+    torch.cuda.synchronize()
+    zr = ref + bias.double().cpu().numpy()
+    assert np.all(np.abs(z.cpu().numpy() - zr) <= tol)
+    assert np.allclose(y.double().cpu().numpy(), R.silu(zr), rtol=1e-2, atol=1e-2)
+    net._gemm(a, K, 1, bkn, N, 1, M, N, K, bias=bias, c32=z, ldc=N, mode=EPI_BIAS)
+    torch.cuda.synchronize()
+    assert np.all(np.abs(z.cpu().numpy() - zr) <= tol)
+    net._gemm(a, K, 1, bkn, N, 1, M, N, K, zprev=zp, ldz=N, c32=z, c16=y, ldc=N, mode=EPI_DSILU)
+    torch.cuda.synchronize()
+    dr = ref * R.dsilu(zp.double().cpu().numpy())
+    assert np.all(np.abs(z.cpu().numpy() - dr) <= tol * 1.2)
+
+
+def test_mlp_forward_backward_against_fp64_oracle():
+    torch = _torch()
+    from opengate_amd.learn.mlp import MLP
+
+    print("[gpu case] mlp fwd/bwd vs oracle", flush=True)
+    net = MLP([8, 64, 64, 64, 6], seed=3)
+    g = torch.Generator().manual_seed(5)
+    x = torch.randn(300, 8, generator=g).cuda()
+    out = net.forward(x)
+    gout = torch.randn(300, 6, generator=g).cuda()
+    dW, db = net.backward(gout)
+    torch.cuda.synchronize()
+    Ws = [w[:k, :n].double().cpu().numpy() for w, k, n in zip(net.W, net.sizes[:-1], net.sizes[1:])]
+    bs = [b[:n].double().cpu().numpy() for b, n in zip(net.b, net.sizes[1:])]
+    o_ref, xs, zs = R.forward(Ws, bs, x.double().cpu().numpy())
+    assert np.linalg.norm(out.double().cpu().numpy() - o_ref) <= 2e-2 * np.linalg.norm(o_ref)
+    dW_ref, db_ref = R.backward(Ws, xs, zs, gout.double().cpu().numpy())
+    for l, (k, n) in enumerate(zip(net.sizes[:-1], net.sizes[1:])):
+        gw = dW[l][:k, :n].double().cpu().numpy()
+        gb = db[l][:n].double().cpu().numpy()
+        assert np.linalg.norm(gw - dW_ref[l]) <= 3e-2 * np.linalg.norm(dW_ref[l]), l
+        assert np.linalg.norm(gb - db_ref[l]) <= 3e-2 * np.linalg.norm(db_ref[l]), l
+
+
+def test_mlp_trains_a_regression():
+    torch = _torch()
+    from opengate_amd.learn.mlp import MLP
+
+    print("[gpu case] mlp training", flush=True)
+    net = MLP([8, 128, 128, 4], seed=1, lr=3e-3)
+    g = torch.Generator().manual_seed(7)
+    W0 = torch.randn(8, 4, generator=g).cuda()
+    losses = []
+    for it in range(300):
+        x = torch.randn(1024, 8, generator=g).cuda()
+        t = torch.tanh(x @ W0)
+        y = net.forward(x)
+        loss = ((y - t) ** 2).mean()
+        dW, db = net.backward(2 * (y - t) / y.numel())
+        net.step(dW, db)
+        losses.append(float(loss))
+    assert losses[-1] < 0.1 * losses[0]
