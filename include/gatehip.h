@@ -110,7 +110,8 @@ typedef struct gh_source_desc {
   int64_t phsp_n;            /* GH_POS_PHSP: number of rows */
   int64_t phsp_start;        /* first row used by history 0 */
   int32_t phsp_iso;          /* 1: replace directions by isotropic ones */
-  int32_t reserved0;
+  int32_t phsp_on_device;    /* 1: phsp[] are device pointers, used in place: they must stay
+                                allocated until the simulation is cleared or its source replaced */
   const float *phsp[8];      /* x, y, z, dx, dy, dz, E, weight (source frame) */
 } gh_source_desc;
 
@@ -175,7 +176,8 @@ int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *
 /* Copies a tally to host. Any pointer may be NULL. counts/dose images only if enabled. */
 int gh_get_tally(gh_ctx *ctx, int32_t sim_id, double *edep, double *edep2, double *dose,
                  double *dose2, double *counts);
-/* Recorded exit-face photons: 8 float arrays of length *n (x y z dx dy dz E w, phantom frame). */
+/* Recorded exit-face photons: 8 float arrays of length *n (x y z dx dy dz E w, phantom frame).
+   out[k] may be host or device pointers (copied with the HIP runtime's default direction). */
 int gh_get_phsp(gh_ctx *ctx, int32_t sim_id, float *const out[8], uint64_t capacity, uint64_t *n);
 /* Per-history detector energy (tally.detector_label_min >= 0), n_histories floats. */
 int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n);
@@ -197,6 +199,21 @@ enum { GH_EPI_STORE = 0,      /* C32 = acc */
 int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
                  int32_t M, int32_t N, int32_t K, const float *bias, const float *zprev, int64_t ldz,
                  float *C32, void *C16, int64_t ldc, int32_t mode, void *stream);
+
+/* Learned exit-plane source (replaces the generator call of GANSourceDefaultGenerator.generator,
+ * sources/gansources.py:644-684, and its copy/move of the keys, :686-757).
+ * gh_gaussian_bf16: out[r*ld + c] = N(0,1) noise in bf16 for rows first_row .. first_row+n_rows-1
+ * (c < n_cols; columns n_cols..ld-1 are zero). Row r depends only on (seed, r). */
+int gh_gaussian_bf16(uint64_t seed, uint64_t first_row, int64_t n_rows, int32_t n_cols, int32_t ld, void *out,
+                     void *stream);
+/* Generator outputs Z (n rows, pitch ldz >= 6, FP32: E, x, y, u', v', w' in Gaussianised space)
+ * -> 8 device float arrays x y z u v w E weight. qtab: 6 x n_knots quantile values at the
+ * Gaussian knots -g_max + 2 g_max k/(n_knots-1). (u', v', w') is the direction in the frame whose
+ * pole is the ray from focal[3] through (x, y, plane_z); the output position is
+ * (x, y, plane_z) + offset[3]. */
+int gh_exitplane_decode(const float *Z, int64_t ldz, int64_t n, const float *qtab, int32_t n_knots, float g_max,
+                        const float focal[3], float plane_z, const float offset[3], float weight,
+                        float *const out[8], void *stream);
 
 /* Device properties used by the autotuner and reports. */
 int gh_device_info(gh_ctx *ctx, int32_t *n_cu, int32_t *lds_bytes, char *name, int32_t name_len);

@@ -46,7 +46,7 @@ class SourceDesc(C.Structure):
         ("momentum", C.c_float * 3), ("focus", C.c_float * 3),
         ("energy_mono", C.c_float), ("energy_max", C.c_float), ("weight", C.c_float), ("phi_full", C.c_int32),
         ("energy_value", FP), ("energy_width", FP), ("energy_cdf", FP),
-        ("phsp_n", C.c_int64), ("phsp_start", C.c_int64), ("phsp_iso", C.c_int32), ("reserved0", C.c_int32),
+        ("phsp_n", C.c_int64), ("phsp_start", C.c_int64), ("phsp_iso", C.c_int32), ("phsp_on_device", C.c_int32),
         ("phsp", FP * 8),
     ]
 
@@ -123,6 +123,10 @@ SIGNATURES = {
     "gh_gemm_bf16": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_int32,
                                C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                C.c_int64, C.c_int32, C.c_void_p]),
+    "gh_gaussian_bf16": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
+                                   C.c_void_p]),
+    "gh_exitplane_decode": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int32, C.c_float, FP,
+                                      C.c_float, FP, C.c_float, FP * 8, C.c_void_p]),
 }
 
 _LIB = None

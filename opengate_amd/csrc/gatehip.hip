@@ -710,6 +710,20 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
     unsafeAtomicAdd(L.energies + threadIdx.x, s_en[threadIdx.x]);
 }
 
+// ---------------------------------------------------------------------- phsp check
+// flags[0] |= 1 if an energy is outside (0, e_hi) or not finite, |= 2 for a bad weight
+__global__ void phsp_check_kernel(const float *__restrict__ E, const float *__restrict__ w, int64_t n, float e_hi,
+                                  unsigned int *flags) {
+  unsigned int bad = 0u;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float e = E[i];
+    if (!(e > 0.0f) || !(e < e_hi)) bad |= 1u;
+    if (!(w[i] >= 0.0f) || !isfinite(w[i])) bad |= 2u;
+  }
+  if (bad) atomicOr(flags, bad);
+}
+
 // ---------------------------------------------------------------------- HU -> labels
 __global__ void hu_to_labels_kernel(const int16_t *__restrict__ hu, int64_t n, const double *__restrict__ bins,
                                     int n_bins, const uint16_t *__restrict__ lut, uint16_t *__restrict__ out) {
@@ -832,8 +846,28 @@ int validate_source(gh_ctx *ctx, const gh_source_desc *src) {
     std::memcpy(&e_hi, &hb, 4);
     auto bad = [&](float e) { return !(e > 0.0f) || !(e < e_hi); };
     bool out_of_grid = false;
-    if (src->pos_type == GH_POS_PHSP) {
-      for (int64_t i = 0; i < src->phsp_n && !out_of_grid; ++i) out_of_grid = bad(src->phsp[6][i]);
+    if (src->pos_type == GH_POS_PHSP && src->phsp_on_device) {
+      // device arrays: checked by a kernel
+      unsigned int *flags = nullptr;
+      HIPCHK(hipMalloc(&flags, sizeof(unsigned int)));
+      unsigned int h = 0;
+      hipError_t e = hipMemset(flags, 0, sizeof(unsigned int));
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(phsp_check_kernel, dim3(std::max(1, ctx->n_cu) * 4), dim3(256), 0, 0, src->phsp[6],
+                           src->phsp[7], src->phsp_n, e_hi, flags);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipMemcpy(&h, flags, sizeof(h), hipMemcpyDeviceToHost);
+      (void)hipFree(flags);
+      if (e != hipSuccess) FAIL(GH_EHIP, std::string("gh_add_simulation: phase-space check: ") + hipGetErrorString(e));
+      if (h & 2u) FAIL(GH_EINVAL, "gh_add_simulation: phase-space weights must be finite and >= 0");
+      out_of_grid = (h & 1u) != 0;
+    } else if (src->pos_type == GH_POS_PHSP) {
+      for (int64_t i = 0; i < src->phsp_n; ++i) {
+        const float w = src->phsp[7][i];
+        if (!(w >= 0.0f) || !std::isfinite(w)) FAIL(GH_EINVAL, "gh_add_simulation: phase-space weights must be finite and >= 0");
+        out_of_grid = out_of_grid || bad(src->phsp[6][i]);
+      }
     } else if (src->ene_type == GH_ENE_MONO) {
       out_of_grid = bad(src->energy_mono);
     } else if (src->ene_type == GH_ENE_RANGE) {
@@ -889,8 +923,12 @@ int upload_source(gh_ctx *ctx, const gh_source_desc *src, SimDev &d, std::vector
   if (src->pos_type == GH_POS_PHSP) {
     d.phsp_n = src->phsp_n; d.phsp_start = src->phsp_start;
     for (int k = 0; k < 8; ++k) {
-      if ((rc = dev_upload(ctx, src->phsp[k], (size_t)src->phsp_n, &t, owned))) return rc;
-      d.phsp[k] = t;
+      if (src->phsp_on_device) {
+        d.phsp[k] = src->phsp[k];  // caller-owned device memory, used in place
+      } else {
+        if ((rc = dev_upload(ctx, src->phsp[k], (size_t)src->phsp_n, &t, owned))) return rc;
+        d.phsp[k] = t;
+      }
     }
   }
   return GH_OK;
@@ -1386,7 +1424,7 @@ int gh_get_phsp(gh_ctx *ctx, int32_t sim_id, float *const out[8], uint64_t capac
   if (out) {
     const uint64_t m = std::min<uint64_t>(stored, capacity);
     for (int k = 0; k < 8; ++k)
-      if (out[k] && m) HIPCHK(hipMemcpy(out[k], h.dev.rec[k], m * sizeof(float), hipMemcpyDeviceToHost));
+      if (out[k] && m) HIPCHK(hipMemcpy(out[k], h.dev.rec[k], m * sizeof(float), hipMemcpyDefault));  // host or device
   }
   return GH_OK;
 }

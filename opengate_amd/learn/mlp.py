@@ -73,11 +73,15 @@ class MLP:
 
     # ------------------------------------------------------------------ passes
     def forward(self, x, keep=True):
-        """x: (M, in) float tensor on the device. Returns (M, out) FP32."""
+        """x: (M, in) float tensor on the device, or an already padded contiguous
+        (M, in padded) bf16 tensor. Returns (M, out) FP32."""
         torch = _torch()
         M = x.shape[0]
-        X = torch.zeros(M, self.psizes[0], device=self.dev, dtype=torch.bfloat16)
-        X[:, : self.sizes[0]] = x.to(torch.bfloat16)
+        if x.dtype == torch.bfloat16 and x.shape[1] == self.psizes[0] and x.is_contiguous():
+            X = x
+        else:
+            X = torch.zeros(M, self.psizes[0], device=self.dev, dtype=torch.bfloat16)
+            X[:, : self.sizes[0]] = x.to(torch.bfloat16)
         xs, zs = [X], []
         L = len(self.W)
         for l in range(L):
@@ -94,6 +98,24 @@ class MLP:
         if keep:
             self._xs, self._zs = xs, zs
         return zs[-1][:, : self.sizes[-1]]
+
+    def infer(self, X, out, bufs):
+        """Inference without keeping activations: X (M, in padded) bf16, out (M, out padded)
+        FP32, bufs two bf16 buffers of at least (M, widest hidden layer). Hidden layers write
+        only their bf16 activations."""
+        M = X.shape[0]
+        cur = X
+        L = len(self.W)
+        for l in range(L):
+            K, N = self.psizes[l], self.psizes[l + 1]
+            if l < L - 1:
+                Y = bufs[l % 2].view(-1)[: M * N].view(M, N)
+                self._gemm(cur, K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c16=Y, ldc=N, mode=EPI_BIAS_SILU)
+                cur = Y
+            else:
+                self._gemm(cur, K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c32=out, ldc=out.stride(0),
+                           mode=EPI_BIAS)
+        return out
 
     def backward(self, grad_out):
         """grad_out: dLoss/dOutput (M, out) FP32. Returns [dW...], [db...] (FP32, padded)."""

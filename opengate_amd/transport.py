@@ -88,12 +88,17 @@ class SourceSpec:
     energy_width: Optional[np.ndarray] = None
     energy_cdf: Optional[np.ndarray] = None
     weight: float = 1.0
-    phsp: Optional[list] = None  # 8 float32 arrays x y z dx dy dz E w (source frame)
+    phsp: Optional[list] = None  # 8 float32 arrays x y z dx dy dz E w (source frame), numpy or device tensors
     phsp_start: int = 0
     phsp_iso: bool = False
 
+    def _phsp_on_device(self):
+        return self.phsp is not None and hasattr(self.phsp[0], "data_ptr")
+
     def max_energy(self):
         if self.pos_type == "phsp":
+            if self._phsp_on_device():
+                return float(self.phsp[6].max()) if self.phsp[6].numel() else 0.0
             return float(np.max(self.phsp[6])) if len(self.phsp[6]) else 0.0
         if self.ene_type == "mono":
             return float(self.energy)
@@ -134,11 +139,17 @@ class SourceSpec:
                 self._ew = np.ascontiguousarray(self.energy_width, dtype=np.float32)
                 d.energy_width = A.fptr(self._ew)
         if self.pos_type == "phsp":
-            self._ph = [np.ascontiguousarray(a, dtype=np.float32) for a in self.phsp]
-            d.phsp_n = len(self._ph[0])
+            if self._phsp_on_device():
+                self._ph = [a.contiguous().float() for a in self.phsp]
+                d.phsp_n = int(self._ph[0].numel())
+                d.phsp_on_device = 1
+                d.phsp = (A.FP * 8)(*[C.cast(C.c_void_p(a.data_ptr()), A.FP) for a in self._ph])
+            else:
+                self._ph = [np.ascontiguousarray(a, dtype=np.float32) for a in self.phsp]
+                d.phsp_n = len(self._ph[0])
+                d.phsp = (A.FP * 8)(*[A.fptr(a) for a in self._ph])
             d.phsp_start = int(self.phsp_start)
             d.phsp_iso = int(bool(self.phsp_iso))
-            d.phsp = (A.FP * 8)(*[A.fptr(a) for a in self._ph])
         return d
 
 
@@ -227,6 +238,7 @@ class GpuEngine:
         if rc != A.GH_OK:
             raise GhError(f"gh_create(device={device}) failed with code {rc} (no usable HIP device?)")
         self.ctx = h
+        self.device = int(device)
         self.sims = []
         self.tables = None
 
@@ -326,14 +338,23 @@ class GpuEngine:
                                           A.dptr(res.get("counts"))), "gh_get_tally")
         return res
 
-    def get_phsp(self, sim_id=0):
+    def get_phsp(self, sim_id=0, device=False):
+        """Recorded exit-face photons: 8 arrays (x, y, z, u, v, w, E, weight) and the number of
+        photons that reached the face (may exceed the capacity). device=True returns device
+        tensors (copied on the GPU) instead of numpy arrays."""
         _, _, tal = self.sims[sim_id]
         n = C.c_uint64()
         self._check(self.lib.gh_get_phsp(self.ctx, int(sim_id), (A.FP * 8)(), 0, C.byref(n)), "gh_get_phsp")
         m = min(int(n.value), int(tal.record_capacity))
-        arrs = [np.zeros(m, dtype=np.float32) for _ in range(8)]
-        self._check(self.lib.gh_get_phsp(self.ctx, int(sim_id), (A.FP * 8)(*[A.fptr(a) for a in arrs]), m,
-                                         C.byref(n)), "gh_get_phsp")
+        if device:
+            import torch
+
+            arrs = [torch.empty(m, dtype=torch.float32, device=f"cuda:{self.device}") for _ in range(8)]
+            ptrs = [C.cast(C.c_void_p(a.data_ptr()), A.FP) for a in arrs]
+        else:
+            arrs = [np.zeros(m, dtype=np.float32) for _ in range(8)]
+            ptrs = [A.fptr(a) for a in arrs]
+        self._check(self.lib.gh_get_phsp(self.ctx, int(sim_id), (A.FP * 8)(*ptrs), m, C.byref(n)), "gh_get_phsp")
         return arrs, int(n.value)
 
     def get_event_edep(self, sim_id, n):

@@ -77,3 +77,52 @@ def batch64(seed=3, n_sims=64):
         tal = TallySpec(size=(n, n, n), spacing=(sp, sp, sp))
         out.append((ph, src, tal, s["histories"]))
     return tables, out, {"desc": BATCH64_DESC}
+
+
+# ---------------------------------------------------------------------- exit plane
+EXITPLANE_DESC = ("exitplane: tungsten block 40 mm thick with a 20x20 mm aperture (seed-generated edge "
+                  "roughness, 0.5 mm voxels) under a 2.5 mm Al filter, 64x64 mm; raw 120 kVp Kramers point "
+                  "source 100 mm above the filter, 10 deg half-angle cone; photons recorded on the exit face")
+COLLIMATOR_SPACING = 0.5
+COLLIMATOR_SHAPE = (128, 128, 85)  # x, y, z voxels: W at z in [0, 40) mm, Al at [40, 42.5) mm
+COLLIMATOR_SOURCE = (32.0, 32.0, 142.5)  # point source, collimator box frame (mm)
+# collimator box frame -> ct256 phantom frame: the source lands where the ct256 cone
+# source sits (600 mm above the phantom centre), the aperture axis on the phantom centre
+COLLIMATOR_TO_CT256 = (96.0, 96.0, 521.5)
+
+
+def collimator_labels(seed=4):
+    """uint8 labels (z, y, x): 0 air, 1 tungsten, 2 aluminium. The aperture is 20x20 mm
+    centred on the source axis; each 5 mm slab of the block moves each aperture edge by
+    -1..+1 voxel (seeded), so the block is a seed-generated casting, not an ideal cut."""
+    g = np.random.default_rng(seed)
+    nx, ny, nz = COLLIMATOR_SHAPE
+    sp = COLLIMATOR_SPACING
+    lab = np.zeros((nz, ny, nx), np.uint8)
+    nw = int(round(40.0 / sp))
+    lab[:nw] = 1
+    lab[nw:] = 2
+    c = (np.arange(nx) + 0.5) * sp
+    for z0 in range(0, nw, 10):
+        e = g.integers(-1, 2, size=4) * sp  # x-, x+, y-, y+ edge offsets
+        inx = (c >= 22.0 + e[0]) & (c < 42.0 + e[1])
+        iny = (c >= 22.0 + e[2]) & (c < 42.0 + e[3])
+        lab[z0:z0 + 10][:, iny[:, None] & inx[None, :]] = 0
+    return lab
+
+
+def exitplane(seed=4, e_max=0.1201, capacity=0):
+    """Collimator stage: returns (tables, phantom, source, tally, info). The tally records
+    photons leaving through the -z face (capacity records); the world is vacuum."""
+    db = MaterialDatabase()
+    mats = [db.get("G4_AIR"), db.get("G4_W"), db.get("G4_Al")]
+    tables = xs.build_tables(mats, e_max)
+    sp = COLLIMATOR_SPACING
+    ph = PhantomSpec(collimator_labels(seed), (sp, sp, sp), world_material=-1)
+    e, w = P.kramers_spectrum(al_mm=0.0, db=db)
+    ev, ew, cdf = spectrum_histogram(e, w)
+    src = SourceSpec(pos_trans=COLLIMATOR_SOURCE, theta=(0.0, np.deg2rad(10.0)), ene_type="spectrum_histogram",
+                     energy_value=ev, energy_width=ew, energy_cdf=cdf)
+    tal = TallySpec(size=(8, 8, 8), spacing=(8.0, 8.0, 85 * sp / 8), edep2=False, aligned=False, record_face=4,
+                    record_capacity=int(capacity))
+    return tables, ph, src, tal, {"desc": EXITPLANE_DESC, "names": [m.name for m in mats]}

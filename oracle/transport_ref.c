@@ -873,3 +873,82 @@ float ref_mu_at(const gh_physics_desc *phys, int m, float E) {
   energy_index(&P, E, &x);
   return node_interp(phys->mu_total + (size_t)m * P.nb1, &x);
 }
+
+/* ------------------------------------------------------------------ learned exit-plane source
+ * Restatements of csrc/exitplane.hip (generator noise and decoder) in the same FP32
+ * operation order, so that the GPU output is bit-identical. */
+
+/* N(0,1) noise as float (the GPU rounds it to bf16): row r, columns 0..ld-1 (zero from n_cols) */
+void ref_gaussian(uint64_t seed, uint64_t first, int64_t n, int cols, int ld, float *out) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t r = 0; r < n; ++r) {
+    const uint64_t id = first + (uint64_t)r;
+    for (int c0 = 0; c0 < ld; c0 += 4) {
+      const uint32_t ctr[4] = {(uint32_t)id, (uint32_t)(id >> 32), 0x4E4F4953u, (uint32_t)(c0 >> 2)};
+      uint32_t x[4];
+      ref_philox4x32_10(ctr, key, x);
+      float u[4];
+      for (int j = 0; j < 4; ++j) u[j] = (float)((x[j] >> 8) | 1u) * 0x1p-24f;
+      const float ra = sqrtf(-2.0f * ref_log(u[0])), rb = sqrtf(-2.0f * ref_log(u[2]));
+      float sa, ca, sb, cb;
+      ref_sincos(6.28318530717958647692f * u[1], &sa, &ca);
+      ref_sincos(6.28318530717958647692f * u[3], &sb, &cb);
+      const float g[4] = {ra * ca, ra * sa, rb * cb, rb * sb};
+      for (int j = 0; j < 4; ++j) {
+        const int c = c0 + j;
+        if (c < ld) out[r * ld + c] = c < cols ? g[j] : 0.0f;
+      }
+    }
+  }
+}
+
+static float qdecode_ref(const float *q, int K, float scale, float g_max, float g) {
+  float t = (g + g_max) * scale;
+  t = fminf(fmaxf(t, 0.0f), (float)(K - 1));
+  int k = (int)t;
+  if (k > K - 2) k = K - 2;
+  const float a = t - (float)k;
+  return fmaf(a, q[k + 1] - q[k], q[k]);
+}
+
+static void normalize3_ref(float *x, float *y, float *z) {
+  const float n2 = fmaf(*z, *z, fmaf(*y, *y, *x * *x));
+  const float inv = 1.0f / sqrtf(n2);
+  *x = *x * inv; *y = *y * inv; *z = *z * inv;
+}
+
+void ref_exitplane_decode(const float *Z, int64_t ldz, int64_t n, const float *qtab, int K, float g_max,
+                          const float focal[3], float plane_z, const float offset[3], float weight, float *const out[8]) {
+  const float scale = (float)(K - 1) / (2.0f * g_max);
+  for (int64_t i = 0; i < n; ++i) {
+    const float *zr = Z + i * ldz;
+    const float E = qdecode_ref(qtab, K, scale, g_max, zr[0]);
+    const float px = qdecode_ref(qtab + K, K, scale, g_max, zr[1]);
+    const float py = qdecode_ref(qtab + 2 * K, K, scale, g_max, zr[2]);
+    float lu = qdecode_ref(qtab + 3 * K, K, scale, g_max, zr[3]);
+    float lv = qdecode_ref(qtab + 4 * K, K, scale, g_max, zr[4]);
+    float lw = qdecode_ref(qtab + 5 * K, K, scale, g_max, zr[5]);
+    const float l2 = fmaf(lw, lw, fmaf(lv, lv, lu * lu));
+    if (!(l2 > 0.0f)) { lu = 0.0f; lv = 0.0f; lw = 1.0f; }
+    else normalize3_ref(&lu, &lv, &lw);
+    float nx = px - focal[0], ny = py - focal[1], nz = plane_z - focal[2];
+    normalize3_ref(&nx, &ny, &nz);
+    const float up = sqrtf(fmaf(ny, ny, nx * nx));
+    float e1x, e1y, e1z, e2x, e2y;
+    if (up > 1e-7f) {
+      const float iu = 1.0f / up;
+      e1x = (nx * nz) * iu; e1y = (ny * nz) * iu; e1z = -up;
+      e2x = -ny * iu; e2y = nx * iu;
+    } else {
+      e1x = nz; e1y = 0.0f; e1z = 0.0f;
+      e2x = 0.0f; e2y = 1.0f;
+    }
+    float du = fmaf(lw, nx, fmaf(lv, e2x, lu * e1x));
+    float dv = fmaf(lw, ny, fmaf(lv, e2y, lu * e1y));
+    float dw = fmaf(lw, nz, lu * e1z);
+    normalize3_ref(&du, &dv, &dw);
+    out[0][i] = px + offset[0]; out[1][i] = py + offset[1]; out[2][i] = plane_z + offset[2];
+    out[3][i] = du; out[4][i] = dv; out[5][i] = dw;
+    out[6][i] = E; out[7][i] = weight;
+  }
+}

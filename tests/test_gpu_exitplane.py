@@ -1,0 +1,186 @@
+"""Learned exit-plane source on the GPU: noise and decoder bit-exact against the C
+oracle, chunking invariance, and the end-to-end check that a generator trained on
+recorded collimator photons reproduces the held-out photons (fluence, penumbra,
+dose in water) at a large speed gain over the upstream transport."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import exitplane_ref as R
+from oracle import mlp_ref as MR
+
+pytestmark = pytest.mark.gpu
+
+FOCAL = (32.0, 32.0, 142.5)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _records(n, seed=0):
+    g = np.random.default_rng(seed)
+    x = g.uniform(12, 52, n)
+    y = g.uniform(12, 52, n)
+    d = np.stack([x - FOCAL[0], y - FOCAL[1], np.full(n, -FOCAL[2])], 1)
+    k = g.uniform(size=n) < 0.2
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    d[k] += g.normal(scale=0.2, size=(k.sum(), 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    return [a.astype(np.float32) for a in (x, y, np.zeros(n), d[:, 0], d[:, 1], d[:, 2], g.uniform(0.02, 0.12, n),
+                                           np.ones(n))]
+
+
+def _generator(seed=3):
+    from opengate_amd.learn.exitplane import ExitPlaneGenerator, QuantileMap, exit_features
+
+    g = ExitPlaneGenerator(FOCAL, 0.0, seed=seed)
+    g.qmap = QuantileMap.fit(exit_features(_records(50000), FOCAL, 0.0, g.dev), n_knots=1025)
+    return g
+
+
+def test_noise_bit_exact():
+    torch = _torch()
+    g = _generator()
+    print("[gpu case] gh_gaussian_bf16 3000x8", flush=True)
+    X = g.noise(3000, 0xABCDEF12345, first=123)
+    torch.cuda.synchronize()
+    want = MR.to_bf16(R.gaussian(0xABCDEF12345, 123, 3000, 8, 8)).astype(np.float32)
+    np.testing.assert_array_equal(X.float().cpu().numpy(), want)
+
+
+def test_decode_bit_exact():
+    torch = _torch()
+    import ctypes as C
+
+    from opengate_amd import _abi as A
+
+    g = _generator()
+    rng = np.random.default_rng(1)
+    Z = rng.uniform(-6.5, 6.5, size=(5000, 8)).astype(np.float32)
+    Z[:7, 3:6] = 0.0
+    Z[7, :] = np.nan
+    Zd = torch.from_numpy(Z).cuda()
+    out = [torch.empty(5000, device="cuda") for _ in range(8)]
+    focal = (C.c_float * 3)(*FOCAL)
+    off = (C.c_float * 3)(96.0, 96.0, 521.5)
+    print("[gpu case] gh_exitplane_decode 5000", flush=True)
+    rc = g.lib.gh_exitplane_decode(C.c_void_p(Zd.data_ptr()), 8, 5000, C.c_void_p(g.qmap.qtab.data_ptr()),
+                                   g.qmap.n_knots, g.qmap.g_max, focal, 0.0, off, 0.75,
+                                   (A.FP * 8)(*[C.cast(C.c_void_p(a.data_ptr()), A.FP) for a in out]), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    ref = R.decode(Z, g.qmap.qtab.cpu().numpy(), g.qmap.g_max, FOCAL, 0.0, (96.0, 96.0, 521.5), 0.75)
+    for k in range(8):
+        np.testing.assert_array_equal(out[k].cpu().numpy(), ref[k])
+
+
+def test_generate_chunking_infer_and_decode():
+    torch = _torch()
+    g = _generator()
+    print("[gpu case] generate 10000 (chunks 10000 / 3001 / offset)", flush=True)
+    a = g.generate(10000, seed=5, chunk=10000)
+    b = g.generate(10000, seed=5, chunk=3001)
+    c = g.generate(4000, seed=5, first=6000)
+    torch.cuda.synchronize()
+    for k in range(8):
+        np.testing.assert_array_equal(a[k].cpu().numpy(), b[k].cpu().numpy())
+        np.testing.assert_array_equal(a[k][6000:].cpu().numpy(), c[k].cpu().numpy())
+    # inference path == training forward; decode of it == oracle decode == generate
+    X = g.noise(10000, 5)
+    Zf = g.mlp.forward(X, keep=False)
+    Zi = torch.empty(10000, 8, device="cuda")
+    bufs = [torch.empty(10000, 256, device="cuda", dtype=torch.bfloat16) for _ in range(2)]
+    g.mlp.infer(X, Zi, bufs)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(Zf.cpu().numpy(), Zi[:, :6].cpu().numpy())
+    ref = R.decode(Zi.cpu().numpy(), g.qmap.qtab.cpu().numpy(), g.qmap.g_max, FOCAL, 0.0)
+    for k in range(8):
+        np.testing.assert_array_equal(a[k].cpu().numpy(), ref[k])
+    # the network agrees with the FP64 model of the same bf16 weights
+    W = [w.double().cpu().numpy() for w in g.mlp.W]
+    bb = [b_.double().cpu().numpy() for b_ in g.mlp.b]
+    zr, _, _ = MR.forward(W, bb, X.float().cpu().numpy()[:512])
+    assert np.max(np.abs(Zi[:512].cpu().numpy() - zr)) < 0.05 * max(1.0, np.abs(zr).max())
+
+
+def _stage1(engine, n, seed, capacity):
+    from opengate_amd import workloads as WL
+
+    T, ph, src, tal, _ = WL.exitplane(capacity=capacity)
+    engine.clear()
+    engine.set_physics(T)
+    sid = engine.add_simulation(ph, src, tal)
+    _torch().cuda.synchronize()
+    t = time.perf_counter()
+    c = engine.run([n], seed)
+    dt = time.perf_counter() - t
+    rec, nrec = engine.get_phsp(sid, device=True)
+    assert nrec <= capacity
+    return rec, nrec, dt, c
+
+
+def _water_dose(engine, phsp, n, seed):
+    from opengate_amd.physics import xs
+    from opengate_amd.physics.materials import MaterialDatabase
+    from opengate_amd.transport import PhantomSpec, SourceSpec, TallySpec
+
+    db = MaterialDatabase()
+    T = xs.build_tables([db.get("G4_WATER")], 0.1201)
+    lab = np.zeros((40, 48, 48), np.uint8)
+    ph = PhantomSpec(lab, (2.0, 2.0, 2.0), world_material=-1)
+    # exit plane 300 mm above the water surface, aperture axis on the phantom axis
+    src = SourceSpec(pos_type="phsp", phsp=phsp, pos_trans=(48.0 - 32.0, 48.0 - 32.0, 80.0 + 300.0))
+    tal = TallySpec(size=(48, 48, 40), spacing=(2.0, 2.0, 2.0))
+    engine.clear()
+    engine.set_physics(T)
+    sid = engine.add_simulation(ph, src, tal)
+    engine.run([n], seed)
+    return engine.get_tally(sid)["edep"]
+
+
+def test_learned_source_reproduces_plain_transport(engine):
+    torch = _torch()
+    from opengate_amd.learn.exitplane import ExitPlaneGenerator, compare_exit_planes
+
+    print("[gpu case] exitplane stage 1: 2e7 + 2e7 histories", flush=True)
+    train, ntr, dt1, c1 = _stage1(engine, 20_000_000, 11, 2_500_000)
+    hold, nho, _, _ = _stage1(engine, 20_000_000, 12, 2_500_000)
+    print(f"stage 1: {ntr} recorded photons in {dt1:.3f} s ({c1['steps']} Woodcock steps)", flush=True)
+    assert ntr > 1_000_000 and nho > 1_000_000
+    gen = ExitPlaneGenerator(FOCAL, 0.0, seed=0)
+    print("[gpu case] exitplane fit 1500 steps", flush=True)
+    hist = gen.fit([a[:ntr] for a in train], steps=1500, batch=8192, projections=64, lr=1e-3, seed=2)
+    assert np.mean(hist[-100:]) < 0.2 * np.mean(hist[:100])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    fake = gen.generate(nho, seed=9)
+    torch.cuda.synchronize()
+    dtg = time.perf_counter() - t
+    ref = [a[:nho] for a in hold]
+    m = compare_exit_planes(fake, ref, distance=300.0)
+    print("exit-plane comparison:", m, flush=True)
+    gain = (dt1 / ntr) / (dtg / nho)
+    print(f"generation {nho / dtg:.3e} photons/s vs upstream transport {ntr / dt1:.3e} photons/s: gain {gain:.1f}x",
+          flush=True)
+    assert 0.97 < m["core_ratio"] < 1.03
+    for a, b in zip(m["penumbra_gen"], m["penumbra_ref"]):
+        assert abs(a - b) <= 1.5
+    assert m["w1_E"] < 0.002 and m["w1_x"] < 0.5 and m["w1_y"] < 0.5
+    assert gain > 5.0
+    # dose in water from the held-out photons vs the same number of generated photons
+    print("[gpu case] exitplane dose in water", flush=True)
+    d_ref = _water_dose(engine, ref, nho, 21)
+    d_gen = _water_dose(engine, fake, nho, 21)
+    blk = lambda d: d.reshape(10, 4, 12, 4, 12, 4).sum((1, 3, 5))  # noqa: E731  8 mm blocks
+    br, bg = blk(d_ref), blk(d_gen)
+    core = br > 0.3 * br.max()
+    ratio = bg[core].sum() / br[core].sum()
+    rel = np.abs(bg[core] - br[core]) / br[core]
+    print(f"dose: core blocks {core.sum()}, integral ratio {ratio:.4f}, median |rel diff| {np.median(rel):.4f}",
+          flush=True)
+    assert abs(ratio - 1) < 0.03
+    assert np.median(rel) < 0.05

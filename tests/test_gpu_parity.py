@@ -118,6 +118,42 @@ def test_phase_space_source_iso_and_file_directions(engine):
         compare(c, rc, g, r)
 
 
+def test_phase_space_source_on_device(engine):
+    """The same records handed over as device arrays (used in place) give the oracle's result;
+    an out-of-table energy in device memory is rejected before any transport."""
+    import torch
+
+    from opengate_amd.transport import GhError
+
+    T = tables()
+    ph = small_phantom(24, 2.0)
+    rng = np.random.default_rng(4)
+    n = 3000
+    cx, cy, cz = centre(ph)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    arrs = [np.full(n, cx, np.float32), np.full(n, cy, np.float32) + rng.uniform(-4, 4, n).astype(np.float32),
+            np.full(n, cz, np.float32), d[:, 0].astype(np.float32), d[:, 1].astype(np.float32),
+            d[:, 2].astype(np.float32), rng.uniform(0.02, 0.14, n).astype(np.float32),
+            rng.uniform(0.5, 1.5, n).astype(np.float32)]
+    dev = [torch.from_numpy(a).cuda() for a in arrs]
+    tal = aligned_tally(ph)
+    engine.clear()
+    engine.set_physics(T)
+    sid = engine.add_simulation(ph, SourceSpec(pos_type="phsp", phsp=dev, phsp_start=5), tal)
+    c = engine.run([10_000], 21)
+    g = engine.get_tally(sid)
+    r, rc = O.run(T, ph, SourceSpec(pos_type="phsp", phsp=arrs, phsp_start=5), tal, 10_000, 21)
+    compare(c, rc, g, r)
+    engine.clear()
+    engine.set_physics(T)
+    for k, v in ((6, float("nan")), (6, -0.01), (7, -1.0), (7, float("inf"))):
+        bad = [t.clone() for t in dev]
+        bad[k][17] = v
+        with pytest.raises(GhError):
+            engine.add_simulation(ph, SourceSpec(pos_type="phsp", phsp=bad), tal)
+
+
 def test_batched_launch_equals_separate_runs(engine):
     T = tables(e_max=0.21)
     specs = []
