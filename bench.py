@@ -23,6 +23,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -68,7 +70,11 @@ def main():
     ap.add_argument("--slow-min", type=int, default=16)
     ap.add_argument("--refill-min", type=int, default=16)
     ap.add_argument("--grid", type=int, default=0)
-    ap.add_argument("--workload", default="ct256", choices=("ct256", "batch64"))
+    ap.add_argument("--workload", default="ct256", choices=("ct256", "batch64", "exitplane"))
+    ap.add_argument("--ep-train-photons", type=float, default=1e8,
+                    help="exitplane: recorded collimator photons used to train the generator")
+    ap.add_argument("--ep-fit-steps", type=int, default=4000, help="exitplane: generator training steps")
+    ap.add_argument("--ep-chunk", type=float, default=1e8, help="exitplane: photons per transport chunk")
     ap.add_argument("--autotune", action="store_true", help="pick launch parameters with gh_autotune first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy x5 (reference kernel for rocprofv3 FETCH/WRITE_SIZE)")
@@ -93,6 +99,8 @@ def main():
 
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "exitplane":
+        return bench_exitplane(args)
     eng = GpuEngine(0)
     tables, ph, src, tal, info = workloads.ct256(eng)
     eng.set_physics(tables)
@@ -201,6 +209,171 @@ def bench_batch(args):
     }
     eng.close()
     print(json.dumps(line), flush=True)
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _collimator_chunks(eng, n_photons, seed, chunk, hist_per_photon):
+    """Plain upstream transport through the collimator until n_photons exit photons are
+    recorded; yields (device records, count, seconds) per chunk."""
+    import torch
+
+    got = 0
+    k = 0
+    while got < n_photons:
+        want = int(min(chunk, n_photons - got))
+        hist = int(want * hist_per_photon * 1.02) + 1000
+        eng.reset_tally(0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        eng.run([hist], seed, first_history=[k * (1 << 40)])
+        rec, n_rec = eng.get_phsp(0, device=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        m = min(want, len(rec[0]))
+        got += m
+        k += 1
+        yield [a[:m] for a in rec], m, dt, hist
+
+
+def bench_exitplane(args):
+    """Learned exit-plane source vs plain transport through the collimator, both driving
+    the same number of exit-plane photons into the ct256 phantom."""
+    import torch
+
+    from opengate_amd import workloads as WL
+    from opengate_amd.learn.exitplane import ExitPlaneGenerator, compare_exit_planes
+    from opengate_amd.transport import GpuEngine, SourceSpec
+
+    n_total = int(args.histories)
+    chunk = int(args.ep_chunk)
+    cap = int(chunk * 1.1)
+    Tc, phc, srcc, talc, infoc = WL.exitplane(capacity=cap)
+    col = GpuEngine(0)
+    col.set_physics(Tc)
+    col.add_simulation(phc, srcc, talc)
+    # histories per recorded photon from a short run
+    c = col.run([2_000_000], 99)
+    hpp = 2_000_000 / max(1, c["recorded"])
+    col.reset_tally(0)
+    # ---- training set and fit
+    n_train = int(args.ep_train_photons)
+    train = [torch.empty(n_train, device="cuda") for _ in range(8)]
+    off = 0
+    t_train_rec = 0.0
+    for rec, m, dt, _ in _collimator_chunks(col, n_train, 1, chunk, hpp):
+        for a, b in zip(train, rec):
+            a[off:off + m] = b
+        off += m
+        t_train_rec += dt
+        _log(f"training photons {off}/{n_train}")
+    gen = ExitPlaneGenerator(WL.COLLIMATOR_SOURCE, 0.0, seed=0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    hist = gen.fit(train, steps=args.ep_fit_steps, batch=16384, projections=128, lr=1e-3, seed=2, log_every=500)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t
+    _log(f"fit {args.ep_fit_steps} steps in {t_fit:.1f} s, final loss {float(np.mean(hist[-50:])):.5f}")
+    del train
+    # ---- phantom stage (ct256), one context
+    ct = GpuEngine(0)
+    Tp, php, srcp, talp, infop = WL.ct256(ct)
+    ct.set_physics(Tp)
+
+    def phsp_source(rec):
+        return SourceSpec(pos_type="phsp", phsp=rec, pos_trans=WL.COLLIMATOR_TO_CT256)
+
+    # ---- plain: upstream transport + phantom
+    t_up = t_ph = 0.0
+    done = 0
+    holdout = None
+    for rec, m, dt, _ in _collimator_chunks(col, n_total, 7, chunk, hpp):
+        if holdout is None:
+            holdout = [a[:4_000_000].clone() for a in rec]
+        if not ct.sims:
+            ct.add_simulation(php, phsp_source(rec), talp)
+        else:
+            ct.set_source(0, phsp_source(rec))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ct.run([m], 31, first_history=[done])
+        torch.cuda.synchronize()
+        t_ph += time.perf_counter() - t
+        t_up += dt
+        done += m
+        _log(f"plain {done}/{n_total}")
+    plain = ct.get_tally(0)
+    # ---- learned: generator + phantom
+    ct.reset_tally(0)
+    t_gen = t_ph2 = 0.0
+    done = 0
+    buf = [torch.empty(chunk, device="cuda") for _ in range(8)]
+    first_rec = None
+    while done < n_total:
+        m = min(chunk, n_total - done)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rec = gen.generate(m, seed=5, first=done, out=[b[:m] for b in buf])
+        torch.cuda.synchronize()
+        t_gen += time.perf_counter() - t
+        if first_rec is None:
+            first_rec = [a[:4_000_000].clone() for a in rec]
+        ct.set_source(0, phsp_source(rec))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ct.run([m], 31, first_history=[done])
+        torch.cuda.synchronize()
+        t_ph2 += time.perf_counter() - t
+        done += m
+        _log(f"learned {done}/{n_total}")
+    learned = ct.get_tally(0)
+    plane = compare_exit_planes(first_rec, holdout, distance=393.5)
+    dose = compare_dose(learned["edep"], plain["edep"])
+    gen_rate = n_total / t_gen
+    line = {
+        "metric": METRIC, "value": n_total / (t_gen + t_ph2), "unit": "photons/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": (t_gen + t_ph2) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 generator (fp32 accumulate) + fp32 transport",
+        "data": "synthetic (seed-generated collimator and phantom, recorded training photons)",
+        "config": {"workload": infoc["desc"] + " -> " + infop["desc"], "exit_photons": n_total,
+                   "form": "learned exit-plane source (6x256 MLP) driving the ct256 phantom",
+                   "training_photons": n_train, "fit_steps": args.ep_fit_steps, "fit_s": t_fit,
+                   "training_record_s": t_train_rec,
+                   "generator_photons_per_s": gen_rate,
+                   "generator_tflops": gen.flops_per_photon() * gen_rate / 1e12,
+                   "plain_upstream_photons_per_s": n_total / t_up,
+                   "plain_photons_per_s": n_total / (t_up + t_ph),
+                   "phantom_transport_photons_per_s": n_total / t_ph2,
+                   "gain_upstream": t_up / t_gen, "gain_end_to_end": (t_up + t_ph) / (t_gen + t_ph2),
+                   "collimator_histories_per_exit_photon": hpp,
+                   "exit_plane": plane, "dose": dose},
+    }
+    col.close()
+    ct.close()
+    print(json.dumps(line), flush=True)
+
+
+def compare_dose(test, ref, block=4):
+    """Dose agreement on block x block x block voxel blocks: blocks above 20 % of the
+    maximum (the field), integral ratio, median / 95th percentile relative difference,
+    and the central-axis depth-dose (20 x 20 mm column) worst relative difference."""
+    nz, ny, nx = ref.shape
+    b = block
+    crop = lambda d: d[: nz // b * b, : ny // b * b, : nx // b * b]  # noqa: E731
+    red = lambda d: crop(d).reshape(nz // b, b, ny // b, b, nx // b, b).sum((1, 3, 5))  # noqa: E731
+    R_, T_ = red(ref), red(test)
+    core = R_ > 0.2 * R_.max()
+    rel = np.abs(T_[core] - R_[core]) / R_[core]
+    cy, cx = ny // 2, nx // 2
+    col = lambda d: d[:, cy - 10:cy + 10, cx - 10:cx + 10].sum((1, 2))  # noqa: E731
+    dr, dt = col(ref), col(test)
+    m = dr > 0.2 * dr.max()
+    return {"block_mm": b, "core_blocks": int(core.sum()), "integral_ratio": float(T_[core].sum() / R_[core].sum()),
+            "median_rel_diff": float(np.median(rel)), "p95_rel_diff": float(np.percentile(rel, 95)),
+            "depth_dose_max_rel_diff": float(np.max(np.abs(dt[m] - dr[m]) / dr[m])),
+            "total_ratio": float(test.sum() / ref.sum())}
 
 
 if __name__ == "__main__":

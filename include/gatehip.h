@@ -57,6 +57,8 @@ enum { GH_ENE_MONO = 0, GH_ENE_LINES = 1, GH_ENE_HISTOGRAM = 2, GH_ENE_RANGE = 3
 /* tally flags */
 enum { GH_TALLY_EDEP2 = 1, GH_TALLY_DOSE = 2, GH_TALLY_DOSE2 = 4, GH_TALLY_COUNTS = 8,
        GH_TALLY_ALIGNED = 16 };
+/* record_face value that records photons leaving through any face of the phantom box */
+enum { GH_FACE_ALL = 6 };
 
 typedef struct gh_ctx gh_ctx;
 
@@ -120,7 +122,8 @@ typedef struct gh_tally_desc {
   int32_t flags;             /* GH_TALLY_* ; ALIGNED: the tally grid is the phantom grid */
   float corner[3];           /* lower corner of the tally grid in the phantom frame */
   float spacing[3];
-  int32_t record_face;       /* -1 off, else 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z (vacuum world only) */
+  int32_t record_face;       /* -1 off, else 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z, 6 (GH_FACE_ALL) any face
+                                (vacuum world only) */
   uint32_t record_capacity;  /* max photons recorded */
   int32_t detector_label_min;/* -1 off, else per-history energy in labels >= this value */
   int32_t reserved0;
@@ -179,7 +182,8 @@ int gh_get_tally(gh_ctx *ctx, int32_t sim_id, double *edep, double *edep2, doubl
 /* Recorded exit-face photons: 8 float arrays of length *n (x y z dx dy dz E w, phantom frame).
    out[k] may be host or device pointers (copied with the HIP runtime's default direction). */
 int gh_get_phsp(gh_ctx *ctx, int32_t sim_id, float *const out[8], uint64_t capacity, uint64_t *n);
-/* Per-history detector energy (tally.detector_label_min >= 0), n_histories floats. */
+/* Per-history detector energy (tally.detector_label_min >= 0), n_histories floats; out may be a
+   host or a device pointer. */
 int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n);
 
 /* HU image -> uint16 labels: out[i] = labels_sorted[#{k : bins[k] <= hu[i]}]
@@ -214,6 +218,46 @@ int gh_gaussian_bf16(uint64_t seed, uint64_t first_row, int64_t n_rows, int32_t 
 int gh_exitplane_decode(const float *Z, int64_t ldz, int64_t n, const float *qtab, int32_t n_knots, float g_max,
                         const float focal[3], float plane_z, const float offset[3], float weight,
                         float *const out[8], void *stream);
+
+/* SPECT heads for the learned collimator-detector response (ARFActor, actors/arfactors.py:140-463:
+ * apply / arf_build_image_from_projected_points, and GateARFActor::SteppingAction,
+ * GateARFActor.cpp:60-105) and for the training set of ARFTrainingDatasetActor
+ * (GateARFTrainingDatasetActor.cpp:85-140: E, theta = acos(d_normal), phi = atan2(d_y, d_x),
+ * window id + 1 with 0 = not detected). */
+typedef struct gh_spect_desc {
+  float center[3];        /* point on the gantry axis (axis = +z), phantom frame [mm] */
+  float radius;           /* axis to collimator face [mm] */
+  int32_t n_proj;         /* heads (projections), head k at angle0 + 2 pi k / n_proj */
+  int32_t nx, ny;         /* image columns (tangential), rows (axial) */
+  float angle0;           /* [rad] */
+  float spacing[2];       /* pixel size (tangential, axial) [mm] */
+  float cos_theta_max;    /* photons with d.normal <= cos_theta_max do not reach the crystal */
+  float crystal_depth;    /* collimator face -> plane where pixels are assigned [mm] */
+  float pitch;            /* hexagonal hole lattice pitch of the detector model [mm] (0: none) */
+  float patch_origin[3];  /* detector-model frame: lattice point where reduced entries land */
+  float norm[6];          /* NN input row = (E n0 + n1, theta n2 + n3, phi n4 + n5, 0...) */
+} gh_spect_desc;
+
+/* Photons that left the phantom (8 device arrays x y z u v w E weight, phantom frame, vacuum
+ * outside) -> (photon, head) pairs: NN input rows x16 (capacity x 8 bf16, may be NULL), pixel
+ * index (head*ny + iy)*nx + ix, weight, optionally (theta, phi) and the photon in the detector
+ * model frame (det_rec, may be NULL). angle_cs: device array of n_proj (cos, sin) pairs.
+ * *count (device, zeroed by the caller) receives the number of pairs; rows beyond capacity are
+ * counted but not written. */
+int gh_spect_pairs(const float *const rec[8], int64_t n, const gh_spect_desc *det, const float *angle_cs,
+                   int64_t capacity, void *x16, int32_t *pix, float *wgt, float *const det_rec[8],
+                   float *theta_phi, unsigned long long *count, void *stream);
+/* Deposited energy per event (device) -> window class (0 none, k = 1..n_win) after Gaussian
+ * blur FWHM = r0 sqrt(e0 E) (digitizer energy blurring + energy windows); win: host array of
+ * n_win (lo, hi) pairs in MeV, half-open. Event i uses noise keyed by (seed, first + i). */
+int gh_energy_windows(const float *edep, int64_t n, uint64_t seed, uint64_t first, float r0, float e0,
+                      const float *win, int32_t n_win, int32_t *cls, void *stream);
+/* Projection images [head][window][ny][nx] (FP64, device) += weight x softmax(logits)[1..n_win]
+ * per pair (ARF), or += weight at the simulated window (plain); img2 (optional) += squares. */
+int gh_arf_accumulate(const float *logits, int64_t ldl, int64_t m, const int32_t *pix, const float *wgt,
+                      int32_t n_win, int32_t plane, double *img, double *img2, void *stream);
+int gh_counts_accumulate(const int32_t *cls, const int32_t *pix, const float *wgt, int64_t m, int32_t n_win,
+                         int32_t plane, double *img, double *img2, void *stream);
 
 /* Device properties used by the autotuner and reports. */
 int gh_device_info(gh_ctx *ctx, int32_t *n_cu, int32_t *lds_bytes, char *name, int32_t name_len);

@@ -60,6 +60,14 @@ class TallyDesc(C.Structure):
     ]
 
 
+class SpectDesc(C.Structure):
+    _fields_ = [
+        ("center", C.c_float * 3), ("radius", C.c_float), ("n_proj", C.c_int32), ("nx", C.c_int32),
+        ("ny", C.c_int32), ("angle0", C.c_float), ("spacing", C.c_float * 2), ("cos_theta_max", C.c_float),
+        ("crystal_depth", C.c_float), ("pitch", C.c_float), ("patch_origin", C.c_float * 3), ("norm", C.c_float * 6),
+    ]
+
+
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("refill_interval", C.c_int32), ("chunk", C.c_int32),
                 ("slow_min", C.c_int32), ("refill_min", C.c_int32)]
@@ -127,6 +135,14 @@ SIGNATURES = {
                                    C.c_void_p]),
     "gh_exitplane_decode": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int32, C.c_float, FP,
                                       C.c_float, FP, C.c_float, FP * 8, C.c_void_p]),
+    "gh_spect_pairs": (C.c_int, [FP * 8, C.c_int64, C.POINTER(SpectDesc), C.c_void_p, C.c_int64, C.c_void_p,
+                                 C.c_void_p, C.c_void_p, C.POINTER(FP), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gh_energy_windows": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_float, C.c_float, FP,
+                                    C.c_int32, C.c_void_p, C.c_void_p]),
+    "gh_arf_accumulate": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gh_counts_accumulate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _LIB = None

@@ -261,7 +261,7 @@ class SimulationStatisticsActor(ActorBase):
         p.write_text(json.dumps(out, indent=1, default=float))
 
 
-FACES = {"-x": 0, "+x": 1, "-y": 2, "+y": 3, "-z": 4, "+z": 5}
+FACES = {"-x": 0, "+x": 1, "-y": 2, "+y": 3, "-z": 4, "+z": 5, "all": 6}
 
 
 class PhaseSpaceActor(ActorBase):

@@ -571,7 +571,7 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
         ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
         const float t = fmaxf(tf, 0.0f);
         const float ex = fmaf(t, ph.u, ph.x), ey = fmaf(t, ph.v, ph.y), ez = fmaf(t, ph.w, ph.z);
-        if (s.rec_face >= 0 && face == s.rec_face) {
+        if (s.rec_face >= 0 && (face == s.rec_face || s.rec_face == GH_FACE_ALL)) {
           const uint64_t m = __ballot(1);
           const uint32_t leader = __ffsll((unsigned long long)m) - 1;
           unsigned long long base = 0;
@@ -1093,7 +1093,7 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
     FAIL(GH_EINVAL, "aligned tally must have the phantom size");
   if (tal->record_face >= 0 && ph->world_material >= 0)
     FAIL(GH_EINVAL, "exit-face recording needs a vacuum world");
-  if (tal->record_face > 5) FAIL(GH_EINVAL, "record_face must be -1..5");
+  if (tal->record_face > GH_FACE_ALL) FAIL(GH_EINVAL, "record_face must be -1..6");
   HIPCHK(hipSetDevice(ctx->device));
 
   SimHost h;
@@ -1436,7 +1436,7 @@ int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n) {
   if (h.det < 0 || !h.dev.event_edep) FAIL(GH_ESTATE, "gh_get_event_edep: detector scoring not enabled");
   if (n > h.event_cap) FAIL(GH_ERANGE, "gh_get_event_edep: n beyond the last run");
   HIPCHK(hipSetDevice(ctx->device));
-  if (n) HIPCHK(hipMemcpy(out, h.dev.event_edep, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (n) HIPCHK(hipMemcpy(out, h.dev.event_edep, n * sizeof(float), hipMemcpyDefault));  // host or device
   return GH_OK;
 }
 

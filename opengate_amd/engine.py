@@ -17,6 +17,7 @@ import numpy as np
 from .actors.dose import GY_FACTOR, DoseActor, PhaseSpaceActor, SimulationStatisticsActor, FACES
 from .geometry.volumes import BoxVolume, ImageVolume
 from .physics import xs
+from .sources.gan import GANSource
 from .sources.generic import GenericSource, PhaseSpaceSource
 from .transport import GpuEngine, PhantomSpec, SourceSpec, TallySpec, spectrum_histogram, spectrum_lines
 from .units import g4_units
@@ -97,6 +98,15 @@ class SimulationEngine:
     def source_spec(self, src, n):
         vm = self.sim.volume_manager
         Ra, ta = vm.global_transform(src.attached if isinstance(src.attached, str) else src.attached.name)
+        if isinstance(src, GANSource):
+            # GateGANSource.cpp:227-254: position R_g (R_l p + t_l) + t_g, direction R_g d
+            seed = int(src.seed) if src.seed is not None else int(self._seed)
+            arrs = src.generate(n, seed, device=f"cuda:{self.sim.device}")
+            Rl = _rot(src.position.rotation)
+            tl = np.asarray(src.position.translation, dtype=np.float64)
+            return SourceSpec(pos_type="phsp", pos_rot=self.R.T @ Ra @ Rl,
+                              pos_trans=tuple(self.R.T @ (Ra @ tl + ta - self.t) + self.L / 2),
+                              dir_rot=self.R.T @ Ra, phsp=arrs)
         if isinstance(src, PhaseSpaceSource):
             arrs = src.load_arrays()
             Rp = _rot(src.position.rotation) if src.rotate_direction else np.eye(3)
@@ -219,6 +229,7 @@ class SimulationEngine:
         t_init = time.time()
         sim.physics_manager.check_supported()
         seed = sim.resolve_seed()
+        self._seed = seed
         self.engine = GpuEngine(sim.device)
         vol = self.phantom_volume()
         ph, mats, names = self.build_geometry(vol)

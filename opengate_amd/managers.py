@@ -2,7 +2,7 @@
 add_source :2607, add_actor :2610, run :2818; PhysicsManager :695; VolumeManager :1435).
 
 Only the photon-transport-and-dose path is available: the world, Box and Image
-volumes, GenericSource / PhaseSpaceSource with gamma primaries, the gamma physics
+volumes, GenericSource / PhaseSpaceSource / GANSource with gamma primaries, the gamma physics
 options, DoseActor, SimulationStatisticsActor and an exit-face PhaseSpaceActor.
 Anything else raises NotImplementedError instead of silently doing something else.
 """
@@ -15,6 +15,7 @@ from .actors.dose import DoseActor, PhaseSpaceActor, SimulationStatisticsActor
 from .base import Box, GateObject
 from .geometry.volumes import BoxVolume, ImageVolume
 from .physics.materials import MaterialDatabase
+from .sources.gan import GANSource
 from .sources.generic import GenericSource, PhaseSpaceSource
 from .units import g4_units
 
@@ -150,6 +151,8 @@ class Simulation:
             s = GenericSource(name)
         elif source_type == "PhaseSpaceSource":
             s = PhaseSpaceSource(name)
+        elif source_type == "GANSource":
+            s = GANSource(name)
         else:
             raise NotImplementedError(f"source type '{source_type}' is not part of the photon dose path")
         s.simulation = self

@@ -357,9 +357,18 @@ class GpuEngine:
         self._check(self.lib.gh_get_phsp(self.ctx, int(sim_id), (A.FP * 8)(*ptrs), m, C.byref(n)), "gh_get_phsp")
         return arrs, int(n.value)
 
-    def get_event_edep(self, sim_id, n):
-        out = np.zeros(int(n), dtype=np.float32)
-        self._check(self.lib.gh_get_event_edep(self.ctx, int(sim_id), A.fptr(out), int(n)), "gh_get_event_edep")
+    def get_event_edep(self, sim_id, n, device=False):
+        """Energy deposited in the detector labels by each of the first n histories of the
+        last run (numpy, or a device tensor with device=True)."""
+        if device:
+            import torch
+
+            out = torch.empty(int(n), dtype=torch.float32, device=f"cuda:{self.device}")
+            ptr = C.cast(C.c_void_p(out.data_ptr()), A.FP)
+        else:
+            out = np.zeros(int(n), dtype=np.float32)
+            ptr = A.fptr(out)
+        self._check(self.lib.gh_get_event_edep(self.ctx, int(sim_id), ptr, int(n)), "gh_get_event_edep")
         return out
 
     def hu_to_labels(self, hu, bins, labels_sorted):

@@ -643,7 +643,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         float tt = fmaxf(tf, 0.0f);
         float e[3];
         for (int i = 0; i < 3; ++i) e[i] = fmaf(tt, ph.d[i], ph.p[i]);
-        if (S->tal->record_face >= 0 && face == S->tal->record_face) {
+        if (S->tal->record_face >= 0 && (face == S->tal->record_face || S->tal->record_face == GH_FACE_ALL)) {
           if (o->rec[0] && o->rec_n < o->rec_cap) {
             uint64_t k = o->rec_n;
             for (int i = 0; i < 3; ++i) { o->rec[i][k] = e[i]; o->rec[3 + i][k] = ph.d[i]; }
@@ -950,5 +950,97 @@ void ref_exitplane_decode(const float *Z, int64_t ldz, int64_t n, const float *q
     out[0][i] = px + offset[0]; out[1][i] = py + offset[1]; out[2][i] = plane_z + offset[2];
     out[3][i] = du; out[4][i] = dv; out[5][i] = dw;
     out[6][i] = E; out[7][i] = weight;
+  }
+}
+
+/* ------------------------------------------------------------------ SPECT heads
+ * Restatements of csrc/arf.hip. ref_spect_pairs visits every head of every photon in
+ * order (the GPU visits a window of candidate heads and writes pairs in atomic-slot
+ * order): the two produce the same set of pairs with the same FP32 values except
+ * theta / phi, which both take from the platform's acosf / atan2f. */
+void ref_spect_pairs(const float *const rec[8], int64_t n, const gh_spect_desc *D, const float *cs, int64_t capacity,
+                     float *x3, int32_t *pix, float *wgt, float *const det[8], float *tp, uint64_t *count) {
+  const float sin_max = sqrtf(fmaxf(fmaf(-D->cos_theta_max, D->cos_theta_max, 1.0f), 0.0f));
+  const float half_w = 0.5f * (float)D->nx * D->spacing[0], half_h = 0.5f * (float)D->ny * D->spacing[1];
+  const float a2x = 0.5f * D->pitch, a2y = 0.86602540378443865f * D->pitch;
+  uint64_t cnt = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const float px = rec[0][i] - D->center[0], py = rec[1][i] - D->center[1], pz = rec[2][i] - D->center[2];
+    const float du = rec[3][i], dv = rec[4][i], dw = rec[5][i], e = rec[6][i], ww = rec[7][i];
+    if (!(fabsf(dw) < sin_max && ww > 0.0f)) continue;
+    for (int k = 0; k < D->n_proj; ++k) {
+      const float ca = cs[2 * k], sa = cs[2 * k + 1];
+      const float c = fmaf(du, ca, dv * sa);
+      if (!(c > D->cos_theta_max)) continue;
+      const float s = fmaf(px, ca, py * sa);
+      const float t = (D->radius - s) / c;
+      if (!(t >= 0.0f)) continue;
+      const float dy = fmaf(-du, sa, dv * ca);
+      const float ex = fmaf(t, du, px), ey = fmaf(t, dv, py);
+      const float hu = fmaf(-ex, sa, ey * ca);
+      const float hz = fmaf(t, dw, pz);
+      const float f = D->crystal_depth / c;
+      const float bu = fmaf(f, dy, hu), bz = fmaf(f, dw, hz);
+      const float fx = (bu + half_w) / D->spacing[0], fy = (bz + half_h) / D->spacing[1];
+      if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)D->nx && fy < (float)D->ny)) continue;
+      int ix = (int)fx, iy = (int)fy;
+      if (ix > D->nx - 1) ix = D->nx - 1;
+      if (iy > D->ny - 1) iy = D->ny - 1;
+      const uint64_t slot = cnt++;
+      if ((int64_t)slot >= capacity) continue;
+      const float theta = acosf(fminf(c, 1.0f));
+      const float phi = atan2f(dy, dw);
+      if (x3) {
+        x3[3 * slot] = fmaf(e, D->norm[0], D->norm[1]);
+        x3[3 * slot + 1] = fmaf(theta, D->norm[2], D->norm[3]);
+        x3[3 * slot + 2] = fmaf(phi, D->norm[4], D->norm[5]);
+      }
+      if (tp) { tp[2 * slot] = theta; tp[2 * slot + 1] = phi; }
+      pix[slot] = (k * D->ny + iy) * D->nx + ix;
+      wgt[slot] = ww;
+      if (det) {
+        float rx = hz, ry = hu;
+        if (D->pitch > 0.0f) {
+          const float jj = floorf(ry / a2y);
+          ry = fmaf(-jj, a2y, ry);
+          rx = fmaf(-jj, a2x, rx);
+          const float ii = floorf(rx / D->pitch);
+          rx = fmaf(-ii, D->pitch, rx);
+        }
+        det[0][slot] = D->patch_origin[0] + rx;
+        det[1][slot] = D->patch_origin[1] + ry;
+        det[2][slot] = D->patch_origin[2];
+        det[3][slot] = dw;
+        det[4][slot] = dy;
+        det[5][slot] = -c;
+        det[6][slot] = e;
+        det[7][slot] = ww;
+      }
+    }
+  }
+  *count = cnt;
+}
+
+void ref_energy_windows(const float *edep, int64_t n, uint64_t seed, uint64_t first, float r0, float e0,
+                        const float *win, int n_win, int32_t *cls) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t i = 0; i < n; ++i) {
+    const float e = edep[i];
+    int c = 0;
+    if (e > 0.0f) {
+      const uint64_t id = first + (uint64_t)i;
+      const uint32_t ctr[4] = {(uint32_t)id, (uint32_t)(id >> 32), 0x424C5552u, 0u};
+      uint32_t x[4];
+      ref_philox4x32_10(ctr, key, x);
+      const float u0 = (float)((x[0] >> 8) | 1u) * 0x1p-24f, u1 = (float)((x[1] >> 8) | 1u) * 0x1p-24f;
+      float sn, csn;
+      ref_sincos(6.28318530717958647692f * u1, &sn, &csn);
+      const float g = sqrtf(-2.0f * ref_log(u0)) * csn;
+      const float sigma = (r0 * sqrtf(e0 * e)) * 0.42466090014400953f;
+      const float eb = fmaf(sigma, g, e);
+      for (int k = 0; k < n_win; ++k)
+        if (eb >= win[2 * k] && eb < win[2 * k + 1]) { c = k + 1; break; }
+    }
+    cls[i] = c;
   }
 }
