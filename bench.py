@@ -70,7 +70,11 @@ def main():
     ap.add_argument("--slow-min", type=int, default=16)
     ap.add_argument("--refill-min", type=int, default=16)
     ap.add_argument("--grid", type=int, default=0)
-    ap.add_argument("--workload", default="ct256", choices=("ct256", "batch64", "exitplane"))
+    ap.add_argument("--workload", default="ct256", choices=("ct256", "batch64", "exitplane", "spect16"))
+    ap.add_argument("--arf-train-photons", type=float, default=4e7,
+                    help="spect16: photons through the head model that label the ARF training set")
+    ap.add_argument("--arf-fit-steps", type=int, default=3000, help="spect16: ARF training steps")
+    ap.add_argument("--spect-histories", type=float, default=1e8, help="spect16: histories per phantom")
     ap.add_argument("--ep-train-photons", type=float, default=1e8,
                     help="exitplane: recorded collimator photons used to train the generator")
     ap.add_argument("--ep-fit-steps", type=int, default=4000, help="exitplane: generator training steps")
@@ -101,6 +105,8 @@ def main():
         return bench_batch(args)
     if args.workload == "exitplane":
         return bench_exitplane(args)
+    if args.workload == "spect16":
+        return bench_spect(args)
     eng = GpuEngine(0)
     tables, ph, src, tal, info = workloads.ct256(eng)
     eng.set_physics(tables)
@@ -352,6 +358,93 @@ def bench_exitplane(args):
     }
     col.close()
     ct.close()
+    print(json.dumps(line), flush=True)
+
+
+def bench_spect(args):
+    """16 phantoms in one batched launch, their escaping photons projected on 128 heads by
+    the learned detector response and by plain transport through the head model."""
+    import math
+
+    import torch
+
+    from opengate_amd import workloads as WL
+    from opengate_amd.learn.arf import ArfModel, Heads, HeadSimulator, compare_projections, project
+    from opengate_amd.transport import GpuEngine
+
+    theta_max = math.radians(10.0)
+    head = HeadSimulator()
+    n_train = int(args.arf_train_photons)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    rec, E, th, ph = head.training_photons(n_train, 1, theta_max)
+    cls = head.windows(rec, n_train, 3)
+    torch.cuda.synchronize()
+    t_label = time.perf_counter() - t
+    del rec
+    arf = ArfModel(theta_max)
+    X = arf.inputs(E, th, ph)
+    del E, th, ph
+    t = time.perf_counter()
+    arf.fit(X, cls, steps=args.arf_fit_steps, batch=16384, seed=2, log_every=1000)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t
+    del X, cls
+    _log(f"arf: labelled {n_train} photons in {t_label:.1f} s, fit in {t_fit:.1f} s")
+    tables, sims, info = WL.spect16(int(args.spect_histories))
+    eng = GpuEngine(0)
+    eng.set_physics(tables)
+    for ph_, src, tal, _, _ in sims:
+        eng.add_simulation(ph_, src, tal)
+    hist = [h for _, _, _, h, _ in sims]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    c = eng.run(hist, 4242)
+    torch.cuda.synchronize()
+    t_transport = time.perf_counter() - t
+    _log(f"phantoms: {sum(hist)} histories in {t_transport:.2f} s")
+    t_pairs = t_arf = t_plain = 0.0
+    pairs = 0
+    per = []
+    for k, (_, _, _, h, hk) in enumerate(sims):
+        esc, n_esc = eng.get_phsp(k, device=True)
+        heads = Heads(theta_max=theta_max, **hk)
+        res = project(esc, min(n_esc, len(esc[0])), heads, arf=arf, head_sim=head, seed=100 + k)
+        t_pairs += res["t_pairs"]
+        t_arf += res["t_arf"]
+        t_plain += res["t_plain"]
+        pairs += res["pairs"]
+        cmp = compare_projections(res["arf"].cpu().numpy(), res["plain"].cpu().numpy(), res["arf2"].cpu().numpy(),
+                                  res["plain2"].cpu().numpy())
+        per.append(cmp)
+        del esc, res
+        _log(f"phantom {k}: {cmp}")
+    total = sum(hist)
+    ratios = np.array([p["window_ratio"] for p in per])
+    chi2 = np.array([p["chi2_per_pixel"] for p in per if p.get("chi2_per_pixel") is not None])
+    learned_s = t_transport + t_pairs + t_arf
+    plain_s = t_transport + t_pairs + t_plain
+    line = {
+        "metric": METRIC, "value": total / learned_s, "unit": "photons/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": learned_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 transport + bf16 response network (fp32 accumulate)",
+        "data": "synthetic (seed-generated phantoms and head model)",
+        "config": {"workload": info["desc"], "histories_total": total,
+                   "form": "batched phantom transport + learned detector response (4x128 MLP) on 128 heads",
+                   "arf_training_photons": n_train, "arf_label_s": t_label, "arf_fit_s": t_fit,
+                   "phantom_transport_photons_per_s": total / t_transport,
+                   "pairs": pairs, "pairs_per_s": pairs / t_pairs,
+                   "arf_pairs_per_s": pairs / t_arf, "plain_pairs_per_s": pairs / t_plain,
+                   "arf_tflops": arf.flops_per_pair() * pairs / t_arf / 1e12,
+                   "plain_photons_per_s": total / plain_s, "gain_detector": t_plain / t_arf,
+                   "gain_end_to_end": plain_s / learned_s,
+                   "window_ratio_mean": ratios.mean(0).tolist(), "window_ratio_min": ratios.min(0).tolist(),
+                   "window_ratio_max": ratios.max(0).tolist(),
+                   "chi2_per_pixel_median": float(np.median(chi2)) if len(chi2) else None,
+                   "chi2_per_pixel_max": float(chi2.max()) if len(chi2) else None},
+    }
+    head.close()
+    eng.close()
     print(json.dumps(line), flush=True)
 
 

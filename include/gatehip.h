@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GH_ABI_VERSION 1
+#define GH_ABI_VERSION 2
 
 enum {
   GH_OK = 0,
@@ -56,7 +56,8 @@ enum { GH_DIR_ISO = 0, GH_DIR_MOMENTUM = 1, GH_DIR_FOCUSED = 2 };
 enum { GH_ENE_MONO = 0, GH_ENE_LINES = 1, GH_ENE_HISTOGRAM = 2, GH_ENE_RANGE = 3 };
 /* tally flags */
 enum { GH_TALLY_EDEP2 = 1, GH_TALLY_DOSE = 2, GH_TALLY_DOSE2 = 4, GH_TALLY_COUNTS = 8,
-       GH_TALLY_ALIGNED = 16 };
+       GH_TALLY_ALIGNED = 16,
+       GH_TALLY_TLE = 32 /* track-length (Woodcock collision) kerma estimator, TLEDoseActor */ };
 /* record_face value that records photons leaving through any face of the phantom box */
 enum { GH_FACE_ALL = 6 };
 
@@ -126,7 +127,8 @@ typedef struct gh_tally_desc {
                                 (vacuum world only) */
   uint32_t record_capacity;  /* max photons recorded */
   int32_t detector_label_min;/* -1 off, else per-history energy in labels >= this value */
-  int32_t reserved0;
+  float tle_energy_min;      /* GH_TALLY_TLE: photons at or below stop and deposit locally [MeV] */
+  float tle_energy_max;      /* GH_TALLY_TLE: estimator for photons at or below, analog above [MeV] */
 } gh_tally_desc;
 
 typedef struct gh_launch {

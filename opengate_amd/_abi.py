@@ -15,7 +15,7 @@ GH_OK, GH_EINVAL, GH_EHIP, GH_ESTATE, GH_ERANGE = 0, 1, 2, 3, 4
 POS = {"point": 0, "box": 1, "sphere": 2, "ellipsoid": 3, "disc": 4, "cylinder": 5, "phsp": 6}
 DIR = {"iso": 0, "momentum": 1, "focused": 2}
 ENE = {"mono": 0, "spectrum_discrete": 1, "spectrum_histogram": 2, "range": 3}
-TALLY_EDEP2, TALLY_DOSE, TALLY_DOSE2, TALLY_COUNTS, TALLY_ALIGNED = 1, 2, 4, 8, 16
+TALLY_EDEP2, TALLY_DOSE, TALLY_DOSE2, TALLY_COUNTS, TALLY_ALIGNED, TALLY_TLE = 1, 2, 4, 8, 16, 32
 
 FP = C.POINTER(C.c_float)
 DP = C.POINTER(C.c_double)
@@ -56,7 +56,7 @@ class TallyDesc(C.Structure):
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("flags", C.c_int32),
         ("corner", C.c_float * 3), ("spacing", C.c_float * 3),
         ("record_face", C.c_int32), ("record_capacity", C.c_uint32),
-        ("detector_label_min", C.c_int32), ("reserved0", C.c_int32),
+        ("detector_label_min", C.c_int32), ("tle_energy_min", C.c_float), ("tle_energy_max", C.c_float),
     ]
 
 
@@ -171,7 +171,7 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.gh_abi_version() != 1:
+    if lib.gh_abi_version() != 2:
         raise RuntimeError("libgatehip.so ABI version mismatch")
     if path is None:
         _LIB = lib

@@ -261,6 +261,35 @@ class SimulationStatisticsActor(ActorBase):
         p.write_text(json.dumps(out, indent=1, default=float))
 
 
+class TLEDoseActor(DoseActor):
+    """Track-length estimator dose (TLEDoseActor, actors/doseactors.py:722-775;
+    GateTLEDoseActor.cpp:200-300). On this path the estimator is the Woodcock collision
+    estimator of kerma: every tentative collision of a photon inside the phantom scores
+    w E mu_en(E)/mu_majorant(E), whose expectation per unit path is the track-length score
+    w E mu_en L of the reference, and the electrons set in motion by such photons deposit
+    nothing. mu_en/rho comes from the NIST table for both `database` values (EPDL tables
+    are not available here). Threshold types 'None' (all photons) and 'energy' (photons at
+    or below tle_threshold; above it, analog deposits) are supported."""
+
+    user_info_defaults = {
+        "energy_min": (0.0, {"doc": "photons at or below this energy deposit it locally and stop"}),
+        "tle_threshold": (np.inf, {"doc": "energy below which the estimator applies (type 'energy')"}),
+        "tle_threshold_type": ("None", {"allowed_values": ("None", "energy", "max range", "average range")}),
+        "database": ("EPDL", {"allowed_values": ("EPDL", "NIST")}),
+    }
+    actor_type = "TLEDoseActor"
+
+    def tle_energy_bounds(self):
+        """(energy_min, energy_max) in MeV for the tally descriptor."""
+        if self.score_in != "material":
+            raise ValueError(f"TLEDoseActor cannot score in {self.score_in}, only 'material' is allowed.")
+        t = self.tle_threshold_type
+        if t in ("max range", "average range"):
+            raise NotImplementedError(f"TLEDoseActor.tle_threshold_type '{t}' (needs electron CSDA ranges)")
+        e_max = float(self.tle_threshold) / g4_units.MeV if t == "energy" else float("inf")
+        return float(self.energy_min) / g4_units.MeV, e_max
+
+
 FACES = {"-x": 0, "+x": 1, "-y": 2, "+y": 3, "-z": 4, "+z": 5, "all": 6}
 
 

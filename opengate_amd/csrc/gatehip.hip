@@ -62,6 +62,9 @@ struct SimDev {
   float *rec[8];
   int det_label_min;
   float *event_edep;
+  // track-length (collision) kerma estimator
+  int tle;
+  float tle_emin, tle_emax;
   // run
   uint64_t n_hist, first_hist;
 };
@@ -391,7 +394,14 @@ __host__ __device__ inline LdsLayout lds_layout(int n_mat, int nb1, int nx, int 
   return o;
 }
 
-template <bool BATCHED, int STAGE, int BLOCK>
+// TLE: dose by the Woodcock collision estimator of kerma instead of analog deposits
+// (the role of GateTLEDoseActor, GateTLEDoseActor.cpp:200-300): every tentative
+// collision inside the phantom (real or virtual) of a photon at or below tle_emax scores
+// w E mu_en(E, m)/mu_maj(E) in its voxel, whose expectation per unit path is the
+// track-length estimate w E mu_en L; the energy such a photon gives to electrons at real
+// interactions is not deposited. Photons at or below tle_emin (and below the tracking
+// cut) deposit their energy where they are and stop. Single-simulation launches only.
+template <bool BATCHED, int STAGE, int BLOCK, bool TLE>
 __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
   extern __shared__ float lds[];
   const Phys &P = L.P;
@@ -431,6 +441,8 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
 
   Hot hot;
   if (!BATCHED) hot.load(L.sims[0]);
+  const float tle_emax = TLE ? L.sims[0].tle_emax : 0.0f;
+  const float tle_emin = TLE ? L.sims[0].tle_emin : 0.0f;
 
   Photon ph;
   ph.mode = MODE_BOX; ph.sim = 0; ph.hist = 0;
@@ -490,7 +502,7 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
               ev_edep = 0.0f;
               sq.n = 0;
               const bool inside = inside_box(hot, ph.x, ph.y, ph.z);
-              if (ph.E < P.e_cut) {
+              if (ph.E < P.e_cut || (TLE && ph.E <= tle_emin)) {
                 // below the tracking cut from the start: deposit locally (inside) and stop
                 if (inside) {
                   const int ix = (int)floorf(ph.x * hot.isx), iy = (int)floorf(ph.y * hot.isy),
@@ -544,6 +556,11 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
               const int vox = ix + hot.nx * (iy + hot.ny * iz);
               const int lab = hot.labels[vox];
               const float mu = lerp_node(mu_tab + lab * NB1, ec);
+              if (TLE && ph.E <= tle_emax) {
+                const SimDev &s0 = L.sims[0];
+                const float x = (ph.E * ph.wt) * (lerp_node(P.muen + lab * NB1, ec) * ec.inv_maj);
+                deposit(s0, sq, tally_voxel(s0, hot, vox, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
+              }
               if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
                 mode = M_COLLIDE;
                 cvox = vox;
@@ -641,6 +658,8 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
         const int tv = in_world ? -1 : tally_voxel(s, hot, cvox, ph.x, ph.y, ph.z);
         const float irho = P.inv_rho[lab];
         const bool det = !in_world && s.det_label_min >= 0 && lab >= s.det_label_min;
+        // electrons of photons scored by the kerma estimator give nothing locally
+        const bool tle_on = TLE && !in_world && ph.E <= tle_emax;
         mode = in_world ? M_WORLD : M_STEP;
         float x_dep = 0.0f;   // energy given to the medium at this collision
         if (r < pc) {
@@ -648,7 +667,7 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
           float q, c;
           sample_compton(rng, T, P.nx, lab, ph.E, q, c, st.rej);
           const float e1 = ph.E / q;
-          x_dep = (ph.E - e1) * ph.wt;
+          x_dep = tle_on ? 0.0f : (ph.E - e1) * ph.wt;
           float cp, sp;
           azimuth(rng, cp, sp);
           rotate(ph.u, ph.v, ph.w, c, cp, sp);
@@ -661,12 +680,12 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
           rotate(ph.u, ph.v, ph.w, c, cp, sp);
         } else {
           if (!in_world) st.inc(C_PHOTO);
-          x_dep = ph.E * ph.wt;
+          x_dep = tle_on ? 0.0f : ph.E * ph.wt;
           kill = true;
         }
         // local deposition: electron energy, and the photon itself below the tracking cut
         float x_cut = 0.0f;
-        if (!kill && ph.E < P.e_cut) {
+        if (!kill && (ph.E < P.e_cut || (TLE && !in_world && ph.E <= tle_emin))) {
           x_cut = ph.E * ph.wt;
           st.inc(C_CUT);
           kill = true;
@@ -780,6 +799,7 @@ struct SimHost {
 
 struct gh_ctx {
   int device = 0;
+  bool have_muen = false;
   std::string err;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1033,6 +1053,7 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
   for (size_t i = 0; i < M; ++i) inv_rho[i] = ph->density[i] > 0.0f ? 1.0f / ph->density[i] : 0.0f;
   std::vector<float> muen_zero;
   const float *muen = ph->mu_en;
+  ctx->have_muen = muen != nullptr;
   if (!muen) { muen_zero.assign(M * NB1, 0.0f); muen = muen_zero.data(); }
   Phys P{};
   float *t;
@@ -1094,6 +1115,12 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
   if (tal->record_face >= 0 && ph->world_material >= 0)
     FAIL(GH_EINVAL, "exit-face recording needs a vacuum world");
   if (tal->record_face > GH_FACE_ALL) FAIL(GH_EINVAL, "record_face must be -1..6");
+  if (tal->flags & GH_TALLY_TLE) {
+    if (tal->detector_label_min >= 0) FAIL(GH_EINVAL, "a TLE tally cannot score detector events");
+    if (!(tal->tle_energy_min >= 0.0f) || !(tal->tle_energy_max >= 0.0f))
+      FAIL(GH_EINVAL, "TLE energy bounds must be >= 0");
+    if (!ctx->have_muen) FAIL(GH_ESTATE, "a TLE tally needs mu_en tables in gh_set_physics");
+  }
   HIPCHK(hipSetDevice(ctx->device));
 
   SimHost h;
@@ -1156,6 +1183,9 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
   }
   d.det_label_min = tal->detector_label_min;
   h.det = tal->detector_label_min;
+  d.tle = (tal->flags & GH_TALLY_TLE) ? 1 : 0;
+  d.tle_emin = tal->tle_energy_min;
+  d.tle_emax = tal->tle_energy_max;
   ctx->sims.push_back(std::move(h));
   if (sim_id) *sim_id = (int32_t)ctx->sims.size() - 1;
   return gh_reset_tally(ctx, (int32_t)ctx->sims.size() - 1);
@@ -1284,14 +1314,20 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
     FAIL(GH_EINVAL, "gh_run: block must be a multiple of 64, <= 256, or 1024");
   const size_t shmem = stage == 2 ? ctx->lds2_bytes : (stage == 1 ? ctx->lds1_bytes : 0);
   const bool batched = ns > 1;
+  bool tle = false;
+  for (int s = 0; s < ns; ++s) tle = tle || ctx->sims[s].dev.tle;
+  if (tle && batched) FAIL(GH_EINVAL, "gh_run: TLE tallies need a single-simulation launch");
   const void *fn = nullptr;
-#define GH_PICK(B, S, BL) (const void *)&transport_kernel<B, S, BL>
-  if (!batched) {
-    fn = big ? (stage == 2 ? GH_PICK(false, 2, 1024) : GH_PICK(false, 1, 1024))
-             : (stage == 1 ? GH_PICK(false, 1, 256) : GH_PICK(false, 0, 256));
+#define GH_PICK(B, S, BL, TL) (const void *)&transport_kernel<B, S, BL, TL>
+  if (!batched && tle) {
+    fn = big ? (stage == 2 ? GH_PICK(false, 2, 1024, true) : GH_PICK(false, 1, 1024, true))
+             : (stage == 1 ? GH_PICK(false, 1, 256, true) : GH_PICK(false, 0, 256, true));
+  } else if (!batched) {
+    fn = big ? (stage == 2 ? GH_PICK(false, 2, 1024, false) : GH_PICK(false, 1, 1024, false))
+             : (stage == 1 ? GH_PICK(false, 1, 256, false) : GH_PICK(false, 0, 256, false));
   } else {
-    fn = big ? (stage == 2 ? GH_PICK(true, 2, 1024) : GH_PICK(true, 1, 1024))
-             : (stage == 1 ? GH_PICK(true, 1, 256) : GH_PICK(true, 0, 256));
+    fn = big ? (stage == 2 ? GH_PICK(true, 2, 1024, false) : GH_PICK(true, 1, 1024, false))
+             : (stage == 1 ? GH_PICK(true, 1, 256, false) : GH_PICK(true, 0, 256, false));
   }
 #undef GH_PICK
   if (shmem > 64 * 1024) HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));

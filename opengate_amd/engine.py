@@ -14,7 +14,7 @@ import time
 
 import numpy as np
 
-from .actors.dose import GY_FACTOR, DoseActor, PhaseSpaceActor, SimulationStatisticsActor, FACES
+from .actors.dose import FACES, GY_FACTOR, DoseActor, PhaseSpaceActor, SimulationStatisticsActor, TLEDoseActor
 from .geometry.volumes import BoxVolume, ImageVolume
 from .physics import xs
 from .sources.gan import GANSource
@@ -221,7 +221,15 @@ class SimulationEngine:
         return TallySpec(size=tuple(int(x) for x in size), spacing=tuple(spacing), corner=tuple(corner),
                          aligned=aligned, edep2=a.wants_squared(), dose=a.wants_dose() and not aligned,
                          dose2=a.wants_dose() and a.wants_squared() and not aligned,
-                         counts=a.counts.active, record_face=rec_face, record_capacity=rec_cap)
+                         counts=a.counts.active, record_face=rec_face, record_capacity=rec_cap,
+                         **self._tle_fields(a))
+
+    @staticmethod
+    def _tle_fields(a):
+        if not isinstance(a, TLEDoseActor):
+            return {}
+        e_min, e_max = a.tle_energy_bounds()
+        return {"tle": True, "tle_energy_min": e_min, "tle_energy_max": e_max}
 
     # ------------------------------------------------------------------ run
     def run(self):

@@ -11,7 +11,7 @@ from pathlib import Path
 
 import numpy as np
 
-from .actors.dose import DoseActor, PhaseSpaceActor, SimulationStatisticsActor
+from .actors.dose import DoseActor, PhaseSpaceActor, SimulationStatisticsActor, TLEDoseActor
 from .base import Box, GateObject
 from .geometry.volumes import BoxVolume, ImageVolume
 from .physics.materials import MaterialDatabase
@@ -165,8 +165,8 @@ class Simulation:
     def add_actor(self, actor_type, name):
         if name in self.actors:
             raise ValueError(f"an actor named '{name}' already exists")
-        table = {"DoseActor": DoseActor, "SimulationStatisticsActor": SimulationStatisticsActor,
-                 "PhaseSpaceActor": PhaseSpaceActor}
+        table = {"DoseActor": DoseActor, "TLEDoseActor": TLEDoseActor,
+                 "SimulationStatisticsActor": SimulationStatisticsActor, "PhaseSpaceActor": PhaseSpaceActor}
         if actor_type not in table:
             raise NotImplementedError(f"actor type '{actor_type}' is not part of the photon dose path")
         a = table[actor_type](name)

@@ -166,6 +166,9 @@ class TallySpec:
     record_face: int = -1
     record_capacity: int = 0
     detector_label_min: int = -1
+    tle: bool = False  # track-length (Woodcock collision) kerma estimator instead of analog deposits
+    tle_energy_min: float = 0.0  # MeV
+    tle_energy_max: float = float("inf")  # MeV
 
     def flags(self):
         f = 0
@@ -179,6 +182,8 @@ class TallySpec:
             f |= A.TALLY_COUNTS
         if self.aligned:
             f |= A.TALLY_ALIGNED
+        if self.tle:
+            f |= A.TALLY_TLE
         return f
 
     @property
@@ -194,6 +199,8 @@ class TallySpec:
         d.record_face = int(self.record_face)
         d.record_capacity = int(self.record_capacity)
         d.detector_label_min = int(self.detector_label_min)
+        d.tle_energy_min = float(self.tle_energy_min)
+        d.tle_energy_max = float(min(self.tle_energy_max, 3.0e38))
         return d
 
 

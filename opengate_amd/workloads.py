@@ -126,3 +126,29 @@ def exitplane(seed=4, e_max=0.1201, capacity=0):
     tal = TallySpec(size=(8, 8, 8), spacing=(8.0, 8.0, 85 * sp / 8), edep2=False, aligned=False, record_face=4,
                     record_capacity=int(capacity))
     return tables, ph, src, tal, {"desc": EXITPLANE_DESC, "names": [m.name for m in mats]}
+
+
+SPECT16_DESC = ("spect16: 16 phantoms of the batch64 kind (cubes of 64^3..192^3 voxels at 1.5-3 mm, water "
+                "ellipsoid with bone/lung/soft-tissue inserts) with 140.511 keV ellipsoid sources, {hist:.0e} "
+                "histories each in one batched launch; photons leaving each phantom are projected on 128 heads "
+                "(128x128 pixels of 4.8 mm, hexagonal-hole lead collimator 1.5/0.2/35 mm over 9.5 mm NaI)")
+
+
+def spect16(histories=100_000_000, seed=5, n_sims=16):
+    """Returns (tables, [(phantom, source, tally, histories, heads_kwargs), ...], info). The
+    tallies record every photon that leaves the phantom (vacuum world)."""
+    db = MaterialDatabase()
+    sims = P.batch_simulations(seed, n_sims)
+    tables = xs.build_tables([db.get(n) for n in P.BATCH_MATERIALS], 0.1406)
+    out = []
+    for s in sims:
+        n, sp = s["n"], s["spacing"]
+        L = n * sp
+        ph = PhantomSpec(s["labels"], (sp, sp, sp), world_material=-1)
+        src = SourceSpec(pos_type="ellipsoid", pos_param=tuple(s["source_half"]) + (0.0,),
+                         pos_trans=(L / 2, L / 2, L / 2), energy=0.140511)
+        tal = TallySpec(size=(n, n, n), spacing=(sp, sp, sp), edep2=False, record_face=6,
+                        record_capacity=int(histories))
+        heads = {"center": (L / 2, L / 2, L / 2), "radius": L * np.sqrt(2.0) / 2 + 10.0}
+        out.append((ph, src, tal, int(histories), heads))
+    return tables, out, {"desc": SPECT16_DESC.format(hist=histories)}

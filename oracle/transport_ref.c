@@ -550,8 +550,11 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
   t->hn = 0;
   float ev = 0.0f;
   const int det = S->tal->detector_label_min;
+  /* TLE: Woodcock collision estimator of kerma (see the kernel's TLE variant) */
+  const int tle = (S->tal->flags & GH_TALLY_TLE) != 0;
+  const float tle_emin = S->tal->tle_energy_min, tle_emax = S->tal->tle_energy_max;
   int in_box = voxel_of(S, ph.p) >= 0;
-  if (ph.E < d->e_cut) {
+  if (ph.E < d->e_cut || (tle && ph.E <= tle_emin)) {
     if (in_box) {
       int vox = voxel_of(S, ph.p);
       int lab = S->ph->labels[vox];
@@ -593,6 +596,10 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         st->c[K_STEPS]++;
         lab = S->ph->labels[vox];
         float mu = node_interp(d->mu_total + (size_t)lab * P->nb1, &ex);
+        if (tle && ph.E <= tle_emax) {
+          const float x = (ph.E * ph.w) * (node_interp(d->mu_en + (size_t)lab * P->nb1, &ex) * ex.inv_maj);
+          score(t, &sl, tally_index(S, vox, ph.p), x, x * P->inv_rho[lab], st);
+        }
         if ((float)((w[2 * k + 1] >> 8) | 1u) * 0x1p-24f * ex.maj < mu) { outcome = 1; break; }
       }
       if (outcome == 0) continue;
@@ -603,19 +610,22 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         int tv = tally_index(S, vox, ph.p);
         float irho = P->inv_rho[lab];
         int is_det = det >= 0 && lab >= det;
+        const int tle_on = tle && ph.E <= tle_emax;
         if (r < pc) {
           st->c[K_COMPT]++;
           float qq, c;
           compton_sample(&rs, P, lab, ph.E, &qq, &c, &st->c[K_REJ]);
           float e1 = ph.E / qq;
           float x = (ph.E - e1) * ph.w;
-          score(t, &sl, tv, x, x * irho, st);
-          if (is_det) ev += x;
+          if (!tle_on) {
+            score(t, &sl, tv, x, x * irho, st);
+            if (is_det) ev += x;
+          }
           float cp, sp;
           disc_azimuth(&rs, &cp, &sp);
           scatter_dir(ph.d, c, cp, sp);
           ph.E = e1;
-          if (ph.E < d->e_cut) {
+          if (ph.E < d->e_cut || (tle && ph.E <= tle_emin)) {
             float x2 = ph.E * ph.w;
             score(t, &sl, tv, x2, x2 * irho, st);
             if (is_det) ev += x2;
@@ -632,8 +642,10 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         } else {
           st->c[K_PHOTO]++;
           float x = ph.E * ph.w;
-          score(t, &sl, tv, x, x * irho, st);
-          if (is_det) ev += x;
+          if (!tle_on) {
+            score(t, &sl, tv, x, x * irho, st);
+            if (is_det) ev += x;
+          }
           break;
         }
       } else {

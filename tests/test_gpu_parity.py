@@ -154,6 +154,18 @@ def test_phase_space_source_on_device(engine):
             engine.add_simulation(ph, SourceSpec(pos_type="phsp", phsp=bad), tal)
 
 
+def test_tle_estimator_bit_exact(engine):
+    """Track-length (collision) kerma estimator: all photons, then only photons at or
+    below 80 keV (analog above), with an energy floor."""
+    T = tables()
+    ph = small_phantom(32, 2.0)
+    src = SourceSpec(pos_trans=centre(ph), energy=0.1405)
+    for kw in ({"tle": True}, {"tle": True, "tle_energy_max": 0.08, "tle_energy_min": 0.03}):
+        tal = aligned_tally(ph, **kw)
+        c, g, rc, r = run_both(engine, T, ph, src, tal, 100_000, 31)
+        compare(c, rc, g, r)
+
+
 def test_batched_launch_equals_separate_runs(engine):
     T = tables(e_max=0.21)
     specs = []
