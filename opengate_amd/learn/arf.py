@@ -332,11 +332,16 @@ class HeadSimulator:
         energy_windows(self.engine.lib, edep, n, seed, first, cls, windows)
         return cls
 
-    def training_photons(self, n, seed, theta_max, device="cuda"):
+    def training_photons(self, n, seed, theta_max, device="cuda", theta_power=2.0, e_focus=(0.5, 0.100)):
         """n photons uniformly over the lattice cell that gh_spect_pairs reduces entries to
         ([0, pitch) x [0, pitch sqrt(3)/2) from the lattice point at the patch centre), E
-        uniform in E_RANGE, cos(theta) uniform in [cos theta_max, 1], phi uniform in
-        (-pi, pi]: (rec, E, theta, phi)."""
+        theta = theta_max u^theta_power (u uniform), phi uniform in (-pi, pi], E uniform in
+        E_RANGE for a fraction 1 - e_focus[0] of the photons and uniform in
+        [e_focus[1], E_RANGE[1]] for the others: (rec, E, theta, phi). The network learns the
+        detection probability given (E, theta, phi), so the input distribution only decides
+        where it is trained most: with theta_power = 2 about half of the photons fall inside
+        the ~2.5 deg acceptance of the holes, and the energies of unscattered and once-
+        scattered 140 keV photons get half of the samples."""
         torch = _torch()
         g = torch.Generator(device=device).manual_seed(int(seed))
         m = self.model
@@ -345,9 +350,10 @@ class HeadSimulator:
         ox, oy, oz = m.origin
         x = ox + a * m.pitch
         y = oy + b * 0.5 * math.sqrt(3.0) * m.pitch
-        E = E_RANGE[0] + (E_RANGE[1] - E_RANGE[0]) * torch.rand(n, device=device, generator=g)
-        ct = 1.0 - (1.0 - math.cos(theta_max)) * torch.rand(n, device=device, generator=g)
-        theta = torch.acos(ct)
+        lo = torch.where(torch.rand(n, device=device, generator=g) < e_focus[0], e_focus[1], E_RANGE[0])
+        E = lo + (E_RANGE[1] - lo) * torch.rand(n, device=device, generator=g)
+        theta = theta_max * torch.rand(n, device=device, generator=g) ** theta_power
+        ct = torch.cos(theta)
         phi = (2.0 * torch.rand(n, device=device, generator=g) - 1.0) * math.pi
         st = torch.sqrt(1.0 - ct * ct)
         rec = [x, y, torch.full_like(x, oz), st * torch.cos(phi), st * torch.sin(phi), -ct, E, torch.ones_like(x)]
@@ -424,12 +430,22 @@ def spect_pairs_checked(lib, rec, n, heads, cs, buf):
         raise
 
 
-def compare_projections(test, ref, test2=None, ref2=None, min_counts=20.0):
+def compare_projections(test, ref, test2=None, ref2=None, min_counts=20.0, block=1):
     """Agreement of two projection sets [n_proj * n_win, ny, nx] (numpy): per-window total
-    ratio and, over pixels with at least min_counts reference counts, the mean of
-    (test - ref)^2 / (var_test + var_ref) (about 1 when the two agree within their noise)."""
-    out = {}
+    ratio and, over pixels (block x block pixel sums) with at least min_counts reference
+    counts, the mean of (test - ref)^2 / (var_test + var_ref), about 1 when the two agree
+    within their noise; also per window."""
+    out = {"block": block}
     n_win = len(WINDOWS)
+    if block > 1:
+        def red(a):
+            if a is None:
+                return None
+            h, w = a.shape[-2] // block * block, a.shape[-1] // block * block
+            a = a[..., :h, :w]
+            return a.reshape(*a.shape[:-2], h // block, block, w // block, block).sum((-3, -1))
+
+        test, ref, test2, ref2 = red(test), red(ref), red(test2), red(ref2)
     t = test.reshape(-1, n_win, *test.shape[-2:])
     r = ref.reshape(-1, n_win, *ref.shape[-2:])
     out["window_ratio"] = [float(t[:, k].sum() / max(r[:, k].sum(), 1e-300)) for k in range(n_win)]
@@ -439,4 +455,13 @@ def compare_projections(test, ref, test2=None, ref2=None, min_counts=20.0):
         m = ref >= min_counts
         out["pixels"] = int(m.sum())
         out["chi2_per_pixel"] = float(np.mean((test[m] - ref[m]) ** 2 / np.maximum(v[m], 1e-300))) if m.any() else None
+        t4 = test.reshape(-1, n_win, *test.shape[-2:])
+        r4 = ref.reshape(-1, n_win, *ref.shape[-2:])
+        v4 = v.reshape(-1, n_win, *v.shape[-2:])
+        per = []
+        for k in range(n_win):
+            mk = r4[:, k] >= min_counts
+            per.append(float(np.mean((t4[:, k][mk] - r4[:, k][mk]) ** 2 / np.maximum(v4[:, k][mk], 1e-300)))
+                       if mk.any() else None)
+        out["chi2_per_pixel_window"] = per
     return out
