@@ -206,6 +206,19 @@ int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t
                  int32_t M, int32_t N, int32_t K, const float *bias, const float *zprev, int64_t ldz,
                  float *C32, void *C16, int64_t ldc, int32_t mode, void *stream);
 
+/* Inference of a whole dense network in one kernel (activations stay in LDS): layers
+ * Z = X W_l + b_l, SiLU between layers, FP32 output of the last layer. X: M rows of bf16
+ * (pitch ldx, first k_in <= 32 columns used); wt[l]: device bf16 weights transposed,
+ * [N_l][K_l] with K contiguous, the first layer padded to K = 32 and the last to N = 16
+ * (zeros); bias[l]: device FP32 [N_l]; hidden widths all equal to `hidden` (128 or 256);
+ * 2..8 layers; Z: M x ldz FP32, columns 0..n_out-1 (n_out <= 16) written. Same values as
+ * the layer-by-layer gh_gemm_bf16 path. (Replaces the torch forward of the generator in
+ * GANSourceDefaultGenerator.generator, sources/gansources.py:661-669, and of the ARF
+ * network in ARFActor.arf_build_image_from_projected_points, actors/arfactors.py:409-421.) */
+int gh_mlp_forward_bf16(const void *X, int64_t ldx, int64_t M, int32_t n_layers, int32_t k_in, int32_t hidden,
+                        int32_t n_out, const void *const *wt, const float *const *bias, float *Z, int64_t ldz,
+                        void *stream);
+
 /* Learned exit-plane source (replaces the generator call of GANSourceDefaultGenerator.generator,
  * sources/gansources.py:644-684, and its copy/move of the keys, :686-757).
  * gh_gaussian_bf16: out[r*ld + c] = N(0,1) noise in bf16 for rows first_row .. first_row+n_rows-1

@@ -135,6 +135,8 @@ SIGNATURES = {
                                    C.c_void_p]),
     "gh_exitplane_decode": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int32, C.c_float, FP,
                                       C.c_float, FP, C.c_float, FP * 8, C.c_void_p]),
+    "gh_mlp_forward_bf16": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_void_p), C.POINTER(FP), FP, C.c_int64, C.c_void_p]),
     "gh_spect_pairs": (C.c_int, [FP * 8, C.c_int64, C.POINTER(SpectDesc), C.c_void_p, C.c_int64, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.POINTER(FP), C.c_void_p, C.c_void_p, C.c_void_p]),
     "gh_energy_windows": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_float, C.c_float, FP,

@@ -192,3 +192,153 @@ extern "C" int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void 
                      (const __bf16 *)B, sbk, sbn, M, N, K, bias, zprev, ldz, C32, (__bf16 *)C16, ldc, mode);
   return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
 }
+
+// ---------------------------------------------------------------------------------------
+// Fused MLP inference: all layers of a dense network in one kernel, activations kept in
+// LDS. Layer l: Z = X W_l + b_l; hidden layers X_{l+1} = bf16(silu(Z)), the last layer is
+// written as FP32. Weights are given transposed, Wt_l[n][k] (k contiguous), so that each
+// lane's MFMA B fragment (column n, 8 consecutive k) is one 16-byte load from L2-resident
+// memory. A workgroup of 4 waves owns FBM = 64 rows; wave w computes the hidden columns
+// [w H/4, (w+1) H/4) for all 64 rows (4 row tiles x H/64 column tiles of 16x16), reading
+// its A fragments (row, 8 consecutive k) from the LDS activation tile. Per 32-wide k step
+// the B fragments of the next step are loaded before the MFMAs of the current one.
+// The per-element arithmetic (MFMA over k slabs of 32 in order, bias, SiLU, bf16 rounding)
+// is the same as the layer-by-layer GEMM path, so both give bit-identical outputs.
+namespace {
+
+constexpr int FBM = 64;          // rows per workgroup
+constexpr int MAXL = 8;          // layers
+constexpr int K0 = 32;           // first-layer input width (padded)
+constexpr int NOUT = 16;         // last-layer output width (padded)
+
+struct MlpArgs {
+  const __bf16 *wt[MAXL];        // [N_l][K_l]
+  const float *bias[MAXL];       // [N_l]
+  int n_layers;
+  int k_in;                      // valid input columns (<= K0)
+  int n_out;                     // valid output columns (<= NOUT)
+};
+
+template <int H>
+__global__ void __launch_bounds__(256) mlp_forward_kernel(const __bf16 *__restrict__ X, int64_t ldx, int64_t M,
+                                                          MlpArgs a, float *__restrict__ Z, int64_t ldz) {
+  constexpr int P = H + 8;                 // LDS row pitch (bf16)
+  constexpr int CT = H / 64;               // 16-wide column tiles per wave (hidden layers)
+  __shared__ __bf16 act[2][FBM * P];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = (int64_t)blockIdx.x * FBM;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+
+  // input tile -> act[0][:, 0:K0] (zero beyond k_in and beyond M)
+  for (int i = tid; i < FBM * K0; i += 256) {
+    const int r = i / K0, k = i - r * K0;
+    const int64_t gr = row0 + r;
+    act[0][r * P + k] = (gr < M && k < a.k_in) ? X[gr * ldx + k] : (__bf16)0.0f;
+  }
+  __syncthreads();
+
+  int cur = 0;
+  for (int l = 0; l < a.n_layers; ++l) {
+    const bool last = l == a.n_layers - 1;
+    const int K = l == 0 ? K0 : H;
+    const __bf16 *__restrict__ W = a.wt[l];
+    const float *__restrict__ bias = a.bias[l];
+    const __bf16 *in = act[cur];
+    if (!last) {
+      __bf16 *out = act[cur ^ 1];
+      floatx4 acc[4][CT];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < CT; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      const int n0 = wave * (H / 4);
+      bf16x8 bq[CT], bn[CT];
+#pragma unroll
+      for (int j = 0; j < CT; ++j) bq[j] = *(const bf16x8 *)(W + (int64_t)(n0 + j * 16 + fr) * K + fk);
+      for (int k0 = 0; k0 < K; k0 += 32) {
+        if (k0 + 32 < K) {
+#pragma unroll
+          for (int j = 0; j < CT; ++j) bn[j] = *(const bf16x8 *)(W + (int64_t)(n0 + j * 16 + fr) * K + k0 + 32 + fk);
+        }
+        bf16x8 af[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8 *)(in + (i * 16 + fr) * P + k0 + fk);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < CT; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bq[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < CT; ++j) bq[j] = bn[j];
+      }
+      // epilogue: bias + SiLU -> bf16 activation tile
+#pragma unroll
+      for (int j = 0; j < CT; ++j) {
+        const int c = n0 + j * 16 + fr;
+        const float b = bias[c];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = i * 16 + (lane >> 4) * 4 + r;
+            out[row * P + c] = f2bf(silu(acc[i][j][r] + b));
+          }
+      }
+      __syncthreads();
+      cur ^= 1;
+    } else if (wave == 0) {
+      // output layer: NOUT = 16 columns, one column tile, computed by wave 0
+      floatx4 acc[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int k0 = 0; k0 < K; k0 += 32) {
+        const bf16x8 b = *(const bf16x8 *)(W + (int64_t)fr * K + k0 + fk);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const bf16x8 af = *(const bf16x8 *)(in + (i * 16 + fr) * P + k0 + fk);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b, acc[i], 0, 0, 0);
+        }
+      }
+      const int c = fr;
+      if (c < a.n_out) {
+        const float b = bias[c];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int64_t row = row0 + i * 16 + (lane >> 4) * 4 + r;
+            if (row < M) Z[row * ldz + c] = acc[i][r] + b;
+          }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int gh_mlp_forward_bf16(const void *X, int64_t ldx, int64_t M, int32_t n_layers, int32_t k_in,
+                                   int32_t hidden, int32_t n_out, const void *const *wt, const float *const *bias,
+                                   float *Z, int64_t ldz, void *stream) {
+  if (!X || !wt || !bias || !Z || M < 0 || n_layers < 2 || n_layers > MAXL || k_in <= 0 || k_in > K0 ||
+      n_out <= 0 || n_out > NOUT || ldz < n_out || ldx < k_in)
+    return GH_EINVAL;
+  if (hidden != 128 && hidden != 256) return GH_EINVAL;
+  MlpArgs a{};
+  a.n_layers = n_layers;
+  a.k_in = k_in;
+  a.n_out = n_out;
+  for (int l = 0; l < n_layers; ++l) {
+    if (!wt[l] || !bias[l]) return GH_EINVAL;
+    a.wt[l] = (const __bf16 *)wt[l];
+    a.bias[l] = bias[l];
+  }
+  if (M == 0) return GH_OK;
+  const dim3 grid((unsigned)((M + FBM - 1) / FBM));
+  if (hidden == 256)
+    hipLaunchKernelGGL(mlp_forward_kernel<256>, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)X, ldx, M,
+                       a, Z, ldz);
+  else
+    hipLaunchKernelGGL(mlp_forward_kernel<128>, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)X, ldx, M,
+                       a, Z, ldz);
+  return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
+}

@@ -270,12 +270,16 @@ class ArfModel:
         plane = img.shape[-1] * img.shape[-2]
         c = min(int(chunk), max(m, 1))
         Z = torch.empty(c, self.mlp.psizes[-1], device=self.dev)
-        if bufs is None:
+        fused = self.mlp.fused_ok()
+        if bufs is None and not fused:
             bufs = [torch.empty(c, max(self.mlp.psizes[1:-1]), device=self.dev, dtype=torch.bfloat16)
                     for _ in range(2)]
         for s in range(0, m, c):
             k = min(c, m - s)
-            self.mlp.infer(buf.x16[s:s + k], Z[:k], bufs)
+            if fused:
+                self.mlp.infer_fused(buf.x16[s:s + k], Z[:k])
+            else:
+                self.mlp.infer(buf.x16[s:s + k], Z[:k], bufs)
             rc = self.lib.gh_arf_accumulate(_ptr(Z), Z.shape[1], k, C.c_void_p(buf.pix.data_ptr() + 4 * s),
                                             C.c_void_p(buf.wgt.data_ptr() + 4 * s), self.n_win, plane, _ptr(img),
                                             _ptr(img2), _stream())
@@ -387,7 +391,7 @@ def project(rec, n, heads, arf=None, head_sim=None, seed=0, chunk=1 << 22, squar
     step = int(chunk)
     buf = PairBuffer(int(step * pairs_per_photon) + 1024, dev, det=head_sim is not None, x16=arf is not None)
     bufs = None
-    if arf is not None:
+    if arf is not None and not arf.mlp.fused_ok():
         c = min(1 << 21, buf.capacity)
         bufs = [torch.empty(c, max(arf.mlp.psizes[1:-1]), device=dev, dtype=torch.bfloat16) for _ in range(2)]
     s = 0

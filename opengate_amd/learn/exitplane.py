@@ -3,8 +3,9 @@
 The generator maps 8-D Gaussian noise to 6 values (E, x, y, u', v', w') through a
 6-layer dense network (width 256, BF16 weights, FP32 accumulation), evaluated and
 trained with the library's own kernels (gh_gaussian_bf16 for the noise, gh_gemm_bf16
-for every product of the forward and backward passes, gh_exitplane_decode for the
-mapping back to photons). It replaces the upstream transport that produced the
+for every product of the training forward and backward passes, gh_mlp_forward_bf16 for
+generation (all six layers in one kernel), gh_exitplane_decode for the mapping back to
+photons). It replaces the upstream transport that produced the
 exit-plane photons (the role of GANSource + GANSourceDefaultGenerator,
 sources/gansources.py:270-431,457-757).
 
@@ -202,14 +203,21 @@ class ExitPlaneGenerator:
             out = [torch.empty(n, device=self.dev, dtype=torch.float32) for _ in range(8)]
         m = min(int(chunk), max(n, 1))
         X = torch.empty(m, self.mlp.psizes[0], device=self.dev, dtype=torch.bfloat16)
-        bufs = [torch.empty(m, max(self.mlp.psizes[1:-1]), device=self.dev, dtype=torch.bfloat16) for _ in range(2)]
+        bufs = None
+        if not self.mlp.fused_ok():
+            bufs = [torch.empty(m, max(self.mlp.psizes[1:-1]), device=self.dev, dtype=torch.bfloat16)
+                    for _ in range(2)]
         Z = torch.empty(m, self.mlp.psizes[-1], device=self.dev, dtype=torch.float32)
         focal = (C.c_float * 3)(*self.focal)
         zero = (C.c_float * 3)(0.0, 0.0, 0.0)
+        fused = self.mlp.fused_ok()
         for c0 in range(0, n, m):
             k = min(m, n - c0)
             self.noise(k, seed, first=first + c0, out=X[:k])
-            self.mlp.infer(X[:k], Z[:k], bufs)
+            if fused:
+                self.mlp.infer_fused(X[:k], Z[:k])
+            else:
+                self.mlp.infer(X[:k], Z[:k], bufs)
             ptrs = (A.FP * 8)(*[C.cast(C.c_void_p(a.data_ptr() + 4 * c0), A.FP) for a in out])
             rc = self.lib.gh_exitplane_decode(_ptr(Z), Z.shape[1], k, _ptr(self.qmap.qtab), self.qmap.n_knots,
                                               self.qmap.g_max, focal, self.plane_z, zero, float(weight), ptrs,

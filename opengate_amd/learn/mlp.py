@@ -53,6 +53,7 @@ class MLP:
     # ------------------------------------------------------------------ helpers
     def _refresh(self):
         self.W16 = [w.to(_torch().bfloat16).contiguous() for w in self.W]
+        self._fused = None
 
     def _stream(self):
         return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
@@ -115,6 +116,52 @@ class MLP:
             else:
                 self._gemm(cur, K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c32=out, ldc=out.stride(0),
                            mode=EPI_BIAS)
+        return out
+
+    # ------------------------------------------------------------------ fused inference
+    FUSED_K_IN, FUSED_N_OUT, FUSED_MAX_LAYERS = 32, 16, 8
+
+    def fused_ok(self):
+        """gh_mlp_forward_bf16 handles 2..8 layers with equal hidden widths of 128 or 256,
+        at most 32 inputs and 16 outputs."""
+        hidden = set(self.psizes[1:-1])
+        return (2 <= len(self.W) <= self.FUSED_MAX_LAYERS and len(hidden) == 1 and hidden <= {128, 256}
+                and self.psizes[0] <= self.FUSED_K_IN and self.psizes[-1] <= self.FUSED_N_OUT)
+
+    def _fused_weights(self):
+        """Transposed, padded bf16 weights (first layer K -> 32, last layer N -> 16) and
+        biases (last layer -> 16), rebuilt after each optimiser step."""
+        torch = _torch()
+        if self._fused is None:
+            L = len(self.W16)
+            wts, bs = [], []
+            for l, w16 in enumerate(self.W16):
+                K, N = w16.shape
+                Kp = self.FUSED_K_IN if l == 0 else K
+                Np = self.FUSED_N_OUT if l == L - 1 else N
+                t = torch.zeros(Np, Kp, device=self.dev, dtype=torch.bfloat16)
+                t[:N, :K] = w16.t()
+                wts.append(t.contiguous())
+                b = torch.zeros(Np, device=self.dev)
+                b[:N] = self.b[l]
+                bs.append(b)
+            self._fused = (wts, bs)
+        return self._fused
+
+    def infer_fused(self, X, out):
+        """All layers in one kernel (gh_mlp_forward_bf16): X (M, in padded) bf16 with row
+        pitch X.stride(0), out (M, >= out) FP32 with row pitch out.stride(0); only the
+        first sizes[-1] columns of out are written. Same values as infer()."""
+        wts, bs = self._fused_weights()
+        L = len(wts)
+        wp = (C.c_void_p * L)(*[t.data_ptr() for t in wts])
+        bp = (C.POINTER(C.c_float) * L)(*[C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_float)) for t in bs])
+        rc = self.lib.gh_mlp_forward_bf16(C.c_void_p(X.data_ptr()), X.stride(0), X.shape[0], L, self.psizes[0],
+                                          self.psizes[1], self.sizes[-1], wp, bp,
+                                          C.cast(C.c_void_p(out.data_ptr()), C.POINTER(C.c_float)),
+                                          out.stride(0), self._stream())
+        if rc != 0:
+            raise RuntimeError(f"gh_mlp_forward_bf16 failed ({rc})")
         return out
 
     def backward(self, grad_out):

@@ -101,3 +101,35 @@ def test_mlp_trains_a_regression():
         net.step(dW, db)
         losses.append(float(loss))
     assert losses[-1] < 0.1 * losses[0]
+
+
+@pytest.mark.parametrize("sizes,M", [([8, 256, 256, 256, 256, 256, 6], 1000), ([3, 128, 128, 128, 4], 4099),
+                                     ([8, 256, 256, 6], 64), ([5, 128, 128, 16], 130)])
+def test_fused_forward_equals_layered(sizes, M):
+    """gh_mlp_forward_bf16 (all layers in one kernel) gives the layer-by-layer GEMM values
+    bit for bit, and both agree with the FP64 model of the bf16 network."""
+    torch = _torch()
+    from opengate_amd.learn.mlp import MLP
+
+    print(f"[gpu case] fused mlp {sizes} M={M}", flush=True)
+    net = MLP(sizes, seed=7)
+    for b in net.b:
+        b.normal_(0.0, 0.3)
+    net._refresh()
+    assert net.fused_ok()
+    g = torch.Generator().manual_seed(M)
+    X = torch.zeros(M, net.psizes[0], dtype=torch.bfloat16)
+    X[:, : sizes[0]] = torch.randn(M, sizes[0], generator=g).to(torch.bfloat16)
+    X = X.cuda()
+    Zl = torch.full((M, net.psizes[-1]), float("nan"), device="cuda")
+    bufs = [torch.empty(M, max(net.psizes[1:-1]), device="cuda", dtype=torch.bfloat16) for _ in range(2)]
+    net.infer(X, Zl, bufs)
+    Zf = torch.full((M, net.psizes[-1]), float("nan"), device="cuda")
+    net.infer_fused(X, Zf)
+    torch.cuda.synchronize()
+    n = sizes[-1]
+    np.testing.assert_array_equal(Zf[:, :n].cpu().numpy(), Zl[:, :n].cpu().numpy())
+    W = [w.double().cpu().numpy() for w in net.W]
+    bb = [b.double().cpu().numpy() for b in net.b]
+    zr, _, _ = R.forward(W, bb, X.float().cpu().numpy())
+    assert np.max(np.abs(Zf[:, :n].cpu().numpy() - zr[:, :n])) < 0.05 * max(1.0, np.abs(zr).max())
