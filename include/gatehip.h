@@ -202,6 +202,9 @@ enum { GH_EPI_STORE = 0,      /* C32 = acc */
        GH_EPI_BIAS_SILU = 1,  /* C32 = acc + bias (optional), C16 = silu(acc + bias) */
        GH_EPI_BIAS = 2,       /* C32 = acc + bias and/or C16 = bf16(acc + bias) */
        GH_EPI_DSILU = 3 };    /* C32 = acc * silu'(zprev) (optional), C16 = bf16(same) */
+/* In GH_EPI_STORE mode a long reduction over few output tiles (weight gradients) is split
+ * across workgroups and the slices are summed in a fixed order (library-owned workspace,
+ * allocated on first use). */
 int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
                  int32_t M, int32_t N, int32_t K, const float *bias, const float *zprev, int64_t ldz,
                  float *C32, void *C16, int64_t ldc, int32_t mode, void *stream);

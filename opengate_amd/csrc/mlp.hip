@@ -24,6 +24,9 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <mutex>
+
 #include "../../include/gatehip.h"
 
 namespace {
@@ -73,12 +76,14 @@ __device__ __forceinline__ void load8(const __bf16 *base, int64_t s_row, int64_t
   }
 }
 
+// blockIdx.z selects a slice [z k_chunk, (z+1) k_chunk) of the reduction (split-K, store
+// mode only): slice z writes its partial sums at C32 + z * part_stride.
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(const __bf16 *__restrict__ A, int64_t sam, int64_t sak,
                                                         const __bf16 *__restrict__ B, int64_t sbk, int64_t sbn,
                                                         int M, int N, int K, const float *__restrict__ bias,
                                                         const float *__restrict__ zprev, int64_t ldz,
                                                         float *__restrict__ C32, __bf16 *__restrict__ C16,
-                                                        int64_t ldc, int mode) {
+                                                        int64_t ldc, int mode, int k_chunk, int64_t part_stride) {
   __shared__ __bf16 As[BM * LDK];  // [m][k]
   __shared__ __bf16 Bs[BN * LDK];  // [n][k]
   const int tid = threadIdx.x;
@@ -96,7 +101,10 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const __bf16 *__restrict
   const bool a_along_k = sak == 1 || sam != 1;
   const bool b_along_k = sbk == 1 || sbn != 1;
 
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  const int k_begin = blockIdx.z * k_chunk;
+  const int k_end = min(K, k_begin + k_chunk);
+  if (part_stride) C32 += (int64_t)blockIdx.z * part_stride;
+  for (int k0 = k_begin; k0 < k_end; k0 += BK) {
     // ---- stage A slab [64 x 32]
     {
       __bf16 v[8];
@@ -174,6 +182,23 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const __bf16 *__restrict
     }
 }
 
+// C32[m, n] = sum over the split-K slices in order (deterministic)
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(const float *__restrict__ part, int S, int64_t stride,
+                                                            int M, int N, float *__restrict__ C32, int64_t ldc) {
+  const int64_t total = (int64_t)M * N;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int m = (int)(i / N), n = (int)(i - (int64_t)m * N);
+    float acc = 0.0f;
+    for (int z = 0; z < S; ++z) acc += part[(int64_t)z * stride + i];
+    C32[(int64_t)m * ldc + n] = acc;
+  }
+}
+
+// split-K workspace, grown on demand (one per process; the calls are stream-ordered)
+std::mutex g_ws_mutex;
+float *g_ws = nullptr;
+size_t g_ws_bytes = 0;
+
 }  // namespace
 
 extern "C" int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
@@ -187,9 +212,42 @@ extern "C" int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void 
   if (mode == GH_EPI_BIAS && !C16 && !C32) return GH_EINVAL;
   if (mode == GH_EPI_DSILU && !zprev) return GH_EINVAL;
   if (M == 0 || N == 0) return GH_OK;
+  const int tiles = ((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+  // weight gradients (few output tiles, long reduction): split the reduction so that the
+  // launch fills the chip, then sum the slices in a fixed order
+  int S = 1;
+  if (mode == GH_EPI_STORE && tiles < 512 && K >= 2048) {
+    S = std::min((K + 511) / 512, std::max(1, 1024 / tiles));
+    S = std::max(S, 1);
+  }
+  if (S > 1) {
+    int kc = (K + S - 1) / S;
+    kc = (kc + BK - 1) / BK * BK;
+    S = (K + kc - 1) / kc;
+    const size_t need = (size_t)S * M * N * sizeof(float);
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if (g_ws_bytes < need) {
+      if (g_ws) (void)hipFree(g_ws);
+      g_ws = nullptr;
+      g_ws_bytes = 0;
+      if (hipMalloc(&g_ws, need) != hipSuccess) return GH_EHIP;
+      g_ws_bytes = need;
+    }
+    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, S);
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)A, sam, sak,
+                       (const __bf16 *)B, sbk, sbn, M, N, K, bias, zprev, ldz, g_ws, (__bf16 *)nullptr,
+                       (int64_t)N, mode, kc, (int64_t)M * N);
+    if (hipGetLastError() != hipSuccess) return GH_EHIP;
+    const int64_t total = (int64_t)M * N;
+    const int rgrid = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rgrid), dim3(256), 0, (hipStream_t)stream, g_ws, S,
+                       (int64_t)M * N, M, N, C32, ldc);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
+  }
   dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
   hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)A, sam, sak,
-                     (const __bf16 *)B, sbk, sbn, M, N, K, bias, zprev, ldz, C32, (__bf16 *)C16, ldc, mode);
+                     (const __bf16 *)B, sbk, sbn, M, N, K, bias, zprev, ldz, C32, (__bf16 *)C16, ldc, mode,
+                     ((K + BK - 1) / BK) * BK, (int64_t)0);
   return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
 }
 

@@ -13,7 +13,8 @@ def _torch():
     return torch
 
 
-@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 70, 40), (257, 256, 256), (33, 8, 8), (96, 6, 3)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 70, 40), (257, 256, 256), (33, 8, 8), (96, 6, 3),
+                                   (128, 136, 8192)])
 def test_gemm_modes_and_layouts(M, N, K):
     torch = _torch()
     from opengate_amd.learn.mlp import EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU, EPI_STORE, MLP
