@@ -64,6 +64,7 @@ __global__ void __launch_bounds__(256) spect_pairs_kernel(const float *__restric
                                                           const float *__restrict__ E, const float *__restrict__ wt,
                                                           int64_t n, gh_spect_desc D, const float *__restrict__ cs,
                                                           PairOut o) {
+  __shared__ int owner_lane[4][64];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const float two_pi = 6.28318530717958647692f;
   const float dalpha = two_pi / (float)D.n_proj;
@@ -75,8 +76,8 @@ __global__ void __launch_bounds__(256) spect_pairs_kernel(const float *__restric
   const int jlo = all_heads ? 0 : -margin, jhi = all_heads ? D.n_proj - 1 : margin;
   const float half_w = 0.5f * (float)D.nx * D.spacing[0], half_h = 0.5f * (float)D.ny * D.spacing[1];
   const float a2x = 0.5f * D.pitch, a2y = 0.86602540378443865f * D.pitch;
-  // every lane of the wave runs the same number of candidate iterations so that the
-  // slot assignment can use wave-wide ballots
+  // every lane of a wave runs the same rounds so that slots are assigned with wave-wide
+  // ballots
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n;
@@ -93,12 +94,33 @@ __global__ void __launch_bounds__(256) spect_pairs_kernel(const float *__restric
         k0 = (int)floorf(beta / dalpha + 0.5f);
       }
     }
-    for (int j = jlo; j <= jhi; ++j) {
+    // expand the (candidate photon, head) work items of the wave over its 64 lanes: lane
+    // l of round r takes item 64 r + l = (candidate ci, head offset j); the candidate's
+    // data comes from its owner lane by a cross-lane read
+    const uint64_t cmask = __ballot(cand);
+    const int n_cand = __popcll(cmask);
+    if (n_cand == 0) continue;
+    if (cand) owner_lane[threadIdx.x >> 6][mbcnt(cmask)] = (int)lane_id();
+    __builtin_amdgcn_wave_barrier();
+    const int width = jhi - jlo + 1;
+    const int n_items = n_cand * width;
+    for (int it0 = 0; it0 < n_items; it0 += 64) {
+      const int item = it0 + (int)lane_id();
+      const bool act = item < n_items;
+      const int ci = act ? item / width : 0;
+      const int j = jlo + (act ? item - ci * width : 0);
+      const int src = owner_lane[threadIdx.x >> 6][ci];
+      const float qx = __shfl(px, src, 64), qy = __shfl(py, src, 64), qz = __shfl(pz, src, 64);
+      const float qu = __shfl(du, src, 64), qv = __shfl(dv, src, 64), qw = __shfl(dw, src, 64);
+      const float qe = __shfl(e, src, 64), qwt = __shfl(ww, src, 64);
+      const int qk0 = __shfl(k0, src, 64);
       bool hit = false;
       int32_t pixel = 0;
       float c = 0.f, dy = 0.f, hu = 0.f, hz = 0.f;
       int k = 0;
-      if (cand) {
+      if (act) {
+        const float px = qx, py = qy, pz = qz, du = qu, dv = qv, dw = qw;
+        const int k0 = qk0;
         k = ((k0 + j) % D.n_proj + D.n_proj) % D.n_proj;
         const float ca = cs[2 * k], sa = cs[2 * k + 1];
         c = fmaf(du, ca, dv * sa);
@@ -131,10 +153,10 @@ __global__ void __launch_bounds__(256) spect_pairs_kernel(const float *__restric
       const uint64_t slot = slot0 + mbcnt(m);
       if ((int64_t)slot >= o.capacity) continue;
       const float theta = acosf(fminf(c, 1.0f));
-      const float phi = atan2f(dy, dw);
+      const float phi = atan2f(dy, qw);
       if (o.x16) {
         __bf16 *row = o.x16 + slot * 8;
-        row[0] = (__bf16)fmaf(e, D.norm[0], D.norm[1]);
+        row[0] = (__bf16)fmaf(qe, D.norm[0], D.norm[1]);
         row[1] = (__bf16)fmaf(theta, D.norm[2], D.norm[3]);
         row[2] = (__bf16)fmaf(phi, D.norm[4], D.norm[5]);
 #pragma unroll
@@ -142,7 +164,7 @@ __global__ void __launch_bounds__(256) spect_pairs_kernel(const float *__restric
       }
       if (o.tp) { o.tp[2 * slot] = theta; o.tp[2 * slot + 1] = phi; }
       o.pix[slot] = pixel;
-      o.wgt[slot] = ww;
+      o.wgt[slot] = qwt;
       if (o.det[0]) {
         // head frame: x' axial (hz), y' tangential (hu); reduce to the lattice cell
         float rx = hz, ry = hu;
@@ -156,11 +178,11 @@ __global__ void __launch_bounds__(256) spect_pairs_kernel(const float *__restric
         o.det[0][slot] = D.patch_origin[0] + rx;
         o.det[1][slot] = D.patch_origin[1] + ry;
         o.det[2][slot] = D.patch_origin[2];
-        o.det[3][slot] = dw;
+        o.det[3][slot] = qw;
         o.det[4][slot] = dy;
         o.det[5][slot] = -c;
-        o.det[6][slot] = e;
-        o.det[7][slot] = ww;
+        o.det[6][slot] = qe;
+        o.det[7][slot] = qwt;
       }
     }
   }
