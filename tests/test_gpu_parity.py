@@ -166,6 +166,53 @@ def test_tle_estimator_bit_exact(engine):
         compare(c, rc, g, r)
 
 
+@pytest.mark.parametrize("n_mat", [16, 17])
+def test_label_packing_odd_sizes(engine, n_mat):
+    """16 materials: labels packed two per byte on the GPU; 17: one byte each. Odd phantom
+    dimensions (odd voxel count, so the last byte is half used); every label value used."""
+    from opengate_amd.physics.materials import MaterialDatabase
+    from opengate_amd.physics import xs
+
+    from opengate_amd.units import g4_units
+
+    db = MaterialDatabase()
+    for k in range(3):
+        db.add_material_weights(f"WATER_{k}", ["H", "O"], [0.111894, 0.888106],
+                                (1.05 + 0.05 * k) * g4_units.g / g4_units.cm3)
+    names = ["G4_WATER", "G4_BONE_CORTICAL_ICRP", "G4_LUNG_ICRP", "G4_AIR", "G4_TISSUE_SOFT_ICRP", "G4_Al",
+             "G4_MUSCLE_SKELETAL_ICRP", "G4_ADIPOSE_TISSUE_ICRP", "G4_BONE_COMPACT_ICRU", "G4_PLEXIGLASS", "G4_Ti",
+             "G4_SODIUM_IODIDE", "G4_Cu", "G4_Pb", "WATER_0", "WATER_1", "WATER_2"][:n_mat]
+    T = xs.build_tables([db.get(n) for n in names], 0.1405)
+    g = np.random.default_rng(n_mat)
+    lab = g.integers(0, n_mat, size=(11, 13, 15)).astype(np.uint8)
+    lab[-1, -1, -1] = n_mat - 1
+    ph = PhantomSpec(lab, (3.0, 2.5, 2.0))
+    src = SourceSpec(pos_type="box", pos_param=(20.0, 20.0, 15.0), pos_trans=centre(ph), energy=0.1)
+    tal = aligned_tally(ph)
+    c, g_, rc, r = run_both(engine, T, ph, src, tal, 60_000, 3)
+    compare(c, rc, g_, r)
+
+
+def test_degenerate_runs(engine):
+    """Zero histories, a one-voxel phantom, and a source that never reaches the phantom."""
+    T = tables()
+    ph = small_phantom(1, 5.0)
+    src = SourceSpec(pos_trans=centre(ph), energy=0.1405)
+    tal = aligned_tally(ph)
+    engine.clear()
+    engine.set_physics(T)
+    engine.add_simulation(ph, src, tal)
+    c0 = engine.run([0], 1)
+    assert c0["histories"] == 0 and c0["steps"] == 0
+    assert np.all(engine.get_tally(0)["edep"] == 0)
+    c, g, rc, r = run_both(engine, T, ph, src, tal, 10_000, 2)
+    compare(c, rc, g, r)
+    away = SourceSpec(pos_trans=(100.0, 100.0, 100.0), momentum=(1.0, 0.0, 0.0), dir_type="momentum", energy=0.1)
+    c, g, rc, r = run_both(engine, T, ph, away, tal, 10_000, 2)
+    compare(c, rc, g, r)
+    assert c["escaped"] == 10_000 and c["steps"] == 0
+
+
 def test_batched_launch_equals_separate_runs(engine):
     T = tables(e_max=0.21)
     specs = []
