@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GH_ABI_VERSION 2
+#define GH_ABI_VERSION 3
 
 enum {
   GH_OK = 0,
@@ -49,7 +49,8 @@ enum {
 
 /* position types of gh_source_desc.pos_type (GateGenericSource.cpp:312-384) */
 enum { GH_POS_POINT = 0, GH_POS_BOX = 1, GH_POS_SPHERE = 2, GH_POS_ELLIPSOID = 3,
-       GH_POS_DISC = 4, GH_POS_CYLINDER = 5, GH_POS_PHSP = 6 };
+       GH_POS_DISC = 4, GH_POS_CYLINDER = 5, GH_POS_PHSP = 6,
+       GH_POS_VOXELS = 7 /* activity image (VoxelSource), Walker alias table over its voxels */ };
 /* direction types (GateGenericSource.cpp:386-480) */
 enum { GH_DIR_ISO = 0, GH_DIR_MOMENTUM = 1, GH_DIR_FOCUSED = 2 };
 /* energy types (GateGenericSource.cpp:492-664, GateSPSEneDistribution.cpp:16-159) */
@@ -116,6 +117,13 @@ typedef struct gh_source_desc {
   int32_t phsp_on_device;    /* 1: phsp[] are device pointers, used in place: they must stay
                                 allocated until the simulation is cleared or its source replaced */
   const float *phsp[8];      /* x, y, z, dx, dy, dz, E, weight (source frame) */
+  int64_t vox_n;             /* GH_POS_VOXELS: alias-table entries (voxels with activity) */
+  const float *vox_prob;     /* acceptance probability of entry i */
+  const int32_t *vox_alias;  /* alias entry of entry i */
+  const int32_t *vox_index;  /* voxel x + nx (y + ny z) of entry i */
+  int32_t vox_size[3];       /* activity image size (x, y, z) */
+  int32_t reserved2;
+  float vox_spacing[3];      /* voxel size [mm]; source frame: image corner at the origin */
 } gh_source_desc;
 
 typedef struct gh_tally_desc {
@@ -276,6 +284,12 @@ int gh_arf_accumulate(const float *logits, int64_t ldl, int64_t m, const int32_t
                       int32_t n_win, int32_t plane, double *img, double *img2, void *stream);
 int gh_counts_accumulate(const int32_t *cls, const int32_t *pix, const float *wgt, int64_t m, int32_t n_win,
                          int32_t plane, double *img, double *img2, void *stream);
+
+/* Walker alias table of n non-negative weights (host arrays; no device needed): the voxel
+ * source samples entry i uniformly, keeps it with probability prob[i], else takes alias[i]
+ * (GateSPSVoxelsPosDistribution::VGenerateOne, GateSPSVoxelsPosDistribution.cpp:24-52,
+ * samples the same distribution with nested CDF searches). */
+int gh_alias_table(const double *w, int64_t n, float *prob, int32_t *alias);
 
 /* Device properties used by the autotuner and reports. */
 int gh_device_info(gh_ctx *ctx, int32_t *n_cu, int32_t *lds_bytes, char *name, int32_t name_len);

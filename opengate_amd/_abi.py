@@ -12,7 +12,7 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libgatehip.so"
 
 GH_OK, GH_EINVAL, GH_EHIP, GH_ESTATE, GH_ERANGE = 0, 1, 2, 3, 4
-POS = {"point": 0, "box": 1, "sphere": 2, "ellipsoid": 3, "disc": 4, "cylinder": 5, "phsp": 6}
+POS = {"point": 0, "box": 1, "sphere": 2, "ellipsoid": 3, "disc": 4, "cylinder": 5, "phsp": 6, "voxels": 7}
 DIR = {"iso": 0, "momentum": 1, "focused": 2}
 ENE = {"mono": 0, "spectrum_discrete": 1, "spectrum_histogram": 2, "range": 3}
 TALLY_EDEP2, TALLY_DOSE, TALLY_DOSE2, TALLY_COUNTS, TALLY_ALIGNED, TALLY_TLE = 1, 2, 4, 8, 16, 32
@@ -48,6 +48,9 @@ class SourceDesc(C.Structure):
         ("energy_value", FP), ("energy_width", FP), ("energy_cdf", FP),
         ("phsp_n", C.c_int64), ("phsp_start", C.c_int64), ("phsp_iso", C.c_int32), ("phsp_on_device", C.c_int32),
         ("phsp", FP * 8),
+        ("vox_n", C.c_int64), ("vox_prob", FP), ("vox_alias", C.POINTER(C.c_int32)),
+        ("vox_index", C.POINTER(C.c_int32)), ("vox_size", C.c_int32 * 3), ("reserved2", C.c_int32),
+        ("vox_spacing", C.c_float * 3),
     ]
 
 
@@ -135,6 +138,7 @@ SIGNATURES = {
                                    C.c_void_p]),
     "gh_exitplane_decode": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int32, C.c_float, FP,
                                       C.c_float, FP, C.c_float, FP * 8, C.c_void_p]),
+    "gh_alias_table": (C.c_int, [DP, C.c_int64, FP, C.POINTER(C.c_int32)]),
     "gh_mlp_forward_bf16": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_void_p), C.POINTER(FP), FP, C.c_int64, C.c_void_p]),
     "gh_spect_pairs": (C.c_int, [FP * 8, C.c_int64, C.POINTER(SpectDesc), C.c_void_p, C.c_int64, C.c_void_p,
@@ -173,7 +177,7 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.gh_abi_version() != 2:
+    if lib.gh_abi_version() != 3:
         raise RuntimeError("libgatehip.so ABI version mismatch")
     if path is None:
         _LIB = lib

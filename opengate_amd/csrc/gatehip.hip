@@ -52,6 +52,11 @@ struct SimDev {
   const float *e_val, *e_width, *e_cdf;
   int64_t phsp_n, phsp_start;
   const float *phsp[8];
+  int64_t vox_n;
+  const float *vox_prob;
+  const int32_t *vox_alias, *vox_index;
+  int vox_nx, vox_ny;
+  float vox_s[3];
   // tally
   double *edep, *edep2, *dose, *dose2, *counts;
   int aligned, tnx, tny, tnz;
@@ -295,6 +300,19 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
         break;
       }
       case GH_POS_DISC:
+      case GH_POS_VOXELS: {
+        // Walker alias table: entry i uniform (32-bit multiply-shift), kept with
+        // probability prob[i], else its alias; then uniform inside the voxel
+        const int64_t i = (int64_t)(((uint64_t)rng_next(g) * (uint64_t)s.vox_n) >> 32);
+        const int64_t j = (u01(g) < s.vox_prob[i]) ? i : (int64_t)s.vox_alias[i];
+        const int v = s.vox_index[j];
+        const int ix = v % s.vox_nx, iyz = v / s.vox_nx;
+        const int iy = iyz % s.vox_ny, iz = iyz / s.vox_ny;
+        x = ((float)ix + u01(g)) * s.vox_s[0];
+        y = ((float)iy + u01(g)) * s.vox_s[1];
+        z = ((float)iz + u01(g)) * s.vox_s[2];
+        break;
+      }
       case GH_POS_CYLINDER: {
         float a, b, r2;
         int it = 0;
@@ -854,7 +872,21 @@ void free_all(std::vector<void *> &v) {
 namespace {
 
 int validate_source(gh_ctx *ctx, const gh_source_desc *src) {
-  if (src->pos_type < GH_POS_POINT || src->pos_type > GH_POS_PHSP) FAIL(GH_EINVAL, "invalid pos_type");
+  if (src->pos_type < GH_POS_POINT || src->pos_type > GH_POS_VOXELS) FAIL(GH_EINVAL, "invalid pos_type");
+  if (src->pos_type == GH_POS_VOXELS) {
+    if (src->vox_n <= 0 || src->vox_n > 0x7FFFFFFF || !src->vox_prob || !src->vox_alias || !src->vox_index)
+      FAIL(GH_EINVAL, "voxel source needs a non-empty alias table");
+    const int64_t nvox = (int64_t)src->vox_size[0] * src->vox_size[1] * src->vox_size[2];
+    if (src->vox_size[0] <= 0 || src->vox_size[1] <= 0 || src->vox_size[2] <= 0 || nvox > 0x7FFFFFFF)
+      FAIL(GH_EINVAL, "voxel source: invalid image size");
+    for (int k = 0; k < 3; ++k)
+      if (!(src->vox_spacing[k] > 0.0f)) FAIL(GH_EINVAL, "voxel source: invalid spacing");
+    for (int64_t i = 0; i < src->vox_n; ++i) {
+      if (src->vox_index[i] < 0 || src->vox_index[i] >= nvox) FAIL(GH_EINVAL, "voxel source: voxel index out of range");
+      if (src->vox_alias[i] < 0 || src->vox_alias[i] >= src->vox_n) FAIL(GH_EINVAL, "voxel source: alias out of range");
+      if (!(src->vox_prob[i] >= 0.0f && src->vox_prob[i] <= 1.0f)) FAIL(GH_EINVAL, "voxel source: probability outside [0, 1]");
+    }
+  }
   if (src->dir_type < GH_DIR_ISO || src->dir_type > GH_DIR_FOCUSED) FAIL(GH_EINVAL, "invalid dir_type");
   if (src->ene_type < GH_ENE_MONO || src->ene_type > GH_ENE_RANGE) FAIL(GH_EINVAL, "invalid ene_type");
   if ((src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) &&
@@ -946,6 +978,19 @@ int upload_source(gh_ctx *ctx, const gh_source_desc *src, SimDev &d, std::vector
       if ((rc = dev_upload(ctx, src->energy_width, src->n_energy, &t, owned))) return rc;
       d.e_width = t;
     }
+  }
+  d.vox_n = 0; d.vox_prob = nullptr; d.vox_alias = nullptr; d.vox_index = nullptr;
+  if (src->pos_type == GH_POS_VOXELS) {
+    d.vox_n = src->vox_n;
+    d.vox_nx = src->vox_size[0]; d.vox_ny = src->vox_size[1];
+    for (int k = 0; k < 3; ++k) d.vox_s[k] = src->vox_spacing[k];
+    if ((rc = dev_upload(ctx, src->vox_prob, (size_t)src->vox_n, &t, owned))) return rc;
+    d.vox_prob = t;
+    int32_t *ti;
+    if ((rc = dev_upload(ctx, src->vox_alias, (size_t)src->vox_n, &ti, owned))) return rc;
+    d.vox_alias = ti;
+    if ((rc = dev_upload(ctx, src->vox_index, (size_t)src->vox_n, &ti, owned))) return rc;
+    d.vox_index = ti;
   }
   if (src->pos_type == GH_POS_PHSP) {
     d.phsp_n = src->phsp_n; d.phsp_start = src->phsp_start;
@@ -1490,6 +1535,39 @@ int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n) {
   if (n > h.event_cap) FAIL(GH_ERANGE, "gh_get_event_edep: n beyond the last run");
   HIPCHK(hipSetDevice(ctx->device));
   if (n) HIPCHK(hipMemcpy(out, h.dev.event_edep, n * sizeof(float), hipMemcpyDefault));  // host or device
+  return GH_OK;
+}
+
+// Walker alias table (Vose's construction, FP64): entry i is kept with probability
+// prob[i], otherwise replaced by alias[i]; P(i) = (prob[i] + sum_{j: alias[j] = i}
+// (1 - prob[j])) / n = w[i] / sum(w). Host-only helper (no device needed).
+int gh_alias_table(const double *w, int64_t n, float *prob, int32_t *alias) {
+  if (!w || !prob || !alias || n <= 0 || n > 0x7FFFFFFF) return GH_EINVAL;
+  double total = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (!(w[i] >= 0.0) || !std::isfinite(w[i])) return GH_EINVAL;
+    total += w[i];
+  }
+  if (!(total > 0.0)) return GH_EINVAL;
+  std::vector<double> p((size_t)n);
+  std::vector<int32_t> small, large;
+  small.reserve((size_t)n);
+  large.reserve((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    p[i] = w[i] * (double)n / total;
+    (p[i] < 1.0 ? small : large).push_back((int32_t)i);
+  }
+  while (!small.empty() && !large.empty()) {
+    const int32_t a = small.back(), b = large.back();
+    small.pop_back();
+    large.pop_back();
+    prob[a] = (float)p[a];
+    alias[a] = b;
+    p[b] = (p[b] + p[a]) - 1.0;
+    (p[b] < 1.0 ? small : large).push_back(b);
+  }
+  for (int32_t i : large) { prob[i] = 1.0f; alias[i] = i; }
+  for (int32_t i : small) { prob[i] = 1.0f; alias[i] = i; }  // rounding leftovers
   return GH_OK;
 }
 

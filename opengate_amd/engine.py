@@ -18,7 +18,7 @@ from .actors.dose import FACES, GY_FACTOR, DoseActor, PhaseSpaceActor, Simulatio
 from .geometry.volumes import BoxVolume, ImageVolume
 from .physics import xs
 from .sources.gan import GANSource
-from .sources.generic import GenericSource, PhaseSpaceSource
+from .sources.generic import GenericSource, PhaseSpaceSource, VoxelSource
 from .transport import GpuEngine, PhantomSpec, SourceSpec, TallySpec, spectrum_histogram, spectrum_lines
 from .units import g4_units
 
@@ -129,7 +129,17 @@ class SimulationEngine:
         Rg = Ra @ Rl
         Tg = Ra @ np.asarray(pos.translation, dtype=np.float64) + ta
         kind = pos.type
-        if kind == "point":
+        if isinstance(src, VoxelSource):
+            # voxel corner frame: p_attached = p + (-size/2 spacing + translation), then the
+            # attached volume's global transform (GateSPSVoxelsPosDistribution.cpp:44-51)
+            act, sp = src.activity_image()
+            size = np.array(act.shape[::-1], dtype=np.float64)
+            corner = -size / 2 * sp + np.asarray(pos.translation, dtype=np.float64)
+            Rg = Ra
+            Tg = Ra @ corner + ta
+            kind = "voxels"
+            pp = (0.0, 0.0, 0.0, 0.0)
+        elif kind == "point":
             pp = (0.0, 0.0, 0.0, 0.0)
         elif kind == "box":
             pp = tuple(pos.size) + (0.0,)
@@ -151,6 +161,9 @@ class SimulationEngine:
         spec = SourceSpec(pos_type=kind, pos_param=pp, pos_rot=self.R.T @ Rg,
                           pos_trans=tuple(self.R.T @ (Tg - self.t) + self.L / 2), dir_rot=dir_rot,
                           weight=1.0 if src.weight is None or src.weight < 0 else float(src.weight))
+        if kind == "voxels":
+            spec.activity = act
+            spec.activity_spacing = tuple(sp)
         if d.type == "iso":
             spec.dir_type = "iso"
             spec.theta = (float(d.theta[0]), float(d.theta[1]))

@@ -2,7 +2,8 @@
 add_source :2607, add_actor :2610, run :2818; PhysicsManager :695; VolumeManager :1435).
 
 Only the photon-transport-and-dose path is available: the world, Box and Image
-volumes, GenericSource / PhaseSpaceSource / GANSource with gamma primaries, the gamma physics
+volumes, GenericSource / VoxelSource / PhaseSpaceSource / GANSource with gamma
+primaries, the gamma physics
 options, DoseActor, SimulationStatisticsActor and an exit-face PhaseSpaceActor.
 Anything else raises NotImplementedError instead of silently doing something else.
 """
@@ -16,7 +17,7 @@ from .base import Box, GateObject
 from .geometry.volumes import BoxVolume, ImageVolume
 from .physics.materials import MaterialDatabase
 from .sources.gan import GANSource
-from .sources.generic import GenericSource, PhaseSpaceSource
+from .sources.generic import GenericSource, PhaseSpaceSource, VoxelSource
 from .units import g4_units
 
 VACUUM_MATERIALS = ("G4_Galactic", "Vacuum", "G4_VACUUM")
@@ -153,6 +154,8 @@ class Simulation:
             s = PhaseSpaceSource(name)
         elif source_type == "GANSource":
             s = GANSource(name)
+        elif source_type == "VoxelSource":
+            s = VoxelSource(name)
         else:
             raise NotImplementedError(f"source type '{source_type}' is not part of the photon dose path")
         s.simulation = self

@@ -5,7 +5,9 @@ parameter groups), :333-420 (source attributes); number of primaries `n` or
 `activity` (sources/base.py:23-80). PhaseSpaceSource: opengate/sources/phspsources.py
 :253-482 (keys, entry_start, global_flag, isotropic_direction, translate / rotate).
 The ROOT reader of the reference (uproot) is replaced by numpy .npz / .npy-dict files
-that hold the same branch names.
+that hold the same branch names. VoxelSource: opengate/sources/voxelsources.py:13-136
+(activity image, origin -size/2 spacing + translation + spacing/2 in the attached
+volume, then the volume's global transform; GateSPSVoxelsPosDistribution.cpp:24-52).
 """
 import numpy as np
 
@@ -113,3 +115,22 @@ class PhaseSpaceSource(SourceBase):
         w = data.get(self.weight_key)
         cols.append(w if w is not None else np.ones(len(cols[0])))
         return [np.ascontiguousarray(c, dtype=np.float32) for c in cols]
+
+
+class VoxelSource(GenericSource):
+    """GenericSource whose positions follow a 3-D activity image (normalised to sum 1).
+    Positions are uniform inside the sampled voxel; `position.rotation` is not applied,
+    as in the reference (only the attached volume's rotation is)."""
+
+    user_info_defaults = {
+        "image": (None, {"doc": "activity image: file (.mhd/.mha/.npy) or opengate_amd.Image"}),
+    }
+    source_type = "VoxelSource"
+
+    def activity_image(self):
+        from ..image import Image, read_image
+
+        if self.image is None:
+            raise ValueError(f"VoxelSource '{self.name}': image is not set")
+        img = self.image if isinstance(self.image, Image) else read_image(self.image)
+        return np.asarray(img.array, dtype=np.float64), np.asarray(img.spacing, dtype=np.float64)

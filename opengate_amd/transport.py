@@ -69,6 +69,24 @@ def spectrum_histogram(edges, weights):
     return e[:-1].astype(np.float32), np.diff(e).astype(np.float32), cdf
 
 
+def voxel_alias_table(activity):
+    """(flat voxel indices with activity > 0, prob, alias) for an activity image (z, y, x):
+    the Walker alias table of the library (gh_alias_table, a host-only routine)."""
+    a = np.ascontiguousarray(activity, dtype=np.float64).ravel()
+    if np.any(~np.isfinite(a)) or np.any(a < 0):
+        raise ValueError("activity must be finite and >= 0")
+    idx = np.flatnonzero(a > 0).astype(np.int32)
+    if len(idx) == 0:
+        raise ValueError("activity image is empty")
+    w = np.ascontiguousarray(a[idx])
+    prob = np.zeros(len(idx), np.float32)
+    alias = np.zeros(len(idx), np.int32)
+    rc = A.load_library().gh_alias_table(A.dptr(w), len(idx), A.fptr(prob), alias.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc != 0:
+        raise ValueError(f"gh_alias_table failed ({rc})")
+    return idx, prob, alias
+
+
 @dataclass
 class SourceSpec:
     pos_type: str = "point"
@@ -91,6 +109,8 @@ class SourceSpec:
     phsp: Optional[list] = None  # 8 float32 arrays x y z dx dy dz E w (source frame), numpy or device tensors
     phsp_start: int = 0
     phsp_iso: bool = False
+    activity: Optional[np.ndarray] = None  # voxels: activity image (z, y, x), image corner at the source origin
+    activity_spacing: tuple = (1.0, 1.0, 1.0)  # voxels: voxel size (x, y, z) mm
 
     def _phsp_on_device(self):
         return self.phsp is not None and hasattr(self.phsp[0], "data_ptr")
@@ -138,6 +158,19 @@ class SourceSpec:
             if self.ene_type == "spectrum_histogram":
                 self._ew = np.ascontiguousarray(self.energy_width, dtype=np.float32)
                 d.energy_width = A.fptr(self._ew)
+        if self.pos_type == "voxels":
+            if self.activity is None or self.activity.ndim != 3:
+                raise ValueError("a voxel source needs a 3-D activity image (z, y, x)")
+            if not hasattr(self, "_vox") or self._vox[0] is not self.activity:
+                self._vox = (self.activity,) + voxel_alias_table(self.activity)
+            _, idx, prob, alias = self._vox
+            nz, ny, nx = self.activity.shape
+            d.vox_n = len(idx)
+            d.vox_prob = A.fptr(prob)
+            d.vox_alias = alias.ctypes.data_as(C.POINTER(C.c_int32))
+            d.vox_index = idx.ctypes.data_as(C.POINTER(C.c_int32))
+            d.vox_size = (C.c_int32 * 3)(nx, ny, nz)
+            d.vox_spacing = _f3(self.activity_spacing)
         if self.pos_type == "phsp":
             if self._phsp_on_device():
                 self._ph = [a.contiguous().float() for a in self.phsp]

@@ -473,6 +473,18 @@ static void emit(const sim_t *S, stream_t *rs, uint64_t h, photon_t *ph) {
       lp[0] = src->pos_param[0] * a;
       lp[1] = src->pos_param[ell ? 1 : 0] * b;
       lp[2] = src->pos_param[ell ? 2 : 0] * c;
+    } else if (src->pos_type == GH_POS_VOXELS) {
+      /* Walker alias table (GateSPSVoxelsPosDistribution::VGenerateOne samples the same
+         distribution with three nested CDF searches) */
+      const int64_t i = (int64_t)(((uint64_t)stream_word(rs) * (uint64_t)src->vox_n) >> 32);
+      const int64_t j = (uni(rs) < src->vox_prob[i]) ? i : (int64_t)src->vox_alias[i];
+      const int v = src->vox_index[j];
+      const int nx = src->vox_size[0], ny = src->vox_size[1];
+      const int ix = v % nx, iyz = v / nx;
+      const int iy = iyz % ny, iz = iyz / ny;
+      lp[0] = ((float)ix + uni(rs)) * src->vox_spacing[0];
+      lp[1] = ((float)iy + uni(rs)) * src->vox_spacing[1];
+      lp[2] = ((float)iz + uni(rs)) * src->vox_spacing[2];
     } else if (src->pos_type == GH_POS_DISC || src->pos_type == GH_POS_CYLINDER) {
       float a, b, r2;
       int tries = 0;

@@ -213,6 +213,20 @@ def test_degenerate_runs(engine):
     assert c["escaped"] == 10_000 and c["steps"] == 0
 
 
+def test_voxel_source(engine):
+    """Activity-image source (alias table on the GPU): counters and tallies of the oracle."""
+    T = tables()
+    ph = small_phantom(24, 2.0)
+    g = np.random.default_rng(5)
+    act = g.gamma(0.8, size=(6, 7, 8))
+    act[act < 0.4] = 0.0
+    src = SourceSpec(pos_type="voxels", activity=act, activity_spacing=(3.0, 2.5, 2.0), pos_trans=(12.0, 14.0, 16.0),
+                     energy=0.1405)
+    tal = aligned_tally(ph, counts=True)
+    c, g_, rc, r = run_both(engine, T, ph, src, tal, 100_000, 13)
+    compare(c, rc, g_, r)
+
+
 def test_batched_launch_equals_separate_runs(engine):
     T = tables(e_max=0.21)
     specs = []
