@@ -489,6 +489,47 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
   const uint32_t slow_min = (uint32_t)L.slow_min;
   const uint32_t refill_min = (uint32_t)L.refill_min;
 
+  // world transport outside the phantom box (homogeneous material or vacuum); run at
+  // refill for photons born outside the box (all new lanes together) and in the slow
+  // phase for photons that left it. The event stream of the history supplies the
+  // interaction distance either way, so where it runs does not change the result.
+  auto world_step = [&](const SimDev &s, bool &kill) {
+        // world outside the phantom box (homogeneous material or vacuum)
+        float tn, tf;
+        int face;
+        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
+        const bool hit = (tn < tf) && (tf > EXIT_EPS);
+        const float tin = fmaxf(tn, 0.0f);
+        if (hot.world_mat < 0) {
+          if (hit) {
+            ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
+            mode = M_STEP;
+          } else {
+            st.inc(C_ESCAPED);
+            st.add_e(E_ESC, (double)(ph.E * ph.wt));
+            kill = true;
+          }
+        } else {
+          const int wm = hot.world_mat;
+          const float muw = lerp_node(mu_tab + wm * NB1, ec);
+          const float tw = world_exit(s, ph.x, ph.y, ph.z, ph.u, ph.v, ph.w);
+          const float d = (muw > 0.0f) ? (-glog(u01(rng)) / muw) : INFINITY;
+          if (hit && d >= tin && tin < tw) {
+            ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
+            mode = M_STEP;
+          } else if (d >= tw) {
+            st.inc(C_ESCAPED);
+            st.add_e(E_ESC, (double)(ph.E * ph.wt));
+            kill = true;
+          } else {
+            ph.x = fmaf(d, ph.u, ph.x); ph.y = fmaf(d, ph.v, ph.y); ph.z = fmaf(d, ph.w, ph.z);
+            st.inc(C_WORLD);
+            mode = M_COLLIDE;
+            clab = wm;
+            cvox = -1;
+          }
+        }
+      };
   for (;;) {
     // ------------------------------------------------------------- refill
     {
@@ -546,6 +587,14 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
               } else {
                 ecache_set(ec, ph.E, maj_tab);
                 mode = inside ? M_STEP : M_WORLD;
+                if (mode == M_WORLD) {
+                  bool kill = false;
+                  world_step(s, kill);
+                  if (kill) {
+                    if (s.event_edep) s.event_edep[ph.hist] = 0.0f;
+                    mode = M_DEAD;
+                  }
+                }
               }
             }
           }
@@ -637,43 +686,7 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
           mode = M_WORLD;
         }
       }
-      if (mode == M_WORLD && !kill) {
-        // world outside the phantom box (homogeneous material or vacuum)
-        float tn, tf;
-        int face;
-        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
-        const bool hit = (tn < tf) && (tf > EXIT_EPS);
-        const float tin = fmaxf(tn, 0.0f);
-        if (hot.world_mat < 0) {
-          if (hit) {
-            ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
-            mode = M_STEP;
-          } else {
-            st.inc(C_ESCAPED);
-            st.add_e(E_ESC, (double)(ph.E * ph.wt));
-            kill = true;
-          }
-        } else {
-          const int wm = hot.world_mat;
-          const float muw = lerp_node(mu_tab + wm * NB1, ec);
-          const float tw = world_exit(s, ph.x, ph.y, ph.z, ph.u, ph.v, ph.w);
-          const float d = (muw > 0.0f) ? (-glog(u01(rng)) / muw) : INFINITY;
-          if (hit && d >= tin && tin < tw) {
-            ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
-            mode = M_STEP;
-          } else if (d >= tw) {
-            st.inc(C_ESCAPED);
-            st.add_e(E_ESC, (double)(ph.E * ph.wt));
-            kill = true;
-          } else {
-            ph.x = fmaf(d, ph.u, ph.x); ph.y = fmaf(d, ph.v, ph.y); ph.z = fmaf(d, ph.w, ph.z);
-            st.inc(C_WORLD);
-            mode = M_COLLIDE;
-            clab = wm;
-            cvox = -1;
-          }
-        }
-      }
+      if (mode == M_WORLD && !kill) world_step(s, kill);
       if (mode == M_COLLIDE && !kill) {
         const bool in_world = cvox < 0;
         const int lab = clab;
