@@ -32,6 +32,7 @@
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 64, BN = 64, BK = 32;
@@ -40,9 +41,12 @@ constexpr int LDK = BK + 8;  // LDS row pitch in bf16 (16-byte aligned rows, few
 __device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
 __device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware exp2 and reciprocal (1 ulp): the network
+// tolerates it, and the GEMM and fused paths share these definitions
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -256,18 +260,24 @@ extern "C" int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void 
 // LDS. Layer l: Z = X W_l + b_l; hidden layers X_{l+1} = bf16(silu(Z)), the last layer is
 // written as FP32. Weights are given transposed, Wt_l[n][k] (k contiguous), so that each
 // lane's MFMA B fragment (column n, 8 consecutive k) is one 16-byte load from L2-resident
-// memory. A workgroup of 4 waves owns FBM = 64 rows; wave w computes the hidden columns
-// [w H/4, (w+1) H/4) for all 64 rows (4 row tiles x H/64 column tiles of 16x16), reading
-// its A fragments (row, 8 consecutive k) from the LDS activation tile. Per 32-wide k step
-// the B fragments of the next step are loaded before the MFMAs of the current one.
+// memory. A workgroup of 4 waves owns 16 RT rows; in a hidden layer wave w computes the
+// columns [w H/4, (w+1) H/4) for all rows (RT row tiles x H/64 column tiles of 16x16),
+// reading its A fragments (row, 8 consecutive k) from the LDS activation tile; the B
+// fragments are loaded PF 32-wide k steps ahead of the MFMAs that use them. The output
+// layer (16 columns) spreads its row tiles over the 4 waves.
 // The per-element arithmetic (MFMA over k slabs of 32 in order, bias, SiLU, bf16 rounding)
 // is the same as the layer-by-layer GEMM path, so both give bit-identical outputs.
 namespace {
 
-constexpr int FBM = 64;          // rows per workgroup
 constexpr int MAXL = 8;          // layers
 constexpr int K0 = 32;           // first-layer input width (padded)
 constexpr int NOUT = 16;         // last-layer output width (padded)
+#ifndef GH_MLP_RT
+#define GH_MLP_RT 4              // 16-row tiles per workgroup (64 rows)
+#endif
+#ifndef GH_MLP_PF
+#define GH_MLP_PF 1              // B-fragment prefetch distance in k steps
+#endif
 
 struct MlpArgs {
   const __bf16 *wt[MAXL];        // [N_l][K_l]
@@ -277,99 +287,133 @@ struct MlpArgs {
   int n_out;                     // valid output columns (<= NOUT)
 };
 
-template <int H>
+template <int H, int RT>
+constexpr size_t mlp_lds_bytes() { return 2ull * (16 * RT) * (H + 8) * sizeof(__bf16); }
+
+template <int H, int RT, int PF>
 __global__ void __launch_bounds__(256) mlp_forward_kernel(const __bf16 *__restrict__ X, int64_t ldx, int64_t M,
                                                           MlpArgs a, float *__restrict__ Z, int64_t ldz) {
   constexpr int P = H + 8;                 // LDS row pitch (bf16)
   constexpr int CT = H / 64;               // 16-wide column tiles per wave (hidden layers)
-  __shared__ __bf16 act[2][FBM * P];
+  constexpr int ROWS = 16 * RT;
+  extern __shared__ __align__(16) unsigned char mlp_lds[];
+  __bf16 *act0 = (__bf16 *)mlp_lds;
+  __bf16 *act1 = act0 + ROWS * P;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t row0 = (int64_t)blockIdx.x * FBM;
+  const int64_t row0 = (int64_t)blockIdx.x * ROWS;
   const int fr = lane & 15, fk = (lane >> 4) * 8;
 
-  // input tile -> act[0][:, 0:K0] (zero beyond k_in and beyond M)
-  for (int i = tid; i < FBM * K0; i += 256) {
+  // input tile -> act0[:, 0:K0] (zero beyond k_in and beyond M)
+  for (int i = tid; i < ROWS * K0; i += 256) {
     const int r = i / K0, k = i - r * K0;
     const int64_t gr = row0 + r;
-    act[0][r * P + k] = (gr < M && k < a.k_in) ? X[gr * ldx + k] : (__bf16)0.0f;
+    act0[r * P + k] = (gr < M && k < a.k_in) ? X[gr * ldx + k] : (__bf16)0.0f;
   }
   __syncthreads();
 
-  int cur = 0;
+  __bf16 *in = act0, *out = act1;
   for (int l = 0; l < a.n_layers; ++l) {
     const bool last = l == a.n_layers - 1;
     const int K = l == 0 ? K0 : H;
     const __bf16 *__restrict__ W = a.wt[l];
     const float *__restrict__ bias = a.bias[l];
-    const __bf16 *in = act[cur];
     if (!last) {
-      __bf16 *out = act[cur ^ 1];
-      floatx4 acc[4][CT];
+      floatx4 acc[RT][CT];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < RT; ++i)
 #pragma unroll
         for (int j = 0; j < CT; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
       const int n0 = wave * (H / 4);
-      bf16x8 bq[CT], bn[CT];
+      const int nk = K / 32;
+      // B fragments of k steps ks (b0), ks+1 (b1) and, for PF = 2, ks+2 (b2), rotated by
+      // register moves so that every array index is static
+      const __bf16 *wrow[CT];
 #pragma unroll
-      for (int j = 0; j < CT; ++j) bq[j] = *(const bf16x8 *)(W + (int64_t)(n0 + j * 16 + fr) * K + fk);
-      for (int k0 = 0; k0 < K; k0 += 32) {
-        if (k0 + 32 < K) {
-#pragma unroll
-          for (int j = 0; j < CT; ++j) bn[j] = *(const bf16x8 *)(W + (int64_t)(n0 + j * 16 + fr) * K + k0 + 32 + fk);
-        }
-        bf16x8 af[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8 *)(in + (i * 16 + fr) * P + k0 + fk);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < CT; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bq[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int j = 0; j < CT; ++j) bq[j] = bn[j];
-      }
-      // epilogue: bias + SiLU -> bf16 activation tile
+      for (int j = 0; j < CT; ++j) wrow[j] = W + (int64_t)(n0 + j * 16 + fr) * K + fk;
+      bf16x8 b0[CT], b1[CT], b2[CT];
 #pragma unroll
       for (int j = 0; j < CT; ++j) {
-        const int c = n0 + j * 16 + fr;
-        const float b = bias[c];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = i * 16 + (lane >> 4) * 4 + r;
-            out[row * P + c] = f2bf(silu(acc[i][j][r] + b));
-          }
+        b0[j] = *(const bf16x8 *)(wrow[j]);
+        if (PF >= 2 && nk > 1) b1[j] = *(const bf16x8 *)(wrow[j] + 32);
       }
-      __syncthreads();
-      cur ^= 1;
-    } else if (wave == 0) {
-      // output layer: NOUT = 16 columns, one column tile, computed by wave 0
-      floatx4 acc[4];
+      for (int ks = 0; ks < nk; ++ks) {
+        if (PF >= 2) {
+          if (ks + 2 < nk) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-      for (int k0 = 0; k0 < K; k0 += 32) {
-        const bf16x8 b = *(const bf16x8 *)(W + (int64_t)fr * K + k0 + fk);
+            for (int j = 0; j < CT; ++j) b2[j] = *(const bf16x8 *)(wrow[j] + (ks + 2) * 32);
+          }
+        } else if (ks + 1 < nk) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const bf16x8 af = *(const bf16x8 *)(in + (i * 16 + fr) * P + k0 + fk);
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b, acc[i], 0, 0, 0);
+          for (int j = 0; j < CT; ++j) b1[j] = *(const bf16x8 *)(wrow[j] + (ks + 1) * 32);
+        }
+        bf16x8 af[RT];
+#pragma unroll
+        for (int i = 0; i < RT; ++i) af[i] = *(const bf16x8 *)(in + (i * 16 + fr) * P + ks * 32 + fk);
+        // operands swapped (W tile as A, activations as B): the accumulator holds the
+        // transposed tile, lane l owning 4 consecutive output columns of one row, so the
+        // epilogue writes 4 packed bf16 per 8-byte LDS store
+#pragma unroll
+        for (int i = 0; i < RT; ++i)
+#pragma unroll
+          for (int j = 0; j < CT; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0[j], af[i], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < CT; ++j) {
+          b0[j] = b1[j];
+          if (PF >= 2) b1[j] = b2[j];
         }
       }
-      const int c = fr;
-      if (c < a.n_out) {
-        const float b = bias[c];
+      // epilogue: bias + SiLU -> bf16 activation tile; lane (fr, g = lane / 16) holds
+      // rows i*16 + fr, columns n0 + 16 j + 4 g + r (r = 0..3)
+      const int g4 = (lane >> 4) * 4;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < CT; ++j) {
+        const int c = n0 + j * 16 + g4;
+        const floatx4 b = *(const floatx4 *)(bias + c);
+#pragma unroll
+        for (int i = 0; i < RT; ++i) {
+          bf16x4 v;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) v[r] = f2bf(silu(acc[i][j][r] + b[r]));
+          *(bf16x4 *)(out + (i * 16 + fr) * P + c) = v;
+        }
+      }
+      __syncthreads();
+      __bf16 *t = in; in = out; out = t;
+    } else {
+      // output layer: NOUT = 16 columns, one column tile; row tiles spread over the waves
+      for (int i = wave; i < RT; i += 4) {
+        floatx4 acc = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < K; k0 += 32) {
+          const bf16x8 b = *(const bf16x8 *)(W + (int64_t)fr * K + k0 + fk);
+          const bf16x8 af = *(const bf16x8 *)(in + (i * 16 + fr) * P + k0 + fk);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b, acc, 0, 0, 0);
+        }
+        const int c = fr;
+        if (c < a.n_out) {
+          const float b = bias[c];
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int64_t row = row0 + i * 16 + (lane >> 4) * 4 + r;
-            if (row < M) Z[row * ldz + c] = acc[i][r] + b;
+            if (row < M) Z[row * ldz + c] = acc[r] + b;
           }
+        }
       }
     }
   }
+}
+
+template <int H>
+int launch_mlp(const __bf16 *X, int64_t ldx, int64_t M, const MlpArgs &a, float *Z, int64_t ldz, hipStream_t st) {
+  constexpr int RT = GH_MLP_RT, PF = GH_MLP_PF;
+  constexpr size_t lds = mlp_lds_bytes<H, RT>();
+  const void *fn = (const void *)&mlp_forward_kernel<H, RT, PF>;
+  if (lds > 64 * 1024 &&
+      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    return GH_EHIP;
+  const dim3 grid((unsigned)((M + 16 * RT - 1) / (16 * RT)));
+  hipLaunchKernelGGL((mlp_forward_kernel<H, RT, PF>), grid, dim3(256), lds, st, X, ldx, M, a, Z, ldz);
+  return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
 }
 
 }  // namespace
@@ -391,12 +435,6 @@ extern "C" int gh_mlp_forward_bf16(const void *X, int64_t ldx, int64_t M, int32_
     a.bias[l] = bias[l];
   }
   if (M == 0) return GH_OK;
-  const dim3 grid((unsigned)((M + FBM - 1) / FBM));
-  if (hidden == 256)
-    hipLaunchKernelGGL(mlp_forward_kernel<256>, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)X, ldx, M,
-                       a, Z, ldz);
-  else
-    hipLaunchKernelGGL(mlp_forward_kernel<128>, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)X, ldx, M,
-                       a, Z, ldz);
-  return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
+  if (hidden == 256) return launch_mlp<256>((const __bf16 *)X, ldx, M, a, Z, ldz, (hipStream_t)stream);
+  return launch_mlp<128>((const __bf16 *)X, ldx, M, a, Z, ldz, (hipStream_t)stream);
 }
