@@ -119,3 +119,32 @@ def test_uncertainty_goal_stops_early(tmp_path):
     sim.run()
     assert stats.counts["events"] < 2_000_000
     assert sim.engine.goal_uncertainty <= 0.05
+
+
+def test_tle_dose_actor_script_matches_oracle(tmp_path):
+    """TLEDoseActor in a user script (threshold type 'energy', energy_min): the facade hands
+    a TLE tally to the C-ABI and the GPU result equals the oracle's on the same inputs."""
+    u = gate.g4_units
+    sim, dose, stats = build_sim(tmp_path)
+    del sim.actors["dose"]
+    tle = sim.add_actor("TLEDoseActor", "tle")
+    tle.attached_to = "ct"
+    tle.size = "like_image_volume"
+    tle.spacing = "like_image_volume"
+    tle.edep_uncertainty.active = True
+    tle.tle_threshold_type = "energy"
+    tle.tle_threshold = 120 * u.keV
+    tle.energy_min = 20 * u.keV
+    tle.output_filename = "tle.mhd"
+    print("[gpu case] facade TLEDoseActor", flush=True)
+    sim.run()
+    eng = sim.engine
+    tables, ph, tal = eng.inputs
+    assert tal.tle and abs(tal.tle_energy_max - 0.12) < 1e-12 and abs(tal.tle_energy_min - 0.02) < 1e-12
+    edep = np.zeros_like(tle.edep.image.array)
+    for spec, n, first in eng.executed:
+        r, _ = O.run(tables, ph, spec, tal, n, eng.seed, first=first)
+        edep += r["edep"]
+    assert edep.sum() > 0
+    assert np.max(np.abs(tle.edep.image.array - edep)) <= 1e-9 * edep.max()
+    assert (tmp_path / "tle_edep_uncertainty.mhd").exists()
