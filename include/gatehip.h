@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GH_ABI_VERSION 3
+#define GH_ABI_VERSION 4
 
 enum {
   GH_OK = 0,
@@ -192,6 +192,11 @@ int gh_get_tally(gh_ctx *ctx, int32_t sim_id, double *edep, double *edep2, doubl
 /* Recorded exit-face photons: 8 float arrays of length *n (x y z dx dy dz E w, phantom frame).
    out[k] may be host or device pointers (copied with the HIP runtime's default direction). */
 int gh_get_phsp(gh_ctx *ctx, int32_t sim_id, float *const out[8], uint64_t capacity, uint64_t *n);
+/* Empties the exit-face recorder of a simulation (tallies are kept), so that a long run can
+ * be recorded and consumed in history chunks, as the hit collection of a PhaseSpaceActor is
+ * emptied into its file every clear_every entries (FillToRootIfNeeded,
+ * GatePhaseSpaceActor.cpp:241-247). */
+int gh_reset_phsp(gh_ctx *ctx, int32_t sim_id);
 /* Per-history detector energy (tally.detector_label_min >= 0), n_histories floats; out may be a
    host or a device pointer. */
 int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n);
@@ -273,10 +278,31 @@ typedef struct gh_spect_desc {
 int gh_spect_pairs(const float *const rec[8], int64_t n, const gh_spect_desc *det, const float *angle_cs,
                    int64_t capacity, void *x16, int32_t *pix, float *wgt, float *const det_rec[8],
                    float *theta_phi, unsigned long long *count, void *stream);
-/* Deposited energy per event (device) -> window class (0 none, k = 1..n_win) after Gaussian
- * blur FWHM = r0 sqrt(e0 E) (digitizer energy blurring + energy windows); win: host array of
- * n_win (lo, hi) pairs in MeV, half-open. Event i uses noise keyed by (seed, first + i). */
-int gh_energy_windows(const float *edep, int64_t n, uint64_t seed, uint64_t first, float r0, float e0,
+/* Energy blurring laws of GateDigitizerBlurringActor (GateDigitizerBlurringActor.cpp:75-94;
+ * parameters as set by DigitizerBlurringActor.set_param_*, actors/digitizers.py:524-570):
+ *   GAUSSIAN        sigma fixed [MeV]
+ *   INVERSE_SQUARE  FWHM = resolution sqrt(reference E)       (FWHM/E = resolution at reference)
+ *   LINEAR          FWHM = (slope (E - reference) + resolution) E
+ * with sigma = FWHM / (2 sqrt(2 ln 2)). NONE leaves the energy as it is. */
+#define GH_BLUR_NONE 0
+#define GH_BLUR_GAUSSIAN 1
+#define GH_BLUR_INVERSE_SQUARE 2
+#define GH_BLUR_LINEAR 3
+typedef struct {
+  int32_t law;        /* GH_BLUR_* */
+  float sigma;        /* GAUSSIAN [MeV] */
+  float resolution;   /* INVERSE_SQUARE, LINEAR: FWHM / E at the reference energy */
+  float reference;    /* INVERSE_SQUARE, LINEAR: reference energy [MeV] */
+  float slope;        /* LINEAR [1/MeV] */
+} gh_blur;
+
+/* Deposited energy per event (device) -> window class (0 none, k = 1..n_win): the energy is
+ * blurred by `blur`, then the first window k with lo <= E < hi is taken
+ * (GateDigitizerEnergyWindowsActor::ApplyThreshold, GateDigitizerEnergyWindowsActor.cpp:117-131;
+ * with overlapping windows the caller runs one window per call: the blurred energy of an
+ * event is the same in every call). win: host array of n_win (lo, hi) pairs in MeV. Events
+ * with no energy get class 0. Event i draws its noise from the key (seed, first + i). */
+int gh_energy_windows(const float *edep, int64_t n, uint64_t seed, uint64_t first, const gh_blur *blur,
                       const float *win, int32_t n_win, int32_t *cls, void *stream);
 /* Projection images [head][window][ny][nx] (FP64, device) += weight x softmax(logits)[1..n_win]
  * per pair (ARF), or += weight at the simulated window (plain); img2 (optional) += squares. */

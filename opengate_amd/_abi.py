@@ -71,6 +71,14 @@ class SpectDesc(C.Structure):
     ]
 
 
+BLUR_LAWS = {"none": 0, "Gaussian": 1, "InverseSquare": 2, "Linear": 3}
+
+
+class Blur(C.Structure):
+    _fields_ = [("law", C.c_int32), ("sigma", C.c_float), ("resolution", C.c_float), ("reference", C.c_float),
+                ("slope", C.c_float)]
+
+
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("refill_interval", C.c_int32), ("chunk", C.c_int32),
                 ("slow_min", C.c_int32), ("refill_min", C.c_int32)]
@@ -121,6 +129,7 @@ SIGNATURES = {
     "gh_clear_simulations": (C.c_int, [C.c_void_p]),
     "gh_set_source": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(SourceDesc)]),
     "gh_reset_tally": (C.c_int, [C.c_void_p, C.c_int32]),
+    "gh_reset_phsp": (C.c_int, [C.c_void_p, C.c_int32]),
     "gh_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_uint64,
                          C.POINTER(Launch), C.POINTER(Counters)]),
     "gh_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_uint64, C.POINTER(Launch),
@@ -143,7 +152,7 @@ SIGNATURES = {
                                       C.POINTER(C.c_void_p), C.POINTER(FP), FP, C.c_int64, C.c_void_p]),
     "gh_spect_pairs": (C.c_int, [FP * 8, C.c_int64, C.POINTER(SpectDesc), C.c_void_p, C.c_int64, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.POINTER(FP), C.c_void_p, C.c_void_p, C.c_void_p]),
-    "gh_energy_windows": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.c_float, C.c_float, FP,
+    "gh_energy_windows": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(Blur), FP,
                                     C.c_int32, C.c_void_p, C.c_void_p]),
     "gh_arf_accumulate": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -177,7 +186,7 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.gh_abi_version() != 3:
+    if lib.gh_abi_version() != 4:
         raise RuntimeError("libgatehip.so ABI version mismatch")
     if path is None:
         _LIB = lib

@@ -20,8 +20,9 @@
 // stands for the whole periodic collimator.
 //
 // gh_energy_windows: deposited energy per event -> energy window (0 = none) after a
-// Gaussian blur with FWHM = r0 sqrt(e0 E) (resolution r0 at e0, inverse-square-root
-// law), noise from Philox4x32-10 (key = seed, counter = (event lo, event hi, "BLUR", 0)).
+// Gaussian blur whose width follows one of the digitizer laws (fixed sigma, inverse-square
+// root FWHM = resolution sqrt(reference E), linear FWHM/E), noise from Philox4x32-10
+// (key = seed, counter = (event lo, event hi, "BLUR", 0)).
 //
 // gh_arf_accumulate / gh_counts_accumulate: add the window probabilities of the network
 // (softmax of its n_win+1 logits, class 0 = not detected) or the simulated window of
@@ -32,6 +33,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "../../include/gatehip.h"
 #include "gh_device.h"
@@ -193,23 +195,34 @@ struct Windows {
   int n;
 };
 
+// standard deviation of the blurred energy of an event with energy e (MeV); FWHM laws are
+// converted with 1 / (2 sqrt(2 ln 2))
+__device__ __forceinline__ float blur_sigma(const gh_blur &B, float e) {
+  constexpr float fwhm_to_sigma = 0.42466090014400953f;
+  if (B.law == GH_BLUR_GAUSSIAN) return B.sigma;
+  if (B.law == GH_BLUR_INVERSE_SQUARE) return (B.resolution * sqrtf(B.reference * e)) * fwhm_to_sigma;
+  return ((B.slope * (e - B.reference) + B.resolution) * e) * fwhm_to_sigma;
+}
+
 __global__ void __launch_bounds__(256) energy_windows_kernel(const float *__restrict__ edep, int64_t n, uint32_t k0,
-                                                             uint32_t k1, uint64_t first, float r0, float e0,
-                                                             Windows W, int32_t *__restrict__ cls) {
+                                                             uint32_t k1, uint64_t first, gh_blur B, Windows W,
+                                                             int32_t *__restrict__ cls) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const float e = edep[i];
     int c = 0;
     if (e > 0.0f) {
-      const uint64_t id = first + (uint64_t)i;
-      uint32_t x0 = (uint32_t)id, x1 = (uint32_t)(id >> 32), x2 = BLUR_TAG, x3 = 0u;
-      philox10(x0, x1, x2, x3, k0, k1);
-      const float u0 = (float)((x0 >> 8) | 1u) * 0x1p-24f, u1 = (float)((x1 >> 8) | 1u) * 0x1p-24f;
-      float sn, csn;
-      gsincos(6.28318530717958647692f * u1, sn, csn);
-      const float g = sqrtf(-2.0f * glog(u0)) * csn;
-      const float sigma = (r0 * sqrtf(e0 * e)) * 0.42466090014400953f;  // FWHM / (2 sqrt(2 ln 2))
-      const float eb = fmaf(sigma, g, e);
+      float eb = e;
+      if (B.law != GH_BLUR_NONE) {
+        const uint64_t id = first + (uint64_t)i;
+        uint32_t x0 = (uint32_t)id, x1 = (uint32_t)(id >> 32), x2 = BLUR_TAG, x3 = 0u;
+        philox10(x0, x1, x2, x3, k0, k1);
+        const float u0 = (float)((x0 >> 8) | 1u) * 0x1p-24f, u1 = (float)((x1 >> 8) | 1u) * 0x1p-24f;
+        float sn, csn;
+        gsincos(6.28318530717958647692f * u1, sn, csn);
+        const float g = sqrtf(-2.0f * glog(u0)) * csn;
+        eb = fmaf(blur_sigma(B, e), g, e);
+      }
       for (int k = 0; k < W.n; ++k)
         if (eb >= W.lo[k] && eb < W.hi[k]) { c = k + 1; break; }
     }
@@ -294,9 +307,24 @@ extern "C" int gh_spect_pairs(const float *const rec[8], int64_t n, const gh_spe
   return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
 }
 
-extern "C" int gh_energy_windows(const float *edep, int64_t n, uint64_t seed, uint64_t first, float r0, float e0,
+// parameters of a blurring law are finite and give a non-negative width at the law's
+// reference point
+static bool blur_ok(const gh_blur &B) {
+  switch (B.law) {
+    case GH_BLUR_NONE: return true;
+    case GH_BLUR_GAUSSIAN: return B.sigma >= 0.0f && std::isfinite(B.sigma);
+    case GH_BLUR_INVERSE_SQUARE:
+      return B.resolution >= 0.0f && std::isfinite(B.resolution) && B.reference > 0.0f && std::isfinite(B.reference);
+    case GH_BLUR_LINEAR:
+      return B.resolution >= 0.0f && std::isfinite(B.resolution) && B.reference >= 0.0f &&
+             std::isfinite(B.reference) && std::isfinite(B.slope);
+    default: return false;
+  }
+}
+
+extern "C" int gh_energy_windows(const float *edep, int64_t n, uint64_t seed, uint64_t first, const gh_blur *blur,
                                  const float *win, int32_t n_win, int32_t *cls, void *stream) {
-  if (!edep || !cls || !win || n < 0 || n_win <= 0 || n_win > MAX_WIN || !(r0 >= 0.0f) || !(e0 > 0.0f))
+  if (!edep || !cls || !win || !blur || n < 0 || n_win <= 0 || n_win > MAX_WIN || !blur_ok(*blur))
     return GH_EINVAL;
   Windows W{};
   W.n = n_win;
@@ -307,7 +335,7 @@ extern "C" int gh_energy_windows(const float *edep, int64_t n, uint64_t seed, ui
   }
   if (n == 0) return GH_OK;
   hipLaunchKernelGGL(energy_windows_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, edep, n,
-                     (uint32_t)seed, (uint32_t)(seed >> 32), first, r0, e0, W, cls);
+                     (uint32_t)seed, (uint32_t)(seed >> 32), first, *blur, W, cls);
   return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
 }
 

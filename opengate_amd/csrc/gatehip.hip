@@ -104,18 +104,17 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 // --------------------------------------------------------------------- hot fields
+// Per-lane copy of what every Woodcock step reads (in a batched launch each lane may
+// follow another simulation); fields needed only at slow events (box size, world
+// material, tally alignment) are read from the simulation descriptor there.
 struct Hot {
   const uint8_t *labels;
-  int nx, ny, nz, world_mat;
+  int nx, ny, nz;
   float isx, isy, isz;
-  float Lx, Ly, Lz;
-  int aligned;
   int l4;  // labels packed two per byte (low nibble = even voxel)
   __device__ __forceinline__ void load(const SimDev &s) {
-    labels = s.labels; nx = s.nx; ny = s.ny; nz = s.nz; world_mat = s.world_mat;
+    labels = s.labels; nx = s.nx; ny = s.ny; nz = s.nz;
     isx = s.inv_s[0]; isy = s.inv_s[1]; isz = s.inv_s[2];
-    Lx = s.L[0]; Ly = s.L[1]; Lz = s.L[2];
-    aligned = s.aligned;
     l4 = s.lbits == 4;
   }
   __device__ __forceinline__ int label(int vox) const {
@@ -233,12 +232,19 @@ __device__ __forceinline__ void sq_flush_all(const SimDev &s, SqCache &c, LaneSt
 __device__ __forceinline__ void deposit(const SimDev &s, SqCache &c, int tv, float x, float dx,
                                         LaneStats &st) {
   if (tv < 0) { st.add_e(E_OUT, (double)x); return; }
-#define GH_SLOT_HIT(K)                                                     \
-  if (c.n > K && c.v##K == tv) {                                           \
-    c.s##K = c.s##K + x; c.d##K = c.d##K + dx; c.c##K += 1u; return;        \
+  // Slots are updated with value selects, never with one store per slot in separate
+  // branches: the compiler merges such stores into a store through a selected slot
+  // address, which puts the whole cache in scratch memory.
+  const bool h0 = c.n > 0 && c.v0 == tv, h1 = c.n > 1 && c.v1 == tv;
+  const bool h2 = c.n > 2 && c.v2 == tv, h3 = c.n > 3 && c.v3 == tv;
+  if (h0 || h1 || h2 || h3) {
+#define GH_SLOT_ADD(K)                                                     \
+    c.s##K = h##K ? c.s##K + x : c.s##K; c.d##K = h##K ? c.d##K + dx : c.d##K; \
+    c.c##K += h##K ? 1u : 0u;
+    GH_SLOT_ADD(0) GH_SLOT_ADD(1) GH_SLOT_ADD(2) GH_SLOT_ADD(3)
+#undef GH_SLOT_ADD
+    return;
   }
-  GH_SLOT_HIT(0) GH_SLOT_HIT(1) GH_SLOT_HIT(2) GH_SLOT_HIT(3)
-#undef GH_SLOT_HIT
   if (c.n == 4) {
     flush_slot(s, c.v0, c.s0, c.d0, c.c0, st);
     c.v0 = c.v1; c.s0 = c.s1; c.d0 = c.d1; c.c0 = c.c1;
@@ -247,17 +253,21 @@ __device__ __forceinline__ void deposit(const SimDev &s, SqCache &c, int tv, flo
     c.n = 3;
     st.inc(C_EVICT);
   }
-  if (c.n == 0) { c.v0 = tv; c.s0 = x; c.d0 = dx; c.c0 = 1u; }
-  else if (c.n == 1) { c.v1 = tv; c.s1 = x; c.d1 = dx; c.c1 = 1u; }
-  else if (c.n == 2) { c.v2 = tv; c.s2 = x; c.d2 = dx; c.c2 = 1u; }
-  else { c.v3 = tv; c.s3 = x; c.d3 = dx; c.c3 = 1u; }
+#define GH_SLOT_SET(K)                                                     \
+  {                                                                        \
+    const bool t = c.n == K;                                               \
+    c.v##K = t ? tv : c.v##K; c.s##K = t ? x : c.s##K;                     \
+    c.d##K = t ? dx : c.d##K; c.c##K = t ? 1u : c.c##K;                    \
+  }
+  GH_SLOT_SET(0) GH_SLOT_SET(1) GH_SLOT_SET(2) GH_SLOT_SET(3)
+#undef GH_SLOT_SET
   c.n += 1;
 }
 
 // tally voxel of a deposit at (x,y,z) in the phantom frame; vox = phantom voxel
 __device__ __forceinline__ int tally_voxel(const SimDev &s, const Hot &h, int vox, float x, float y,
                                            float z) {
-  if (h.aligned) return vox;
+  if (s.aligned) return vox;
   const int ix = (int)floorf((x - s.tcorner[0]) * s.tinv[0]);
   const int iy = (int)floorf((y - s.tcorner[1]) * s.tinv[1]);
   const int iz = (int)floorf((z - s.tcorner[2]) * s.tinv[2]);
@@ -387,6 +397,9 @@ __device__ __forceinline__ bool inside_box(const Hot &h, float x, float y, float
 // lane modes
 enum { M_DEAD = 0, M_STEP = 1, M_COLLIDE = 2, M_EXIT = 3, M_WORLD = 4 };
 
+// global history id of the lane's current history (event-stream counter words)
+__device__ __forceinline__ uint64_t history_of(const Rng &g) { return ((uint64_t)g.c1 << 32) | g.c0; }
+
 // u01 of a raw Philox word (odd multiples of 2^-24)
 __device__ __forceinline__ float u01w(uint32_t w) { return (float)((w >> 8) | 1u) * 0x1p-24f; }
 
@@ -426,8 +439,12 @@ __host__ __device__ inline LdsLayout lds_layout(int n_mat, int nb1, int nx, int 
 // track-length estimate w E mu_en L; the energy such a photon gives to electrons at real
 // interactions is not deposited. Photons at or below tle_emin (and below the tracking
 // cut) deposit their energy where they are and stop. Single-simulation launches only.
+// The batched form keeps a few more per-lane values (its simulation's grid); the
+// waves-per-EU bound holds it at 128 VGPRs (4 waves per SIMD), which it meets without
+// spilling, like the single-simulation forms do by themselves.
 template <bool BATCHED, int STAGE, int BLOCK, bool TLE>
-__global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BATCHED ? 4 : 1)))
+transport_kernel(Launch L) {
   extern __shared__ float lds[];
   const Phys &P = L.P;
   const int NB1 = P.nb + 1;
@@ -480,7 +497,7 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
   SqCache sq = {};
   float ev_edep = 0.0f;
   int mode = M_DEAD;
-  int cvox = 0, clab = 0;        // voxel and label of a pending collision
+  int cvox = 0;                  // voxel of a pending collision (-1: in the world material)
 
   // wave-uniform work state
   int cur_sim = 0;
@@ -497,10 +514,10 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
         // world outside the phantom box (homogeneous material or vacuum)
         float tn, tf;
         int face;
-        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
+        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, s.L[0], s.L[1], s.L[2], tn, tf, face);
         const bool hit = (tn < tf) && (tf > EXIT_EPS);
         const float tin = fmaxf(tn, 0.0f);
-        if (hot.world_mat < 0) {
+        if (s.world_mat < 0) {
           if (hit) {
             ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
             mode = M_STEP;
@@ -510,7 +527,7 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
             kill = true;
           }
         } else {
-          const int wm = hot.world_mat;
+          const int wm = s.world_mat;
           const float muw = lerp_node(mu_tab + wm * NB1, ec);
           const float tw = world_exit(s, ph.x, ph.y, ph.z, ph.u, ph.v, ph.w);
           const float d = (muw > 0.0f) ? (-glog(u01(rng)) / muw) : INFINITY;
@@ -525,7 +542,6 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
             ph.x = fmaf(d, ph.u, ph.x); ph.y = fmaf(d, ph.v, ph.y); ph.z = fmaf(d, ph.w, ph.z);
             st.inc(C_WORLD);
             mode = M_COLLIDE;
-            clab = wm;
             cvox = -1;
           }
         }
@@ -612,8 +628,8 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
           __ballot(mode == M_COLLIDE || mode == M_EXIT || mode == M_WORLD || (have_work && mode == M_DEAD)));
       if (n_step == 0u || n_slow >= slow_min) break;
       if (mode == M_STEP) {
-        const uint64_t gid = L.sims[BATCHED ? ph.sim : 0].first_hist + ph.hist;
-        uint32_t w0 = (uint32_t)gid, w1 = (uint32_t)(gid >> 32), w2 = (uint32_t)ph.sim, w3 = 0x80000000u | sblock;
+        // the event stream's counter words hold the global history id and the simulation
+        uint32_t w0 = rng.c0, w1 = rng.c1, w2 = rng.c2, w3 = 0x80000000u | sblock;
         philox10(w0, w1, w2, w3, rng.k0, rng.k1);
         sblock++;
 #pragma unroll
@@ -638,7 +654,6 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
               if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
                 mode = M_COLLIDE;
                 cvox = vox;
-                clab = lab;
               }
             } else {
               mode = M_EXIT;
@@ -653,13 +668,13 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
     // collision with the world material and no tally voxel (cvox = -1), so box and
     // world collisions share one copy of the interaction code.
     if (mode == M_COLLIDE || mode == M_EXIT || mode == M_WORLD) {
-      const SimDev &s = L.sims[BATCHED ? ph.sim : 0];
+      const SimDev &s = L.sims[BATCHED ? (int)rng.c2 : 0];
       bool kill = false;
       if (mode == M_EXIT) {
         // leaves the phantom box: exit point from the last position inside
         float tn, tf;
         int face;
-        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, hot.Lx, hot.Ly, hot.Lz, tn, tf, face);
+        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, s.L[0], s.L[1], s.L[2], tn, tf, face);
         const float t = fmaxf(tf, 0.0f);
         const float ex = fmaf(t, ph.u, ph.x), ey = fmaf(t, ph.v, ph.y), ez = fmaf(t, ph.w, ph.z);
         if (s.rec_face >= 0 && (face == s.rec_face || s.rec_face == GH_FACE_ALL)) {
@@ -677,7 +692,7 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
           st.inc(C_RECORDED);
           st.add_e(E_ESC, (double)(ph.E * ph.wt));
           kill = true;
-        } else if (hot.world_mat < 0) {
+        } else if (s.world_mat < 0) {
           st.inc(C_ESCAPED);
           st.add_e(E_ESC, (double)(ph.E * ph.wt));
           kill = true;
@@ -689,7 +704,8 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
       if (mode == M_WORLD && !kill) world_step(s, kill);
       if (mode == M_COLLIDE && !kill) {
         const bool in_world = cvox < 0;
-        const int lab = clab;
+        // the label is read again rather than kept in a register from the step phase
+        const int lab = in_world ? s.world_mat : hot.label(cvox);
         const float r = u01(rng);
         float pc, pr;
         branching(T, NB1, lab, ec, pc, pr);
@@ -745,7 +761,7 @@ __global__ void __launch_bounds__(BLOCK) transport_kernel(Launch L) {
       }
       if (kill) {
         sq_flush_all(s, sq, st);
-        if (s.event_edep) s.event_edep[ph.hist] = ev_edep;
+        if (s.event_edep) s.event_edep[history_of(rng) - s.first_hist] = ev_edep;
         mode = M_DEAD;
       }
     }
@@ -1293,6 +1309,18 @@ int gh_reset_tally(gh_ctx *ctx, int32_t sim_id) {
     if (a) HIPCHK(hipMemsetAsync(a, 0, bytes, ctx->stream));
   if (h.dev.rec_count) HIPCHK(hipMemsetAsync(h.dev.rec_count, 0, sizeof(unsigned long long), ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  return GH_OK;
+}
+
+int gh_reset_phsp(gh_ctx *ctx, int32_t sim_id) {
+  if (!ctx) return GH_EINVAL;
+  if (sim_id < 0 || sim_id >= (int32_t)ctx->sims.size()) FAIL(GH_EINVAL, "gh_reset_phsp: bad sim id");
+  HIPCHK(hipSetDevice(ctx->device));
+  SimHost &h = ctx->sims[sim_id];
+  if (h.dev.rec_count) {
+    HIPCHK(hipMemsetAsync(h.dev.rec_count, 0, sizeof(unsigned long long), ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
   return GH_OK;
 }
 

@@ -27,8 +27,36 @@ from .mlp import MLP
 SIZES = [3, 128, 128, 128, 4]
 # Tc-99m triple energy window (MeV): lower scatter, photopeak 140.5 keV +-10 %, upper scatter
 WINDOWS = ((0.114, 0.126), (0.126, 0.154), (0.154, 0.166))
-RESOLUTION = (0.095, 0.1405)  # FWHM / E at E0 (MeV), FWHM ~ sqrt(E)
 E_RANGE = (0.010, 0.160)
+
+
+@dataclass(frozen=True)
+class Blur:
+    """Energy blurring law of the digitizer (GateDigitizerBlurringActor.cpp:75-94): "none",
+    "Gaussian" (fixed sigma, MeV), "InverseSquare" (FWHM = resolution sqrt(reference E)) or
+    "Linear" (FWHM = (slope (E - reference) + resolution) E)."""
+
+    law: str = "InverseSquare"
+    sigma: float = 0.0
+    resolution: float = 0.095
+    reference: float = 0.1405
+    slope: float = 0.0
+
+    def desc(self):
+        if self.law not in A.BLUR_LAWS:
+            raise ValueError(f"unknown blurring law {self.law!r} (one of {tuple(A.BLUR_LAWS)})")
+        return A.Blur(A.BLUR_LAWS[self.law], float(self.sigma), float(self.resolution), float(self.reference),
+                      float(self.slope))
+
+
+# NaI at 140.5 keV: 9.5 % FWHM, FWHM ~ sqrt(E)
+BLUR = Blur()
+
+
+def windows_overlap(windows):
+    """True when two half-open windows [lo, hi) share energies."""
+    w = sorted((float(lo), float(hi)) for lo, hi in windows)
+    return any(w[k + 1][0] < w[k][1] for k in range(len(w) - 1))
 
 
 def _torch():
@@ -109,14 +137,16 @@ class HeadModel:
         lab[nzc + nzg:] = col[None]
         return lab
 
-    def simulation(self, capacity_events=0):
-        """(tables, phantom, tally) of the head model; the tally scores the per-event NaI energy."""
+    def simulation(self, capacity_events=0, e_max=None):
+        """(tables, phantom, tally) of the head model; the tally scores the per-event NaI energy.
+        The physics tables cover E_RANGE and e_max (MeV) if larger."""
         from ..physics import xs
         from ..physics.materials import MaterialDatabase
         from ..transport import PhantomSpec, TallySpec
 
         db = MaterialDatabase()
-        tables = xs.build_tables([db.get("G4_AIR"), db.get("G4_Pb"), db.get("G4_SODIUM_IODIDE")], E_RANGE[1] * 1.001)
+        top = max(E_RANGE[1], float(e_max or 0.0))
+        tables = xs.build_tables([db.get("G4_AIR"), db.get("G4_Pb"), db.get("G4_SODIUM_IODIDE")], top * 1.001)
         ph = PhantomSpec(self.labels(), (self.voxel,) * 3, world_material=-1)
         tal = TallySpec(size=(1, 1, 1), spacing=(self.patch, self.patch, self.face_z), aligned=False, edep2=False,
                         detector_label_min=2)
@@ -137,6 +167,7 @@ class Heads:
     angle0: float = 0.0
     theta_max: float = math.radians(10.0)
     model: HeadModel = None
+    e_range: tuple = E_RANGE
 
     def __post_init__(self):
         if self.model is None:
@@ -157,7 +188,7 @@ class Heads:
         d.crystal_depth = float(self.model.crystal_depth)
         d.pitch = float(self.model.pitch)
         d.patch_origin = (C.c_float * 3)(*[float(v) for v in self.model.origin])
-        d.norm = (C.c_float * 6)(*[float(v) for v in input_norm(self.theta_max)])
+        d.norm = (C.c_float * 6)(*[float(v) for v in input_norm(self.theta_max, self.e_range)])
         return d
 
     def angle_table(self):
@@ -165,9 +196,9 @@ class Heads:
         return np.stack([np.cos(a), np.sin(a)], 1).astype(np.float32)
 
 
-def input_norm(theta_max):
-    """NN input scaling: E -> [-1, 1] over E_RANGE, theta -> [-1, 1] over [0, theta_max], phi / pi."""
-    lo, hi = E_RANGE
+def input_norm(theta_max, e_range=E_RANGE):
+    """NN input scaling: E -> [-1, 1] over e_range, theta -> [-1, 1] over [0, theta_max], phi / pi."""
+    lo, hi = e_range
     return (2.0 / (hi - lo), -(hi + lo) / (hi - lo), 2.0 / theta_max, -1.0, 1.0 / math.pi, 0.0)
 
 
@@ -199,10 +230,11 @@ def spect_pairs(lib, rec, n, heads, cs, buf):
     return m
 
 
-def energy_windows(lib, edep, n, seed, first, cls, windows=WINDOWS, resolution=RESOLUTION):
+def energy_windows(lib, edep, n, seed, first, cls, windows=WINDOWS, blur=BLUR):
+    """Window class (device int32, 0 = none) of n event energies after the blurring law."""
     win = (C.c_float * (2 * len(windows)))(*[float(v) for w in windows for v in w])
-    rc = lib.gh_energy_windows(_ptr(edep), int(n), int(seed) & (2**64 - 1), int(first), float(resolution[0]),
-                               float(resolution[1]), win, len(windows), _ptr(cls), _stream())
+    rc = lib.gh_energy_windows(_ptr(edep), int(n), int(seed) & (2**64 - 1), int(first), C.byref(blur.desc()), win,
+                               len(windows), _ptr(cls), _stream())
     if rc != 0:
         raise RuntimeError(f"gh_energy_windows failed ({rc})")
     return cls
@@ -212,8 +244,10 @@ def energy_windows(lib, edep, n, seed, first, cls, windows=WINDOWS, resolution=R
 class ArfModel:
     """ARF network and its training on labelled detector-model photons."""
 
-    def __init__(self, theta_max=math.radians(10.0), device="cuda", seed=0, sizes=None, windows=WINDOWS):
+    def __init__(self, theta_max=math.radians(10.0), device="cuda", seed=0, sizes=None, windows=WINDOWS,
+                 e_range=E_RANGE):
         self.theta_max = float(theta_max)
+        self.e_range = (float(e_range[0]), float(e_range[1]))
         self.windows = tuple(tuple(w) for w in windows)
         self.sizes = list(sizes or SIZES)
         if self.sizes[-1] != len(self.windows) + 1:
@@ -229,19 +263,27 @@ class ArfModel:
     def inputs(self, E, theta, phi):
         """NN input rows (bf16, padded to 8) from float tensors."""
         torch = _torch()
-        n = input_norm(self.theta_max)
+        n = input_norm(self.theta_max, self.e_range)
         X = torch.zeros(E.shape[0], self.mlp.psizes[0], device=self.dev, dtype=torch.bfloat16)
         X[:, 0] = (E * n[0] + n[1]).to(torch.bfloat16)
         X[:, 1] = (theta * n[2] + n[3]).to(torch.bfloat16)
         X[:, 2] = (phi * n[4] + n[5]).to(torch.bfloat16)
         return X
 
-    def fit(self, X, cls, steps=3000, batch=16384, lr=2e-3, lr_min=5e-5, seed=1, log_every=0):
-        """Cross-entropy training on rows X (bf16 inputs) with classes cls (int, 0..n_win)."""
+    def fit(self, X, cls, steps=3000, batch=16384, lr=2e-3, lr_min=5e-5, seed=1, log_every=0, class_weight=None):
+        """Cross-entropy training on rows X (bf16 inputs) with classes cls (int, 0..n_win).
+        class_weight (one per class, optional) weights the rows of each class in the loss: a
+        training set whose undetected rows were kept with probability 1/r (Russian roulette of
+        GateARFTrainingDatasetActor.cpp:117-121) is trained with class_weight[0] = r."""
         torch = _torch()
         N = X.shape[0]
         g = torch.Generator(device=self.dev).manual_seed(int(seed))
         cls = cls.long()
+        cw = None
+        if class_weight is not None:
+            cw = torch.as_tensor(class_weight, dtype=torch.float32, device=self.dev)
+            if cw.numel() != self.n_win + 1 or bool((cw < 0).any()):
+                raise ValueError("class_weight needs one non-negative weight per class")
         losses = torch.empty(steps, device=self.dev)
         for it in range(steps):
             self.mlp.lr = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * it / max(1, steps)))
@@ -250,10 +292,17 @@ class ArfModel:
             z = self.mlp.forward(xb)
             logp = torch.log_softmax(z, 1)
             yb = cls[idx]
-            losses[it] = -logp.gather(1, yb[:, None]).mean()
             grad = logp.exp()
             grad[torch.arange(batch, device=self.dev), yb] -= 1.0
-            dW, db = self.mlp.backward(grad / batch)
+            if cw is None:
+                losses[it] = -logp.gather(1, yb[:, None]).mean()
+                grad /= batch
+            else:
+                w = cw[yb]
+                ws = w.sum().clamp_min(1e-30)
+                losses[it] = -(w * logp.gather(1, yb[:, None])[:, 0]).sum() / ws
+                grad *= (w / ws)[:, None]
+            dW, db = self.mlp.backward(grad)
             self.mlp.step(dW, db)
             if log_every and (it % log_every == 0 or it == steps - 1):
                 print(f"[arf fit] step {it} loss {float(losses[it]):.5f}", flush=True)
@@ -293,14 +342,16 @@ class ArfModel:
         torch = _torch()
         d = self.mlp.state_dict()
         torch.save({"sizes": torch.tensor(self.sizes), "W": d["W"], "b": d["b"],
-                    "theta_max": torch.tensor(self.theta_max), "windows": torch.tensor(self.windows)}, path)
+                    "theta_max": torch.tensor(self.theta_max), "windows": torch.tensor(self.windows),
+                    "e_range": torch.tensor(self.e_range, dtype=torch.float64)}, path)
 
     @classmethod
     def load(cls, path, device="cuda"):
         torch = _torch()
         d = torch.load(path, map_location="cpu", weights_only=True)
+        e_range = tuple(d["e_range"].tolist()) if "e_range" in d else E_RANGE
         m = cls(float(d["theta_max"]), device=device, sizes=d["sizes"].tolist(),
-                windows=[tuple(w) for w in d["windows"].tolist()])
+                windows=[tuple(w) for w in d["windows"].tolist()], e_range=e_range)
         m.mlp.load_state_dict({"W": d["W"], "b": d["b"]})
         return m
 
@@ -309,19 +360,19 @@ class ArfModel:
 class HeadSimulator:
     """Plain transport through the head model: labels photons with their energy window."""
 
-    def __init__(self, model=None, device=0):
+    def __init__(self, model=None, device=0, e_max=None):
         from ..transport import GpuEngine
 
         self.model = model or HeadModel()
         self.engine = GpuEngine(device)
-        self.tables, self.phantom, self.tally = self.model.simulation()
+        self.tables, self.phantom, self.tally = self.model.simulation(e_max=e_max)
         self.engine.set_physics(self.tables)
         self.sid = None
 
-    def windows(self, rec, n, seed, first=0, cls=None, windows=WINDOWS):
+    def event_energy(self, rec, n, seed, first=0):
         """Transport n detector-frame photons (8 device arrays, head-model frame) and return
-        the window class of each (device int32)."""
-        torch = _torch()
+        the energy each leaves in the crystal (device float32, MeV): the sum over the event's
+        crystal hits that an adder digitizer forms (GateDigitizerAdderActor)."""
         from ..transport import SourceSpec
 
         src = SourceSpec(pos_type="phsp", phsp=[a[:n] for a in rec])
@@ -330,11 +381,31 @@ class HeadSimulator:
         else:
             self.engine.set_source(self.sid, src)
         self.engine.run([n], seed, first_history=[first])
-        edep = self.engine.get_event_edep(self.sid, n, device=True)
+        return self.engine.get_event_edep(self.sid, n, device=True)
+
+    def windows(self, rec, n, seed, first=0, cls=None, windows=WINDOWS, blur=BLUR):
+        """Window class of each of n detector-frame photons (device int32, 0 = none)."""
+        torch = _torch()
+        edep = self.event_energy(rec, n, seed, first)
         if cls is None:
             cls = torch.empty(n, device=edep.device, dtype=torch.int32)
-        energy_windows(self.engine.lib, edep, n, seed, first, cls, windows)
+        energy_windows(self.engine.lib, edep, n, seed, first, cls, windows, blur)
         return cls
+
+    def photons(self, a, b, E, theta, phi, weight=None):
+        """Detector-frame photons (8 device arrays) entering the collimator face at the lattice-
+        cell point (a, b) in [0, 1)^2 (fractions of the cell spanned by the first lattice vector
+        and the row offset, as gh_spect_pairs reduces entry points) with energy E and direction
+        (theta, phi) relative to the inward face normal (radians)."""
+        torch = _torch()
+        m = self.model
+        ox, oy, oz = m.origin
+        x = ox + a * m.pitch
+        y = oy + b * 0.5 * math.sqrt(3.0) * m.pitch
+        ct = torch.cos(theta)
+        st = torch.sqrt(torch.clamp(1.0 - ct * ct, min=0.0))
+        w = torch.ones_like(x) if weight is None else weight
+        return [x, y, torch.full_like(x, oz), st * torch.cos(phi), st * torch.sin(phi), -ct, E, w]
 
     def training_photons(self, n, seed, theta_max, device="cuda", theta_power=2.0, e_focus=(0.5, 0.100)):
         """n photons uniformly over the lattice cell that gh_spect_pairs reduces entries to
@@ -348,45 +419,54 @@ class HeadSimulator:
         scattered 140 keV photons get half of the samples."""
         torch = _torch()
         g = torch.Generator(device=device).manual_seed(int(seed))
-        m = self.model
         a = torch.rand(n, device=device, generator=g)
         b = torch.rand(n, device=device, generator=g)
-        ox, oy, oz = m.origin
-        x = ox + a * m.pitch
-        y = oy + b * 0.5 * math.sqrt(3.0) * m.pitch
         lo = torch.where(torch.rand(n, device=device, generator=g) < e_focus[0], e_focus[1], E_RANGE[0])
         E = lo + (E_RANGE[1] - lo) * torch.rand(n, device=device, generator=g)
         theta = theta_max * torch.rand(n, device=device, generator=g) ** theta_power
-        ct = torch.cos(theta)
         phi = (2.0 * torch.rand(n, device=device, generator=g) - 1.0) * math.pi
-        st = torch.sqrt(1.0 - ct * ct)
-        rec = [x, y, torch.full_like(x, oz), st * torch.cos(phi), st * torch.sin(phi), -ct, E, torch.ones_like(x)]
-        return rec, E, theta, phi
+        return self.photons(a, b, E, theta, phi), E, theta, phi
 
     def close(self):
         self.engine.close()
 
 
 # ---------------------------------------------------------------------- projections
-def project(rec, n, heads, arf=None, head_sim=None, seed=0, chunk=1 << 22, squared=True, pairs_per_photon=3.0):
+def project(rec, n, heads, arf=None, head_sim=None, seed=0, chunk=1 << 22, squared=True, pairs_per_photon=3.0,
+            windows=None, blur=BLUR, out=None):
     """SPECT projections of n photons that left a phantom (8 device arrays x y z u v w E
     weight, phantom frame, vacuum outside), for the gantry `heads`: learned (arf, an
-    ArfModel) and/or plain (head_sim, a HeadSimulator). Images are FP64 device tensors
-    [n_proj * n_win, ny, nx] (projection-major, as ARFActor's output), with squared sums
-    when `squared`. Returns a dict with the images, the pair count and the time spent in
-    each stage (s)."""
+    ArfModel, whose windows are those it was trained on) and/or plain (head_sim, a
+    HeadSimulator, with the energy `windows` (default: the ArfModel's, else WINDOWS) after
+    the blurring law `blur`; overlapping windows each count an event that falls in them, as
+    the channels of DigitizerEnergyWindowsActor do). Images are FP64 device tensors
+    [n_proj * n_win, ny, nx] (projection-major, as ARFActor's and DigitizerProjectionActor's
+    outputs), with squared sums when `squared`. Passing the dict of a previous call as `out`
+    adds to its images and continues its pair numbering (the noise streams of the plain
+    path), so a long run can be projected in photon chunks. Returns the dict with the
+    images, the pair count and the time spent in each stage (s)."""
     import time
 
     torch = _torch()
     lib = (arf or head_sim.engine).lib
     dev = rec[0].device
-    n_win = arf.n_win if arf is not None else len(WINDOWS)
+    if arf is not None and (abs(heads.theta_max - arf.theta_max) > 1e-12 or tuple(heads.e_range) != arf.e_range):
+        raise ValueError("the gantry's acceptance angle and energy range must be those the ARF network was trained "
+                         f"with (theta_max {arf.theta_max}, e_range {arf.e_range})")
+    if windows is None:
+        windows = arf.windows if arf is not None else WINDOWS
+    windows = tuple(tuple(float(v) for v in w) for w in windows)
+    if arf is not None and head_sim is not None and len(windows) != arf.n_win:
+        raise ValueError("learned and plain projections in one call need the same number of windows")
+    n_win = arf.n_win if arf is not None else len(windows)
     shape = (heads.n_proj * n_win, heads.ny, heads.nx)
-    out = {"pairs": 0, "t_pairs": 0.0, "t_arf": 0.0, "t_plain": 0.0}
-    for name, on in (("arf", arf is not None), ("plain", head_sim is not None)):
-        if on:
-            out[name] = torch.zeros(shape, device=dev, dtype=torch.float64)
-            out[name + "2"] = torch.zeros(shape, device=dev, dtype=torch.float64) if squared else None
+    if out is None:
+        out = {"pairs": 0, "t_pairs": 0.0, "t_arf": 0.0, "t_plain": 0.0}
+        for name, on in (("arf", arf is not None), ("plain", head_sim is not None)):
+            if on:
+                out[name] = torch.zeros(shape, device=dev, dtype=torch.float64)
+                out[name + "2"] = torch.zeros(shape, device=dev, dtype=torch.float64) if squared else None
+    overlap = head_sim is not None and windows_overlap(windows)
     cs = torch.from_numpy(heads.angle_table()).to(dev)
     step = int(chunk)
     buf = PairBuffer(int(step * pairs_per_photon) + 1024, dev, det=head_sim is not None, x16=arf is not None)
@@ -412,16 +492,36 @@ def project(rec, n, heads, arf=None, head_sim=None, seed=0, chunk=1 << 22, squar
             out["t_arf"] += time.perf_counter() - t
         if head_sim is not None and m:
             t = time.perf_counter()
-            cls = head_sim.windows(buf.det, m, seed, first=out["pairs"], windows=WINDOWS[:n_win])
-            rc = lib.gh_counts_accumulate(_ptr(cls), _ptr(buf.pix), _ptr(buf.wgt), m, n_win, heads.plane,
-                                          _ptr(out["plain"]), _ptr(out.get("plain2")), _stream())
-            if rc != 0:
-                raise RuntimeError(f"gh_counts_accumulate failed ({rc})")
+            edep = head_sim.event_energy(buf.det, m, seed, first=out["pairs"])
+            cls = torch.empty(m, device=dev, dtype=torch.int32)
+            if not overlap:
+                energy_windows(lib, edep, m, seed, out["pairs"], cls, windows, blur)
+                _counts(lib, cls, buf, m, n_win, heads.plane, out["plain"], out.get("plain2"))
+            else:
+                # one window per pass: an event's blurred energy is the same in every pass
+                tmp = torch.zeros((heads.n_proj, heads.ny, heads.nx), device=dev, dtype=torch.float64)
+                tmp2 = torch.zeros_like(tmp) if out.get("plain2") is not None else None
+                for c, w in enumerate(windows):
+                    energy_windows(lib, edep, m, seed, out["pairs"], cls, (w,), blur)
+                    tmp.zero_()
+                    if tmp2 is not None:
+                        tmp2.zero_()
+                    _counts(lib, cls, buf, m, 1, heads.plane, tmp, tmp2)
+                    out["plain"].view(heads.n_proj, n_win, heads.ny, heads.nx)[:, c] += tmp
+                    if tmp2 is not None:
+                        out["plain2"].view(heads.n_proj, n_win, heads.ny, heads.nx)[:, c] += tmp2
             torch.cuda.synchronize()
             out["t_plain"] += time.perf_counter() - t
         out["pairs"] += m
         s += k
     return out
+
+
+def _counts(lib, cls, buf, m, n_win, plane, img, img2):
+    rc = lib.gh_counts_accumulate(_ptr(cls), _ptr(buf.pix), _ptr(buf.wgt), m, n_win, plane, _ptr(img), _ptr(img2),
+                                  _stream())
+    if rc != 0:
+        raise RuntimeError(f"gh_counts_accumulate failed ({rc})")
 
 
 def spect_pairs_checked(lib, rec, n, heads, cs, buf):

@@ -21,8 +21,8 @@ def _lib():
         FP = C.POINTER(C.c_float)
         L.ref_spect_pairs.argtypes = [FP * 8, C.c_int64, C.POINTER(A.SpectDesc), FP, C.c_int64, FP,
                                       C.POINTER(C.c_int32), FP, C.POINTER(FP), FP, C.POINTER(C.c_uint64)]
-        L.ref_energy_windows.argtypes = [FP, C.c_int64, C.c_uint64, C.c_uint64, C.c_float, C.c_float, FP, C.c_int,
-                                         C.POINTER(C.c_int32)]
+        L.ref_energy_windows.argtypes = [FP, C.c_int64, C.c_uint64, C.c_uint64, C.c_int, C.c_float, C.c_float,
+                                         C.c_float, C.c_float, FP, C.c_int, C.POINTER(C.c_int32)]
         L._arf_ready = True
     return L
 
@@ -49,12 +49,18 @@ def spect_pairs(rec, desc, cs, capacity):
     return res, int(cnt.value)
 
 
-def energy_windows(edep, seed, first, r0, e0, windows):
+BLUR_LAWS = {"none": 0, "Gaussian": 1, "InverseSquare": 2, "Linear": 3}
+
+
+def energy_windows(edep, seed, first, windows, law="InverseSquare", sigma=0.0, resolution=0.0, reference=0.0,
+                   slope=0.0):
+    """Window class (0 none, k = 1..) of each event energy after the blurring law `law`."""
     edep = np.ascontiguousarray(edep, np.float32)
     win = np.ascontiguousarray(np.asarray(windows, np.float32).reshape(-1, 2))
     cls = np.zeros(len(edep), np.int32)
-    _lib().ref_energy_windows(_fp(edep), len(edep), int(seed) & (2**64 - 1), int(first), float(r0), float(e0),
-                              _fp(win), len(win), cls.ctypes.data_as(C.POINTER(C.c_int32)))
+    _lib().ref_energy_windows(_fp(edep), len(edep), int(seed) & (2**64 - 1), int(first), BLUR_LAWS[law],
+                              float(sigma), float(resolution), float(reference), float(slope), _fp(win), len(win),
+                              cls.ctypes.data_as(C.POINTER(C.c_int32)))
     return cls
 
 

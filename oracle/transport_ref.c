@@ -1045,23 +1045,34 @@ void ref_spect_pairs(const float *const rec[8], int64_t n, const gh_spect_desc *
   *count = cnt;
 }
 
-void ref_energy_windows(const float *edep, int64_t n, uint64_t seed, uint64_t first, float r0, float e0,
-                        const float *win, int n_win, int32_t *cls) {
+/* width of the blurred energy (laws of GateDigitizerBlurringActor.cpp:75-94): law 1 fixed
+ * sigma, 2 FWHM = resolution sqrt(reference e), 3 FWHM = (slope (e - reference) + resolution) e */
+static float ref_blur_sigma(int law, float sigma, float resolution, float reference, float slope, float e) {
+  const float fwhm_to_sigma = 0.42466090014400953f;
+  if (law == 1) return sigma;
+  if (law == 2) return (resolution * sqrtf(reference * e)) * fwhm_to_sigma;
+  return ((slope * (e - reference) + resolution) * e) * fwhm_to_sigma;
+}
+
+void ref_energy_windows(const float *edep, int64_t n, uint64_t seed, uint64_t first, int law, float sigma,
+                        float resolution, float reference, float slope, const float *win, int n_win, int32_t *cls) {
   const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   for (int64_t i = 0; i < n; ++i) {
     const float e = edep[i];
     int c = 0;
     if (e > 0.0f) {
-      const uint64_t id = first + (uint64_t)i;
-      const uint32_t ctr[4] = {(uint32_t)id, (uint32_t)(id >> 32), 0x424C5552u, 0u};
-      uint32_t x[4];
-      ref_philox4x32_10(ctr, key, x);
-      const float u0 = (float)((x[0] >> 8) | 1u) * 0x1p-24f, u1 = (float)((x[1] >> 8) | 1u) * 0x1p-24f;
-      float sn, csn;
-      ref_sincos(6.28318530717958647692f * u1, &sn, &csn);
-      const float g = sqrtf(-2.0f * ref_log(u0)) * csn;
-      const float sigma = (r0 * sqrtf(e0 * e)) * 0.42466090014400953f;
-      const float eb = fmaf(sigma, g, e);
+      float eb = e;
+      if (law != 0) {
+        const uint64_t id = first + (uint64_t)i;
+        const uint32_t ctr[4] = {(uint32_t)id, (uint32_t)(id >> 32), 0x424C5552u, 0u};
+        uint32_t x[4];
+        ref_philox4x32_10(ctr, key, x);
+        const float u0 = (float)((x[0] >> 8) | 1u) * 0x1p-24f, u1 = (float)((x[1] >> 8) | 1u) * 0x1p-24f;
+        float sn, csn;
+        ref_sincos(6.28318530717958647692f * u1, &sn, &csn);
+        const float g = sqrtf(-2.0f * ref_log(u0)) * csn;
+        eb = fmaf(ref_blur_sigma(law, sigma, resolution, reference, slope, e), g, e);
+      }
       for (int k = 0; k < n_win; ++k)
         if (eb >= win[2 * k] && eb < win[2 * k + 1]) { c = k + 1; break; }
     }

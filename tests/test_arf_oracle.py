@@ -122,22 +122,50 @@ def test_lattice_reduction_moves_by_whole_lattice_vectors():
     assert checked > 20
 
 
+def _window_chi2(cls, e0, sigma, win):
+    """chi2 of the window frequencies of events at energy e0 blurred by N(0, sigma)"""
+    n = len(cls)
+    p = [stats.norm.cdf(hi, e0, sigma) - stats.norm.cdf(lo, e0, sigma) for lo, hi in win]
+    p = np.array([1 - sum(p), *p])
+    f = np.bincount(cls, minlength=len(win) + 1)
+    return float(np.sum((f - n * p) ** 2 / (n * p))), f, n * p
+
+
 def test_energy_windows_without_and_with_blur():
     win = ((0.114, 0.126), (0.126, 0.154), (0.154, 0.166))
-    e = np.array([0.0, 0.11, 0.12, 0.1405, 0.16, 0.2], np.float32)
-    np.testing.assert_array_equal(R.energy_windows(e, 1, 0, 0.0, 0.1405, win), [0, 0, 1, 2, 3, 0])
+    e = np.array([0.0, 0.11, 0.12, 0.1405, 0.16, 0.2, 0.114, 0.126, 0.166], np.float32)
+    # no blur: half-open windows [lo, hi) (GateDigitizerEnergyWindowsActor.cpp:127), first match
+    expect = [0, 0, 1, 2, 3, 0, 1, 2, 0]
+    np.testing.assert_array_equal(R.energy_windows(e, 1, 0, win, law="none"), expect)
+    # InverseSquare with zero resolution draws noise but adds nothing
+    np.testing.assert_array_equal(R.energy_windows(e, 1, 0, win, resolution=0.0, reference=0.1405), expect)
     n = 200000
-    cls = R.energy_windows(np.full(n, 0.1405, np.float32), 7, 1000, 0.095, 0.1405, win)
-    sigma = 0.095 * 0.1405 / 2.354820045
-    p = [stats.norm.cdf(0.114, 0.1405, sigma), *[stats.norm.cdf(hi, 0.1405, sigma) - stats.norm.cdf(lo, 0.1405, sigma)
-                                                  for lo, hi in win]]
-    p = np.array([1 - sum(p[1:]), *p[1:]])
-    f = np.bincount(cls, minlength=4)
-    chi2 = np.sum((f - n * p) ** 2 / (n * p))
-    assert chi2 < 30, (f, n * p)
+    cls = R.energy_windows(np.full(n, 0.1405, np.float32), 7, 1000, win, resolution=0.095, reference=0.1405)
+    chi2, f, fe = _window_chi2(cls, 0.1405, 0.095 * 0.1405 / 2.354820045, win)
+    assert chi2 < 30, (f, fe)
     # the noise of event i depends only on (seed, first + i)
-    np.testing.assert_array_equal(R.energy_windows(np.full(10, 0.1405, np.float32), 7, 1500, 0.095, 0.1405, win),
-                                  cls[500:510])
+    np.testing.assert_array_equal(R.energy_windows(np.full(10, 0.1405, np.float32), 7, 1500, win, resolution=0.095,
+                                                   reference=0.1405), cls[500:510])
+
+
+@pytest.mark.parametrize("law,e0,sigma,kw", [
+    # InverseSquare away from the reference: FWHM = R sqrt(E0 E) (GateDigitizerBlurringActor.cpp:80-86)
+    ("InverseSquare", 0.120, 0.095 * math.sqrt(0.1405 * 0.120) / 2.354820045,
+     {"resolution": 0.095, "reference": 0.1405}),
+    # Gaussian: fixed sigma (:75-78)
+    ("Gaussian", 0.1405, 0.008, {"sigma": 0.008}),
+    # Linear: FWHM = (slope (E - E0) + R) E (:88-94)
+    ("Linear", 0.150, (0.4 * (0.150 - 0.1405) + 0.095) * 0.150 / 2.354820045,
+     {"resolution": 0.095, "reference": 0.1405, "slope": 0.4}),
+])
+def test_energy_blur_laws(law, e0, sigma, kw):
+    win = ((0.110, 0.126), (0.126, 0.140), (0.140, 0.154), (0.154, 0.170))
+    cls = R.energy_windows(np.full(200000, e0, np.float32), 11, 0, win, law=law, **kw)
+    chi2, f, fe = _window_chi2(cls, e0, sigma, win)
+    assert chi2 < 35, (law, f, fe)
+    # a wrong width (10 % off) is rejected by the same test
+    chi2_bad, _, _ = _window_chi2(cls, e0, sigma * 1.1, win)
+    assert chi2_bad > 100, (law, chi2_bad)
 
 
 def test_accumulation_models():

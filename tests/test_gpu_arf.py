@@ -65,19 +65,25 @@ def test_pairs_bit_exact():
     assert np.all(x[:, 3:] == 0)
 
 
-def test_energy_windows_bit_exact():
+@pytest.mark.parametrize("law,kw", [
+    ("InverseSquare", {"resolution": 0.095, "reference": 0.1405}),
+    ("Gaussian", {"sigma": 0.006}),
+    ("Linear", {"resolution": 0.09, "reference": 0.1405, "slope": -0.3}),
+    ("none", {}),
+])
+def test_energy_windows_bit_exact(law, kw):
     torch = _torch()
     from opengate_amd import _abi as A
-    from opengate_amd.learn.arf import WINDOWS, energy_windows
+    from opengate_amd.learn.arf import WINDOWS, Blur, energy_windows
 
     lib = A.load_library()
     g = np.random.default_rng(4)
     e = g.uniform(0.0, 0.17, 50000).astype(np.float32)
     e[::7] = 0.0
-    print("[gpu case] gh_energy_windows 50000", flush=True)
+    print(f"[gpu case] gh_energy_windows 50000 {law}", flush=True)
     cls = torch.empty(50000, dtype=torch.int32, device="cuda")
-    energy_windows(lib, torch.from_numpy(e).cuda(), 50000, 99, 123, cls)
-    np.testing.assert_array_equal(cls.cpu().numpy(), R.energy_windows(e, 99, 123, 0.095, 0.1405, WINDOWS))
+    energy_windows(lib, torch.from_numpy(e).cuda(), 50000, 99, 123, cls, WINDOWS, Blur(law=law, **kw))
+    np.testing.assert_array_equal(cls.cpu().numpy(), R.energy_windows(e, 99, 123, WINDOWS, law=law, **kw))
 
 
 def test_accumulate_kernels():

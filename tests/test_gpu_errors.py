@@ -70,7 +70,13 @@ def test_gemm_and_kernel_argument_checks():
     assert lib.gh_mlp_forward_bf16(p(x), 64, 64, 9, 8, 256, 4, w, b, C.cast(p(out), A.FP), 64, None) != 0  # layers
     win = (C.c_float * 2)(0.15, 0.12)  # empty window
     cls = torch.zeros(4, dtype=torch.int32, device="cuda")
-    assert lib.gh_energy_windows(p(out), 4, 1, 0, 0.1, 0.14, win, 1, p(cls), None) != 0
+    blur = A.Blur(A.BLUR_LAWS["InverseSquare"], 0.0, 0.1, 0.14, 0.0)
+    assert lib.gh_energy_windows(p(out), 4, 1, 0, C.byref(blur), win, 1, p(cls), None) != 0
+    good = (C.c_float * 2)(0.12, 0.15)
+    assert lib.gh_energy_windows(p(out), 4, 1, 0, C.byref(A.Blur(7, 0.0, 0.1, 0.14, 0.0)), good, 1, p(cls),
+                                 None) != 0  # unknown law
+    assert lib.gh_energy_windows(p(out), 4, 1, 0, C.byref(A.Blur(1, -1.0, 0.0, 0.0, 0.0)), good, 1, p(cls),
+                                 None) != 0  # negative sigma
     prob = np.zeros(3, np.float32)
     alias = np.zeros(3, np.int32)
     assert lib.gh_alias_table(A.dptr(np.array([1.0, -1.0, 2.0])), 3, A.fptr(prob), alias.ctypes.data_as(C.POINTER(C.c_int32))) != 0
