@@ -14,7 +14,11 @@ import time
 
 import numpy as np
 
+from . import spect
+
 from .actors.dose import FACES, GY_FACTOR, DoseActor, PhaseSpaceActor, SimulationStatisticsActor, TLEDoseActor
+from .actors.digitizers import ARFTrainingDatasetActor
+from .geometry.spect import SPECTHeadVolume
 from .geometry.volumes import BoxVolume, ImageVolume
 from .physics import xs
 from .sources.gan import GANSource
@@ -36,7 +40,8 @@ class SimulationEngine:
 
     # ------------------------------------------------------------------ geometry
     def phantom_volume(self):
-        vols = [v for n, v in self.sim.volume_manager.volumes.items() if n != "world"]
+        vols = [v for n, v in self.sim.volume_manager.volumes.items()
+                if n != "world" and not isinstance(v, SPECTHeadVolume)]
         images = [v for v in vols if isinstance(v, ImageVolume)]
         if len(images) > 1:
             raise NotImplementedError("one ImageVolume per simulation")
@@ -202,7 +207,7 @@ class SimulationEngine:
         return spec
 
     # ------------------------------------------------------------------ tally
-    def tally_spec(self, vol, ph):
+    def tally_spec(self, vol, ph, n_total=0):
         doses = [a for a in self.sim.actors.values() if isinstance(a, DoseActor)]
         if len(doses) > 1:
             raise NotImplementedError("one DoseActor per simulation")
@@ -217,6 +222,11 @@ class SimulationEngine:
             if self.sim.world.material not in VACUUM:
                 raise NotImplementedError("exit-face recording needs a vacuum world")
             rec_face, rec_cap = FACES[r.exit_face], int(r.max_photons)
+        if self.spect is not None:
+            if recs:
+                raise NotImplementedError("a PhaseSpaceActor on the phantom together with SPECT projections")
+            # every photon leaving the phantom box goes to the heads; at most one per history
+            rec_face, rec_cap = FACES["all"], int(max(1, min(n_total, spect.RECORD_CHUNK)))
         self.dose = doses[0] if doses else None
         if self.dose is None:
             return TallySpec(size=tuple(self.size), spacing=tuple(self.spacing), edep2=False,
@@ -251,6 +261,8 @@ class SimulationEngine:
         sim.physics_manager.check_supported()
         seed = sim.resolve_seed()
         self._seed = seed
+        if any(isinstance(a, ARFTrainingDatasetActor) for a in sim.actors.values()):
+            return self._run_arf_training(seed, t_init)
         self.engine = GpuEngine(sim.device)
         vol = self.phantom_volume()
         ph, mats, names = self.build_geometry(vol)
@@ -265,7 +277,11 @@ class SimulationEngine:
         e_max = max(spec.max_energy() for _, spec, _ in runs)
         e_cut = max(float(sim.physics_manager.photon_tracking_cut), 2.0 ** -10)
         tables = xs.build_tables(mats, e_max * (1 + 1e-6), e_cut=e_cut)
-        tal = self.tally_spec(vol, ph)
+        self.spect = None
+        n_total = sum(n for _, _, n in runs)
+        if spect.spect_actors(sim):
+            self.spect = spect.SpectProjections(self, seed, e_max)
+        tal = self.tally_spec(vol, ph, n_total)
         self.engine.set_physics(tables)
         sid = self.engine.add_simulation(ph, runs[0][1], tal)
         t_init = time.time() - t_init
@@ -283,7 +299,13 @@ class SimulationEngine:
                 chunk = min(todo, 2**32 - 1)
                 if self.dose is not None and self.dose.uncertainty_goal is not None:
                     chunk = self._goal_chunk(first, todo)
+                if self.spect is not None:
+                    chunk = min(chunk, tal.record_capacity)
                 c = self.engine.run([chunk], seed, first_history=[first], **sim.launch)
+                if self.spect is not None:
+                    rec, m = self.engine.get_phsp(sid, device=True)
+                    self.spect.consume(rec, min(m, tal.record_capacity))
+                    self.engine.reset_phsp(sid)
                 self.executed.append((spec, chunk, first))
                 counters = c if counters is None else {key: counters[key] + c[key] for key in c}
                 first += chunk
@@ -295,6 +317,21 @@ class SimulationEngine:
         self.seed = seed
         self.finish(vol, ph, tal, names, tables, counters, n_events, duration, t_init)
         return counters
+
+    def _run_arf_training(self, seed, t_init):
+        """ARFTrainingDatasetActor simulation: head model only, no phantom."""
+        sim = self.sim
+        t0 = time.time()
+        n = spect.run_arf_training(sim, seed)
+        duration = time.time() - t0
+        self.seed = seed
+        for s in sim.actors.values():
+            if isinstance(s, SimulationStatisticsActor):
+                s.counts.update(runs=len(sim.run_timing_intervals), events=int(n), tracks=int(n), steps=0,
+                                duration=duration * g4_units.s, init=(t0 - t_init) * g4_units.s)
+                s.raw = {"histories": int(n)}
+                s.write_outputs()
+        return {"histories": int(n)}
 
     # uncertainty goal: GateDoseActor::EndOfEventAction / ComputeMeanUncertainty (GateDoseActor.cpp:245-330)
     def _goal_chunk(self, done, todo):
@@ -349,6 +386,8 @@ class SimulationEngine:
                 raise NotImplementedError("density output needs the dose grid of the image")
             a.store_results(size, spacing, origin, edep, edep2, dose, dose2, t.get("counts"), density, n_events)
             a.write_outputs()
+        if self.spect is not None:
+            self.spect.finish()
         for r in sim.actors.values():
             if isinstance(r, PhaseSpaceActor):
                 arrs, n = self.engine.get_phsp(0)

@@ -342,7 +342,8 @@ class ArfModel:
         torch = _torch()
         d = self.mlp.state_dict()
         torch.save({"sizes": torch.tensor(self.sizes), "W": d["W"], "b": d["b"],
-                    "theta_max": torch.tensor(self.theta_max), "windows": torch.tensor(self.windows),
+                    "theta_max": torch.tensor(self.theta_max, dtype=torch.float64),
+                    "windows": torch.tensor(self.windows, dtype=torch.float64),
                     "e_range": torch.tensor(self.e_range, dtype=torch.float64)}, path)
 
     @classmethod
@@ -450,7 +451,7 @@ def project(rec, n, heads, arf=None, head_sim=None, seed=0, chunk=1 << 22, squar
     torch = _torch()
     lib = (arf or head_sim.engine).lib
     dev = rec[0].device
-    if arf is not None and (abs(heads.theta_max - arf.theta_max) > 1e-12 or tuple(heads.e_range) != arf.e_range):
+    if arf is not None and (abs(heads.theta_max - arf.theta_max) > 1e-9 or tuple(heads.e_range) != arf.e_range):
         raise ValueError("the gantry's acceptance angle and energy range must be those the ARF network was trained "
                          f"with (theta_max {arf.theta_max}, e_range {arf.e_range})")
     if windows is None:
@@ -569,3 +570,32 @@ def compare_projections(test, ref, test2=None, ref2=None, min_counts=20.0, block
                        if mk.any() else None)
         out["chi2_per_pixel_window"] = per
     return out
+
+
+# ---------------------------------------------------------------------- training from a dataset
+def train_arf(dataset, pth_filename=None, steps=4000, batch=16384, lr=2e-3, device="cuda", seed=0, log_every=0):
+    """Train an ArfModel on an ARFTrainingDatasetActor dataset (.npz path or dict with E [MeV],
+    Theta and Phi [deg], window, channels, acceptance_angle, e_range, russian_roulette):
+    the counterpart of training a garf network on the reference's training ROOT file.
+    Undetected rows are weighted by the Russian roulette factor. Saves to pth_filename
+    when given; returns the model."""
+    torch = _torch()
+    d = dict(dataset) if isinstance(dataset, dict) else dict(np.load(dataset, allow_pickle=False))
+    windows = [tuple(float(v) for v in w) for w in np.asarray(d["channels"]).reshape(-1, 2)]
+    model = ArfModel(float(d["acceptance_angle"]), device=device, seed=seed, windows=windows,
+                     sizes=SIZES[:-1] + [len(windows) + 1], e_range=tuple(np.asarray(d["e_range"]).tolist()))
+    E = torch.as_tensor(np.asarray(d["E"], np.float32), device=model.dev)
+    th = torch.as_tensor(np.radians(np.asarray(d["Theta"], np.float64)).astype(np.float32), device=model.dev)
+    ph = torch.as_tensor(np.radians(np.asarray(d["Phi"], np.float64)).astype(np.float32), device=model.dev)
+    cls = torch.as_tensor(np.asarray(d["window"]).astype(np.int64), device=model.dev)
+    if len(E) == 0:
+        raise ValueError("empty ARF training dataset")
+    if int(cls.min()) < 0 or int(cls.max()) > len(windows):
+        raise ValueError("window ids outside 0..number of channels")
+    rr = float(d.get("russian_roulette", 1))
+    cw = [rr] + [1.0] * len(windows) if rr != 1.0 else None
+    model.fit(model.inputs(E, th, ph), cls, steps=steps, batch=min(batch, len(E)), lr=lr, seed=seed + 1,
+              log_every=log_every, class_weight=cw)
+    if pth_filename is not None:
+        model.save(pth_filename)
+    return model

@@ -3,8 +3,10 @@ add_source :2607, add_actor :2610, run :2818; PhysicsManager :695; VolumeManager
 
 Only the photon-transport-and-dose path is available: the world, Box and Image
 volumes, GenericSource / VoxelSource / PhaseSpaceSource / GANSource with gamma
-primaries, the gamma physics
-options, DoseActor, SimulationStatisticsActor and an exit-face PhaseSpaceActor.
+primaries, the gamma physics options, DoseActor, TLEDoseActor,
+SimulationStatisticsActor, an exit-face PhaseSpaceActor, and SPECT detection (a
+SPECTHeadVolume gantry with the digitizer chain ending in DigitizerProjectionActor,
+ARFActor, ARFTrainingDatasetActor).
 Anything else raises NotImplementedError instead of silently doing something else.
 """
 import os
@@ -12,8 +14,11 @@ from pathlib import Path
 
 import numpy as np
 
+from .actors.digitizers import (ARFActor, ARFTrainingDatasetActor, DigitizerAdderActor, DigitizerBlurringActor,
+                                DigitizerEnergyWindowsActor, DigitizerHitsCollectionActor, DigitizerProjectionActor)
 from .actors.dose import DoseActor, PhaseSpaceActor, SimulationStatisticsActor, TLEDoseActor
 from .base import Box, GateObject
+from .geometry.spect import SPECTHeadVolume
 from .geometry.volumes import BoxVolume, ImageVolume
 from .physics.materials import MaterialDatabase
 from .sources.gan import GANSource
@@ -83,9 +88,11 @@ class VolumeManager:
                 v = BoxVolume(name)
             elif t == "image":
                 v = ImageVolume(name)
+            elif t == "specthead":
+                v = SPECTHeadVolume(name)
             else:
                 raise NotImplementedError(f"volume type '{volume_type}' is not part of the photon dose path "
-                                          "(Box and Image volumes are)")
+                                          "(Box, Image and SPECTHead volumes are)")
         if v.name in self.volumes:
             raise ValueError(f"a volume named '{v.name}' already exists")
         v.volume_manager = self
@@ -169,7 +176,12 @@ class Simulation:
         if name in self.actors:
             raise ValueError(f"an actor named '{name}' already exists")
         table = {"DoseActor": DoseActor, "TLEDoseActor": TLEDoseActor,
-                 "SimulationStatisticsActor": SimulationStatisticsActor, "PhaseSpaceActor": PhaseSpaceActor}
+                 "SimulationStatisticsActor": SimulationStatisticsActor, "PhaseSpaceActor": PhaseSpaceActor,
+                 "DigitizerHitsCollectionActor": DigitizerHitsCollectionActor,
+                 "DigitizerAdderActor": DigitizerAdderActor, "DigitizerBlurringActor": DigitizerBlurringActor,
+                 "DigitizerEnergyWindowsActor": DigitizerEnergyWindowsActor,
+                 "DigitizerProjectionActor": DigitizerProjectionActor, "ARFActor": ARFActor,
+                 "ARFTrainingDatasetActor": ARFTrainingDatasetActor}
         if actor_type not in table:
             raise NotImplementedError(f"actor type '{actor_type}' is not part of the photon dose path")
         a = table[actor_type](name)

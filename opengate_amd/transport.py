@@ -397,6 +397,10 @@ class GpuEngine:
         self._check(self.lib.gh_get_phsp(self.ctx, int(sim_id), (A.FP * 8)(*ptrs), m, C.byref(n)), "gh_get_phsp")
         return arrs, int(n.value)
 
+    def reset_phsp(self, sim_id=0):
+        """Empty the exit-face recorder (tallies are kept)."""
+        self._check(self.lib.gh_reset_phsp(self.ctx, int(sim_id)), "gh_reset_phsp")
+
     def get_event_edep(self, sim_id, n, device=False):
         """Energy deposited in the detector labels by each of the first n histories of the
         last run (numpy, or a device tensor with device=True)."""
