@@ -51,6 +51,26 @@ def insert_suffix_before_extension(path, suffix):
     return p.with_name(name + "_" + suffix)
 
 
+def sample_count_path(path):
+    """Sidecar of an image output holding its number of samples: x_edep.mhd ->
+    x_edep_samples.mhd.json (dataitems.py:235-245), read back when job outputs are merged."""
+    p = insert_suffix_before_extension(Path(path), "samples")
+    return p.with_suffix(Path(path).suffix + ".json")
+
+
+def write_sample_count(path, n):
+    """ItkImageDataItem.write sidecar (dataitems.py:1336-1340)."""
+    sample_count_path(path).write_text(json.dumps({"number_of_samples": int(n)}))
+
+
+def read_sample_count(path):
+    """number_of_samples of an image output, or None without a sidecar (dataitems.py:1342-1354)."""
+    p = sample_count_path(path)
+    if not p.exists():
+        return None
+    return int(json.loads(p.read_text())["number_of_samples"])
+
+
 class OutputItem:
     """User interface of one actor output (UserInterfaceToActorOutputImage, actoroutput.py:276)."""
 
@@ -208,6 +228,7 @@ class DoseActor(ActorBase):
                 p = item.get_output_path()
                 p.parent.mkdir(parents=True, exist_ok=True)
                 write_image(item._data, p)
+                write_sample_count(p, self.number_of_samples)
 
     def __str__(self):
         return f"DoseActor {self.name}: {self.number_of_samples} events"

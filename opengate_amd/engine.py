@@ -287,7 +287,12 @@ class SimulationEngine:
         t_init = time.time() - t_init
 
         counters = None
-        first = 0
+        # history-id offset of this job: jobs with one random_seed and disjoint ranges of
+        # history ids draw the streams of one long run
+        first0 = int(getattr(sim, "first_history", 0))
+        if first0 < 0:
+            raise ValueError("first_history must be >= 0")
+        first = first0
         self.executed = []  # (SourceSpec, histories, first history) in launch order
         self.inputs = (tables, ph, tal)
         t0 = time.time()
@@ -298,7 +303,7 @@ class SimulationEngine:
             while todo > 0:
                 chunk = min(todo, 2**32 - 1)
                 if self.dose is not None and self.dose.uncertainty_goal is not None:
-                    chunk = self._goal_chunk(first, todo)
+                    chunk = self._goal_chunk(first - first0, todo)
                 if self.spect is not None:
                     chunk = min(chunk, tal.record_capacity)
                 c = self.engine.run([chunk], seed, first_history=[first], **sim.launch)
@@ -310,10 +315,11 @@ class SimulationEngine:
                 counters = c if counters is None else {key: counters[key] + c[key] for key in c}
                 first += chunk
                 todo -= chunk
-                if self.dose is not None and self.dose.uncertainty_goal is not None and self._goal_reached(first):
+                if (self.dose is not None and self.dose.uncertainty_goal is not None
+                        and self._goal_reached(first - first0)):
                     todo = 0
         duration = time.time() - t0
-        n_events = first
+        n_events = first - first0
         self.seed = seed
         self.finish(vol, ph, tal, names, tables, counters, n_events, duration, t_init)
         return counters

@@ -137,6 +137,8 @@ class Simulation:
         self.g4_verbose = False
         self.check_volumes_overlap = False
         self.device = 0
+        # first history id of this run (a job of a split campaign; see opengate_amd.jobs)
+        self.first_history = 0
         self.launch = {}
         self.warnings = []
         self.last_run = None

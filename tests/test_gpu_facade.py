@@ -148,3 +148,31 @@ def test_tle_dose_actor_script_matches_oracle(tmp_path):
     assert edep.sum() > 0
     assert np.max(np.abs(tle.edep.image.array - edep)) <= 1e-9 * edep.max()
     assert (tmp_path / "tle_edep_uncertainty.mhd").exists()
+
+
+def test_split_jobs_merge_to_the_single_run(tmp_path):
+    """Two jobs with one random_seed and disjoint history ranges (first_history), merged from
+    their files, give the single run's edep, edep_squared and uncertainty (test041-mt
+    pattern: N-way split == 1-way)."""
+    from opengate_amd import jobs as J
+
+    n_total = 200_000
+    outs = []
+    for j, (first, n) in enumerate([(0, n_total)] + J.split_histories(n_total, 2)):
+        d = tmp_path / f"run{j}"
+        sim, dose, stats = build_sim(d, seed=99)
+        sim.sources["tc99m"].n = n
+        sim.first_history = first
+        dose.edep_squared.active = True
+        print(f"[gpu case] job split {j}: first {first} n {n}", flush=True)
+        sim.run()
+        assert dose.number_of_samples == n
+        outs.append(d)
+    merged, n = J.merge_dose_outputs(outs[1:], "out.mhd", tmp_path / "merged")
+    whole = gate.read_image(outs[0] / "out_edep.mhd").array
+    whole2 = gate.read_image(outs[0] / "out_edep_squared.mhd").array
+    assert n == n_total
+    assert np.max(np.abs(merged["edep"].array - whole)) <= 1e-9 * whole.max()
+    assert np.max(np.abs(merged["edep_squared"].array - whole2)) <= 1e-9 * whole2.max()
+    np.testing.assert_allclose(merged["edep_uncertainty"].array,
+                               gate.read_image(outs[0] / "out_edep_uncertainty.mhd").array, rtol=1e-6, atol=1e-12)
