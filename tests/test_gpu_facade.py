@@ -176,3 +176,53 @@ def test_split_jobs_merge_to_the_single_run(tmp_path):
     assert np.max(np.abs(merged["edep_squared"].array - whole2)) <= 1e-9 * whole2.max()
     np.testing.assert_allclose(merged["edep_uncertainty"].array,
                                gate.read_image(outs[0] / "out_edep_uncertainty.mhd").array, rtol=1e-6, atol=1e-12)
+
+
+def test_phase_space_actor_file_feeds_phase_space_source(tmp_path):
+    """PhaseSpaceActor writes the photons leaving the +z face (reference branch names);
+    a second simulation reads the file with PhaseSpaceSource and transports every photon:
+    one history per recorded photon, and its energy bookkeeping closes on the recorded
+    sum of E x weight."""
+    u = gate.g4_units
+    sim = gate.Simulation()
+    sim.output_dir = str(tmp_path)
+    sim.random_seed = 3
+    sim.world.material = "G4_Galactic"
+    box = sim.add_volume("Box", "water")
+    box.size = [60 * u.mm, 60 * u.mm, 60 * u.mm]
+    box.material = "G4_WATER"
+    s = sim.add_source("GenericSource", "tc")
+    s.energy.mono = 140.5 * u.keV
+    s.n = 200_000
+    ps = sim.add_actor("PhaseSpaceActor", "exit")
+    ps.attached_to = "water"
+    ps.exit_face = "+z"
+    ps.output_filename = "exit.npz"
+    print("[gpu case] PhaseSpaceActor record", flush=True)
+    sim.run()
+    d = dict(np.load(tmp_path / "exit.npz"))
+    n = len(d["KineticEnergy"])
+    assert n == ps.number_of_recorded > 1000
+    np.testing.assert_allclose(d["PrePositionLocal_Z"], 30.0, atol=1e-4)
+    assert np.all(d["PreDirectionLocal_Z"] > 0)
+    e_in = float(np.sum(d["KineticEnergy"].astype(np.float64) * d["Weight"].astype(np.float64)))
+
+    sim2 = gate.Simulation()
+    sim2.output_dir = str(tmp_path)
+    sim2.random_seed = 4
+    sim2.world.material = "G4_Galactic"
+    b2 = sim2.add_volume("Box", "slab")
+    b2.size = [200 * u.mm, 200 * u.mm, 20 * u.mm]
+    b2.translation = [0, 0, 50 * u.mm]
+    b2.material = "G4_WATER"
+    src = sim2.add_source("PhaseSpaceSource", "phsp")
+    src.phsp_file = str(tmp_path / "exit.npz")
+    src.global_flag = True  # positions were recorded in a box centred on the world origin
+    src.n = n
+    stats = sim2.add_actor("SimulationStatisticsActor", "stats")
+    print("[gpu case] PhaseSpaceSource replay", flush=True)
+    c = sim2.run()
+    assert stats.counts["events"] == n
+    e_out = c["e_deposited"] + c["e_outside_tally"] + c["e_escaped"] + c["e_world"]
+    assert e_out == pytest.approx(e_in, rel=1e-6)
+    assert c["e_deposited"] > 0.01 * e_in
