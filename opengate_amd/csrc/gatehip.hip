@@ -188,9 +188,9 @@ struct SqCache {
   int n;
 };
 
-// Counters: steps, rejections and deposited energy stay in lane registers; the rare
-// event counters are summed per workgroup in LDS with one atomic per wave event
-// (ballot + popcount), and flushed to global memory once at kernel end.
+// Counters: steps and rejections stay in lane registers; the rare event counters are
+// summed per workgroup in LDS with one atomic per wave event (ballot + popcount); the
+// energy sums go to a per-lane LDS slot; all are flushed to global memory at kernel end.
 enum { C_HIST = 0, C_STEPS, C_DEPOSITS, C_COMPTON, C_RAYLEIGH, C_PHOTO, C_ESCAPED, C_WORLD, C_CUT,
        C_EVICT, C_RECORDED, C_REJ, N_COUNTERS };
 enum { E_DEP = 0, E_OUT, E_ESC, E_WORLD, N_ENERGIES };
@@ -198,13 +198,14 @@ enum { E_DEP = 0, E_OUT, E_ESC, E_WORLD, N_ENERGIES };
 struct LaneStats {
   uint32_t steps, rej;
   unsigned long long *cnt;  // LDS [N_COUNTERS]
-  double *en;               // LDS [N_ENERGIES]
+  double *en;               // LDS [N_ENERGIES][BLOCK]: this lane's column
+  int stride;               // BLOCK
   __device__ __forceinline__ void inc(int k) {
     const uint64_t m = __ballot(1);
     if (__lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1))
       atomicAdd(cnt + k, (unsigned long long)__popcll(m));
   }
-  __device__ __forceinline__ void add_e(int k, double v) { unsafeAtomicAdd(en + k, v); }
+  __device__ __forceinline__ void add_e(int k, double v) { unsafeAtomicAdd(en + k * stride, v); }
 };
 
 __device__ __forceinline__ void flush_slot(const SimDev &s, int vox, float sum, float dsum, uint32_t cnt,
@@ -474,12 +475,15 @@ transport_kernel(Launch L) {
   const float *mu_tab = T.mu;
   const float *maj_tab = T.maj;
   __shared__ unsigned long long s_cnt[N_COUNTERS];
-  __shared__ double s_en[N_ENERGIES];
+  // energy tallies of the statistics: one FP64 slot per lane and category, so that the
+  // lanes of a wave never add to one LDS address at once (64 same-address FP64 atomics
+  // serialise in the LDS unit and hold up the table reads of every wave of the CU)
+  __shared__ double s_en[N_ENERGIES * BLOCK];
   if (threadIdx.x < N_COUNTERS) s_cnt[threadIdx.x] = 0ull;
-  if (threadIdx.x < N_ENERGIES) s_en[threadIdx.x] = 0.0;
+  for (int k = 0; k < N_ENERGIES; ++k) s_en[k * BLOCK + threadIdx.x] = 0.0;
   __syncthreads();
   LaneStats st;
-  st.steps = 0u; st.rej = 0u; st.cnt = s_cnt; st.en = s_en;
+  st.steps = 0u; st.rej = 0u; st.cnt = s_cnt; st.en = s_en + threadIdx.x; st.stride = BLOCK;
 
   Hot hot;
   if (!BATCHED) hot.load(L.sims[0]);
@@ -779,8 +783,10 @@ transport_kernel(Launch L) {
   __syncthreads();
   if (threadIdx.x < N_COUNTERS && s_cnt[threadIdx.x])
     atomicAdd(L.counters + threadIdx.x, s_cnt[threadIdx.x]);
-  if (threadIdx.x < N_ENERGIES && s_en[threadIdx.x] != 0.0)
-    unsafeAtomicAdd(L.energies + threadIdx.x, s_en[threadIdx.x]);
+  for (int k = 0; k < N_ENERGIES; ++k) {
+    const double v = wave_sum<double>(s_en[k * BLOCK + threadIdx.x]);
+    if (lane_id() == 0 && v != 0.0) unsafeAtomicAdd(L.energies + k, v);
+  }
 }
 
 // ---------------------------------------------------------------------- phsp check
