@@ -123,10 +123,20 @@ struct Hot {
   }
 };
 
-// slab test of one axis of the box [0, L]
-__device__ __forceinline__ void slab(float p, float d, float L, int axis, float &tnear, float &tfar, int &face) {
+// reciprocal direction: the slab tests of a photon reuse one set of divisions
+struct InvDir {
+  float x, y, z;
+};
+__device__ __forceinline__ InvDir inv_dir(float dx, float dy, float dz) {
+  InvDir r;
+  r.x = 1.0f / dx; r.y = 1.0f / dy; r.z = 1.0f / dz;
+  return r;
+}
+
+// slab test of one axis of the box [0, L]; inv = 1/d (used only when d != 0)
+__device__ __forceinline__ void slab(float p, float d, float inv, float L, int axis, float &tnear, float &tfar,
+                                     int &face) {
   if (d != 0.0f) {
-    const float inv = 1.0f / d;
     const float t0 = (-p) * inv;
     const float t1 = (L - p) * inv;
     const float tmn = fminf(t0, t1), tmx = fmaxf(t0, t1);
@@ -139,12 +149,12 @@ __device__ __forceinline__ void slab(float p, float d, float L, int axis, float 
 
 // ray vs box [0,L]: entry/exit distance and exit face (0:-x 1:+x 2:-y 3:+y 4:-z 5:+z)
 __device__ __forceinline__ void ray_box(float px, float py, float pz, float dx, float dy, float dz,
-                                        float Lx, float Ly, float Lz, float &tnear, float &tfar,
-                                        int &face) {
+                                        const InvDir &iv, float Lx, float Ly, float Lz, float &tnear,
+                                        float &tfar, int &face) {
   tnear = -INFINITY; tfar = INFINITY; face = -1;
-  slab(px, dx, Lx, 0, tnear, tfar, face);
-  slab(py, dy, Ly, 1, tnear, tfar, face);
-  slab(pz, dz, Lz, 2, tnear, tfar, face);
+  slab(px, dx, iv.x, Lx, 0, tnear, tfar, face);
+  slab(py, dy, iv.y, Ly, 1, tnear, tfar, face);
+  slab(pz, dz, iv.z, Lz, 2, tnear, tfar, face);
 }
 
 __device__ __forceinline__ float world_axis(float p, float d, float inv, float lo, float hi, float t) {
@@ -153,13 +163,13 @@ __device__ __forceinline__ float world_axis(float p, float d, float inv, float l
   return t;
 }
 
-// distance to the world box boundary from inside (iu, iv, iw = 1/direction)
+// distance to the world box boundary from inside (iv = 1/direction)
 __device__ __forceinline__ float world_exit(const SimDev &s, float px, float py, float pz, float dx,
-                                            float dy, float dz) {
+                                            float dy, float dz, const InvDir &iv) {
   float t = INFINITY;
-  t = world_axis(px, dx, 1.0f / dx, s.wlo[0], s.whi[0], t);
-  t = world_axis(py, dy, 1.0f / dy, s.wlo[1], s.whi[1], t);
-  t = world_axis(pz, dz, 1.0f / dz, s.wlo[2], s.whi[2], t);
+  t = world_axis(px, dx, iv.x, s.wlo[0], s.whi[0], t);
+  t = world_axis(py, dy, iv.y, s.wlo[1], s.whi[1], t);
+  t = world_axis(pz, dz, iv.z, s.wlo[2], s.whi[2], t);
   return fmaxf(t, 0.0f);
 }
 
@@ -440,11 +450,12 @@ __host__ __device__ inline LdsLayout lds_layout(int n_mat, int nb1, int nx, int 
 // track-length estimate w E mu_en L; the energy such a photon gives to electrons at real
 // interactions is not deposited. Photons at or below tle_emin (and below the tracking
 // cut) deposit their energy where they are and stop. Single-simulation launches only.
-// The batched form keeps a few more per-lane values (its simulation's grid); the
-// waves-per-EU bound holds it at 128 VGPRs (4 waves per SIMD), which it meets without
-// spilling, like the single-simulation forms do by themselves.
+// The batched form keeps a few more per-lane values (its simulation's grid) and the
+// kerma-estimator form a few more live values in the step loop; the waves-per-EU bound
+// holds both at 128 VGPRs (4 waves per SIMD), which they meet without spilling, as the
+// analog single-simulation forms do by themselves.
 template <bool BATCHED, int STAGE, int BLOCK, bool TLE>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BATCHED ? 4 : 1)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((BATCHED || TLE) ? 4 : 1)))
 transport_kernel(Launch L) {
   extern __shared__ float lds[];
   const Phys &P = L.P;
@@ -514,11 +525,15 @@ transport_kernel(Launch L) {
   // refill for photons born outside the box (all new lanes together) and in the slow
   // phase for photons that left it. The event stream of the history supplies the
   // interaction distance either way, so where it runs does not change the result.
-  auto world_step = [&](const SimDev &s, bool &kill) {
+  // The single-simulation forms compute the reciprocal direction once per slow event and
+  // share it between slab tests; the batched form, which has no registers to spare,
+  // recomputes the same values at each use.
+  auto recip = [&](const InvDir &iv) { return BATCHED ? inv_dir(ph.u, ph.v, ph.w) : iv; };
+  auto world_step = [&](const SimDev &s, bool &kill, const InvDir &iv) {
         // world outside the phantom box (homogeneous material or vacuum)
         float tn, tf;
         int face;
-        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, s.L[0], s.L[1], s.L[2], tn, tf, face);
+        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, recip(iv), s.L[0], s.L[1], s.L[2], tn, tf, face);
         const bool hit = (tn < tf) && (tf > EXIT_EPS);
         const float tin = fmaxf(tn, 0.0f);
         if (s.world_mat < 0) {
@@ -533,7 +548,7 @@ transport_kernel(Launch L) {
         } else {
           const int wm = s.world_mat;
           const float muw = lerp_node(mu_tab + wm * NB1, ec);
-          const float tw = world_exit(s, ph.x, ph.y, ph.z, ph.u, ph.v, ph.w);
+          const float tw = world_exit(s, ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, recip(iv));
           const float d = (muw > 0.0f) ? (-glog(u01(rng)) / muw) : INFINITY;
           if (hit && d >= tin && tin < tw) {
             ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
@@ -609,7 +624,7 @@ transport_kernel(Launch L) {
                 mode = inside ? M_STEP : M_WORLD;
                 if (mode == M_WORLD) {
                   bool kill = false;
-                  world_step(s, kill);
+                  world_step(s, kill, BATCHED ? InvDir{0.0f, 0.0f, 0.0f} : inv_dir(ph.u, ph.v, ph.w));
                   if (kill) {
                     if (s.event_edep) s.event_edep[ph.hist] = 0.0f;
                     mode = M_DEAD;
@@ -674,11 +689,14 @@ transport_kernel(Launch L) {
     if (mode == M_COLLIDE || mode == M_EXIT || mode == M_WORLD) {
       const SimDev &s = L.sims[BATCHED ? (int)rng.c2 : 0];
       bool kill = false;
+      // one set of divisions for the box exit and the world transport that follows it
+      InvDir iv = {0.0f, 0.0f, 0.0f};
+      if (!BATCHED && mode != M_COLLIDE) iv = inv_dir(ph.u, ph.v, ph.w);
       if (mode == M_EXIT) {
         // leaves the phantom box: exit point from the last position inside
         float tn, tf;
         int face;
-        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, s.L[0], s.L[1], s.L[2], tn, tf, face);
+        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, recip(iv), s.L[0], s.L[1], s.L[2], tn, tf, face);
         const float t = fmaxf(tf, 0.0f);
         const float ex = fmaf(t, ph.u, ph.x), ey = fmaf(t, ph.v, ph.y), ez = fmaf(t, ph.w, ph.z);
         if (s.rec_face >= 0 && (face == s.rec_face || s.rec_face == GH_FACE_ALL)) {
@@ -705,7 +723,7 @@ transport_kernel(Launch L) {
           mode = M_WORLD;
         }
       }
-      if (mode == M_WORLD && !kill) world_step(s, kill);
+      if (mode == M_WORLD && !kill) world_step(s, kill, iv);
       if (mode == M_COLLIDE && !kill) {
         const bool in_world = cvox < 0;
         // the label is read again rather than kept in a register from the step phase
