@@ -50,6 +50,7 @@ struct SimDev {
   float mom[3], focus[3];
   float e_mono, e_max, weight;
   const float *e_val, *e_width, *e_cdf;
+  const int32_t *e_guide;  // [SPEC_GUIDE+1] guide of e_cdf
   int64_t phsp_n, phsp_start;
   const float *phsp[8];
   int64_t vox_n;
@@ -365,10 +366,10 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
     }
     // ---- energy (GateSPSEneDistribution.cpp:16-159)
     if (s.ene_type == GH_ENE_LINES) {
-      const int k = cdf_index(s.e_cdf, s.n_energy, u01(g));
+      const int k = cdf_index_guided(s.e_cdf, s.e_guide, u01(g));
       E = s.e_val[k];
     } else if (s.ene_type == GH_ENE_HISTOGRAM) {
-      const int k = cdf_index(s.e_cdf, s.n_energy, u01(g));
+      const int k = cdf_index_guided(s.e_cdf, s.e_guide, u01(g));
       E = fmaf(u01(g), s.e_width[k], s.e_val[k]);
     } else if (s.ene_type == GH_ENE_RANGE) {
       E = fmaf(u01(g), s.e_max - s.e_mono, s.e_mono);
@@ -462,6 +463,7 @@ transport_kernel(Launch L) {
   const int NB1 = P.nb + 1;
   Tabs T;
   T.mu = P.mu; T.maj = P.maj; T.pcpr = P.pcpr; T.rx = P.rx; T.rcdf = P.rcdf; T.sinc = P.sinc;
+  T.rguide = P.rguide;
   if (STAGE >= 1) {
     const LdsLayout o = lds_layout(P.n_mat, NB1, P.nx, STAGE);
     const int n_mu = P.n_mat * NB1;
@@ -1004,6 +1006,7 @@ int validate_source(gh_ctx *ctx, const gh_source_desc *src) {
 int upload_source(gh_ctx *ctx, const gh_source_desc *src, SimDev &d, std::vector<void *> &owned) {
   int rc;
   d.e_val = d.e_width = d.e_cdf = nullptr;
+  d.e_guide = nullptr;
   d.phsp_n = 0;
   d.phsp_start = 0;
   for (int k = 0; k < 8; ++k) d.phsp[k] = nullptr;
@@ -1027,6 +1030,17 @@ int upload_source(gh_ctx *ctx, const gh_source_desc *src, SimDev &d, std::vector
     d.e_val = t;
     if ((rc = dev_upload(ctx, src->energy_cdf, src->n_energy, &t, owned))) return rc;
     d.e_cdf = t;
+    // guide[b] = smallest k with cdf[k] > b / SPEC_GUIDE (n - 1 at the top edge)
+    std::vector<int32_t> guide(SPEC_GUIDE + 1);
+    for (int b = 0; b <= SPEC_GUIDE; ++b) {
+      const float edge = (float)b / (float)SPEC_GUIDE;
+      int k = 0;
+      while (k < src->n_energy - 1 && !(src->energy_cdf[k] > edge)) ++k;
+      guide[b] = k;
+    }
+    int32_t *tg;
+    if ((rc = dev_upload(ctx, (const int32_t *)guide.data(), guide.size(), &tg, owned))) return rc;
+    d.e_guide = tg;
     if (src->ene_type == GH_ENE_HISTOGRAM) {
       if ((rc = dev_upload(ctx, src->energy_width, src->n_energy, &t, owned))) return rc;
       d.e_width = t;
@@ -1174,6 +1188,22 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
   if ((rc = dev_upload(ctx, muen, M * NB1, &t, ctx->phys_owned))) return rc; P.muen = t;
   if ((rc = dev_upload(ctx, ph->ray_x, NX, &t, ctx->phys_owned))) return rc; P.rx = t;
   if ((rc = dev_upload(ctx, ph->ray_cdf, M * NX, &t, ctx->phys_owned))) return rc; P.rcdf = t;
+  {
+    // guide[m][b] = largest i <= NX-2 with ray_cdf[m][i] <= b / RAY_GUIDE
+    std::vector<int32_t> guide(M * (RAY_GUIDE + 1));
+    for (size_t m = 0; m < M; ++m) {
+      const float *G = ph->ray_cdf + m * NX;
+      int i = 0;
+      for (int b = 0; b <= RAY_GUIDE; ++b) {
+        const float edge = (float)b / (float)RAY_GUIDE;
+        while (i + 1 <= (int)NX - 2 && G[i + 1] <= edge) ++i;
+        guide[m * (RAY_GUIDE + 1) + b] = i;
+      }
+    }
+    int32_t *tg;
+    if ((rc = dev_upload(ctx, (const int32_t *)guide.data(), guide.size(), &tg, ctx->phys_owned))) return rc;
+    P.rguide = tg;
+  }
   if ((rc = dev_upload(ctx, ph->incoh_s, M * NX, &t, ctx->phys_owned))) return rc; P.sinc = t;
   if ((rc = dev_upload(ctx, inv_rho.data(), M, &t, ctx->phys_owned))) return rc; P.inv_rho = t;
   P.n_mat = (int)M;

@@ -169,6 +169,44 @@ GH_DEV int bsearch_le(const float *a, int n, float x) {
   return lo;
 }
 
+// Guide tables (C. Chen and H. Asau, "On generating random variates from an empirical
+// distribution", AIIE Transactions 6 (1974) 163-166): for a non-decreasing table and
+// 2^k equal bins of [0, 1), guide[b] is the search result at the bin's lower edge, so a
+// search for a value in bin b only has to look between guide[b] and guide[b + 1] and
+// returns exactly what the search over the whole table returns.
+constexpr int SPEC_GUIDE = 256;    // bins of the source-spectrum CDF guide
+constexpr int RAY_GUIDE = 1024;    // bins of the Rayleigh int F^2 guide (per material)
+
+// largest i in [lo, hi-1] with a[i] <= x, given a[lo] <= x and (a[hi] > x or hi = n-1)
+GH_DEV int bsearch_le_range(const float *a, int lo, int hi, float x) {
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// bsearch_le(a, n, x) for x >= 0 with guide[b] = bsearch_le(a, n, b / RAY_GUIDE)
+GH_DEV int bsearch_le_guided(const float *a, const int32_t *guide, int n, float x) {
+  const int b = min((int)(x * (float)RAY_GUIDE), RAY_GUIDE - 1);
+  return bsearch_le_range(a, guide[b], min(guide[b + 1] + 1, n - 1), x);
+}
+
+// smallest k in [lo, hi] with cdf[k] > u, given cdf[lo-1] <= u (or lo = 0) and cdf[hi] > u
+GH_DEV int cdf_index_range(const float *cdf, int lo, int hi, float u) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// cdf_index(cdf, n, u) for u in (0, 1) with guide[b] = cdf_index(cdf, n, b / SPEC_GUIDE)
+GH_DEV int cdf_index_guided(const float *cdf, const int32_t *guide, float u) {
+  const int b = (int)(u * (float)SPEC_GUIDE);
+  return cdf_index_range(cdf, guide[b], guide[b + 1], u);
+}
+
 // smallest k in [0, n-1] with cdf[k] > u (cdf ascending, cdf[n-1] = 1 > u)
 GH_DEV int cdf_index(const float *cdf, int n, float u) {
   int lo = 0, hi = n - 1;
@@ -221,6 +259,7 @@ GH_DEV float qgrid_value(const float *ys, int n, float x) {
 
 struct Phys {
   const float *mu;       // [M][NB+1] (global copy)
+  const int32_t *rguide; // [M][RAY_GUIDE+1] guide of rcdf
   const float *maj;      // [NB+1]
   const float *pcpr;     // [M][NB+1][2] (sigma_C/mu, (sigma_C+sigma_R)/mu) interleaved
   const float *muen;     // [M][NB+1]
@@ -235,6 +274,7 @@ struct Phys {
 // Table pointers a kernel reads through (LDS copies or the global arrays)
 struct Tabs {
   const float *mu, *maj, *pcpr, *rx, *rcdf, *sinc;
+  const int32_t *rguide;
 };
 
 // branching ratios at a collision: (sigma_C/mu, (sigma_C+sigma_R)/mu) interpolated at E
@@ -293,7 +333,7 @@ GH_DEV float sample_rayleigh(Rng &g, const Tabs &T, int nx, int lab, float E) {
   float c = 1.0f;
   for (int it = 0; it < MAX_REJECT; ++it) {
     const float t = u01(g) * gmax;
-    const int j = bsearch_le(G, nx, t);
+    const int j = bsearch_le_guided(G, T.rguide + lab * (RAY_GUIDE + 1), nx, t);
     const float g0 = G[j], g1 = G[j + 1];
     const float dg = g1 - g0;
     const float x0 = T.rx[j], x1 = T.rx[j + 1];
