@@ -21,6 +21,7 @@
 //   GH_EPI_DSILU      G32 = acc * silu'(Zprev), Y16 = bf16(G32)     (backward through
 //                     the activation of the previous layer)
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
@@ -262,9 +263,14 @@ extern "C" int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void 
 // lane's MFMA B fragment (column n, 8 consecutive k) is one 16-byte load from L2-resident
 // memory. A workgroup of 4 waves owns 16 RT rows; in a hidden layer wave w computes the
 // columns [w H/4, (w+1) H/4) for all rows (RT row tiles x H/64 column tiles of 16x16),
-// reading its A fragments (row, 8 consecutive k) from the LDS activation tile; the B
-// fragments are loaded PF 32-wide k steps ahead of the MFMAs that use them. The output
-// layer (16 columns) spreads its row tiles over the 4 waves.
+// reading its A fragments (row, 8 consecutive k) from the LDS activation tile. Both
+// operands of k step ks+1 are loaded while the MFMAs of step ks run, into a second set of
+// registers (k steps go in pairs, so no register is copied): a rotation by register
+// copies would make the compiler wait for the next step's loads before this step's MFMAs.
+// One LDS activation buffer serves as input and output of a layer: the accumulators hold
+// the whole output tile, so after a barrier the epilogue overwrites the input in place,
+// which halves the LDS per workgroup (and lets 112-row tiles run at 2 waves per SIMD).
+// The output layer (16 columns) spreads its row tiles over the 4 waves.
 // The per-element arithmetic (MFMA over k slabs of 32 in order, bias, SiLU, bf16 rounding)
 // is the same as the layer-by-layer GEMM path, so both give bit-identical outputs.
 namespace {
@@ -272,11 +278,15 @@ namespace {
 constexpr int MAXL = 8;          // layers
 constexpr int K0 = 32;           // first-layer input width (padded)
 constexpr int NOUT = 16;         // last-layer output width (padded)
-#ifndef GH_MLP_RT
-#define GH_MLP_RT 4              // 16-row tiles per workgroup (64 rows)
+// 16-row tiles per workgroup, per hidden width (measured best on MI355X)
+#ifndef GH_MLP_RT256
+#define GH_MLP_RT256 7           // 112 rows
 #endif
-#ifndef GH_MLP_PF
-#define GH_MLP_PF 1              // B-fragment prefetch distance in k steps
+#ifndef GH_MLP_RT128
+#define GH_MLP_RT128 8           // 128 rows
+#endif
+#ifndef GH_MLP_SB
+#define GH_MLP_SB 1              // one activation buffer, overwritten after a barrier
 #endif
 
 struct MlpArgs {
@@ -287,10 +297,10 @@ struct MlpArgs {
   int n_out;                     // valid output columns (<= NOUT)
 };
 
-template <int H, int RT>
-constexpr size_t mlp_lds_bytes() { return 2ull * (16 * RT) * (H + 8) * sizeof(__bf16); }
+template <int H, int RT, int SB>
+constexpr size_t mlp_lds_bytes() { return (SB ? 1ull : 2ull) * (16 * RT) * (H + 8) * sizeof(__bf16); }
 
-template <int H, int RT, int PF>
+template <int H, int RT, int SB>
 __global__ void __launch_bounds__(256) mlp_forward_kernel(const __bf16 *__restrict__ X, int64_t ldx, int64_t M,
                                                           MlpArgs a, float *__restrict__ Z, int64_t ldz) {
   constexpr int P = H + 8;                 // LDS row pitch (bf16)
@@ -298,7 +308,7 @@ __global__ void __launch_bounds__(256) mlp_forward_kernel(const __bf16 *__restri
   constexpr int ROWS = 16 * RT;
   extern __shared__ __align__(16) unsigned char mlp_lds[];
   __bf16 *act0 = (__bf16 *)mlp_lds;
-  __bf16 *act1 = act0 + ROWS * P;
+  __bf16 *act1 = SB ? act0 : act0 + ROWS * P;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t row0 = (int64_t)blockIdx.x * ROWS;
   const int fr = lane & 15, fk = (lane >> 4) * 8;
@@ -325,44 +335,53 @@ __global__ void __launch_bounds__(256) mlp_forward_kernel(const __bf16 *__restri
         for (int j = 0; j < CT; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
       const int n0 = wave * (H / 4);
       const int nk = K / 32;
-      // B fragments of k steps ks (b0), ks+1 (b1) and, for PF = 2, ks+2 (b2), rotated by
-      // register moves so that every array index is static
       const __bf16 *wrow[CT];
 #pragma unroll
       for (int j = 0; j < CT; ++j) wrow[j] = W + (int64_t)(n0 + j * 16 + fr) * K + fk;
-      bf16x8 b0[CT], b1[CT], b2[CT];
+      const __bf16 *arow = in + fr * P + fk;
+      bf16x8 bA[CT], bB[CT], aA[RT], aB[RT];
+      auto load = [&](bf16x8 *b, bf16x8 *af, int ks) {
 #pragma unroll
-      for (int j = 0; j < CT; ++j) {
-        b0[j] = *(const bf16x8 *)(wrow[j]);
-        if (PF >= 2 && nk > 1) b1[j] = *(const bf16x8 *)(wrow[j] + 32);
-      }
-      for (int ks = 0; ks < nk; ++ks) {
-        if (PF >= 2) {
-          if (ks + 2 < nk) {
+        for (int j = 0; j < CT; ++j) b[j] = *(const bf16x8 *)(wrow[j] + ks * 32);
 #pragma unroll
-            for (int j = 0; j < CT; ++j) b2[j] = *(const bf16x8 *)(wrow[j] + (ks + 2) * 32);
-          }
-        } else if (ks + 1 < nk) {
-#pragma unroll
-          for (int j = 0; j < CT; ++j) b1[j] = *(const bf16x8 *)(wrow[j] + (ks + 1) * 32);
-        }
-        bf16x8 af[RT];
-#pragma unroll
-        for (int i = 0; i < RT; ++i) af[i] = *(const bf16x8 *)(in + (i * 16 + fr) * P + ks * 32 + fk);
-        // operands swapped (W tile as A, activations as B): the accumulator holds the
-        // transposed tile, lane l owning 4 consecutive output columns of one row, so the
-        // epilogue writes 4 packed bf16 per 8-byte LDS store
+        for (int i = 0; i < RT; ++i) af[i] = *(const bf16x8 *)(arow + i * 16 * P + ks * 32);
+      };
+      // operands swapped (W tile as A, activations as B): the accumulator holds the
+      // transposed tile, lane l owning 4 consecutive output columns of one row, so the
+      // epilogue writes 4 packed bf16 per 8-byte LDS store
+      auto mma = [&](const bf16x8 *b, const bf16x8 *af) {
 #pragma unroll
         for (int i = 0; i < RT; ++i)
 #pragma unroll
           for (int j = 0; j < CT; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0[j], af[i], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], af[i], acc[i][j], 0, 0, 0);
+      };
+      // the number of k steps is a compile-time constant (first layer K0 / 32, then
+      // H / 32), so the loop unrolls into straight-line code whose load waits are exact
+      auto steps = [&](auto nk_c) {
+        constexpr int NK = decltype(nk_c)::value;
+        // scheduling barriers keep each step's loads ahead of the previous step's MFMAs
+        // (left alone, the scheduler sinks them next to their first use)
+        load(bA, aA, 0);
 #pragma unroll
-        for (int j = 0; j < CT; ++j) {
-          b0[j] = b1[j];
-          if (PF >= 2) b1[j] = b2[j];
+        for (int ks = 0; ks < NK; ks += 2) {
+          if (ks + 1 < NK) load(bB, aB, ks + 1);
+          __builtin_amdgcn_sched_barrier(0);
+          mma(bA, aA);
+          __builtin_amdgcn_sched_barrier(0);
+          if (ks + 1 < NK) {
+            if (ks + 2 < NK) load(bA, aA, ks + 2);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(bB, aB);
+            __builtin_amdgcn_sched_barrier(0);
+          }
         }
-      }
+      };
+      if (nk == K0 / 32) steps(std::integral_constant<int, K0 / 32>{});
+      else steps(std::integral_constant<int, H / 32>{});
+      // with one buffer, every wave must be done reading the layer input before any
+      // wave overwrites it with the output (the accumulators hold the whole output tile)
+      if (SB) __syncthreads();
       // epilogue: bias + SiLU -> bf16 activation tile; lane (fr, g = lane / 16) holds
       // rows i*16 + fr, columns n0 + 16 j + 4 g + r (r = 0..3)
       const int g4 = (lane >> 4) * 4;
@@ -405,14 +424,14 @@ __global__ void __launch_bounds__(256) mlp_forward_kernel(const __bf16 *__restri
 
 template <int H>
 int launch_mlp(const __bf16 *X, int64_t ldx, int64_t M, const MlpArgs &a, float *Z, int64_t ldz, hipStream_t st) {
-  constexpr int RT = GH_MLP_RT, PF = GH_MLP_PF;
-  constexpr size_t lds = mlp_lds_bytes<H, RT>();
-  const void *fn = (const void *)&mlp_forward_kernel<H, RT, PF>;
+  constexpr int RT = H == 256 ? GH_MLP_RT256 : GH_MLP_RT128, SB = GH_MLP_SB;
+  constexpr size_t lds = mlp_lds_bytes<H, RT, SB>();
+  const void *fn = (const void *)&mlp_forward_kernel<H, RT, SB>;
   if (lds > 64 * 1024 &&
       hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
     return GH_EHIP;
   const dim3 grid((unsigned)((M + 16 * RT - 1) / (16 * RT)));
-  hipLaunchKernelGGL((mlp_forward_kernel<H, RT, PF>), grid, dim3(256), lds, st, X, ldx, M, a, Z, ldz);
+  hipLaunchKernelGGL((mlp_forward_kernel<H, RT, SB>), grid, dim3(256), lds, st, X, ldx, M, a, Z, ldz);
   return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
 }
 
