@@ -223,7 +223,7 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def _collimator_chunks(eng, n_photons, seed, chunk, hist_per_photon):
+def _collimator_chunks(eng, n_photons, seed, chunk, hist_per_photon, launch=None):
     """Plain upstream transport through the collimator until n_photons exit photons are
     recorded; yields (device records, count, seconds) per chunk."""
     import torch
@@ -236,7 +236,7 @@ def _collimator_chunks(eng, n_photons, seed, chunk, hist_per_photon):
         eng.reset_tally(0)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        eng.run([hist], seed, first_history=[k * (1 << 40)])
+        eng.run([hist], seed, first_history=[k * (1 << 40)], **(launch or {}))
         rec, n_rec = eng.get_phsp(0, device=True)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
@@ -266,12 +266,15 @@ def bench_exitplane(args):
     c = col.run([2_000_000], 99)
     hpp = 2_000_000 / max(1, c["recorded"])
     col.reset_tally(0)
+    # launch parameters of each stage from the autotuner, shared by the plain and learned
+    # paths (trial histories leave tallies and recorders as they were)
+    launch_col, _ = col.autotune([2_000_000])
     # ---- training set and fit
     n_train = int(args.ep_train_photons)
     train = [torch.empty(n_train, device="cuda") for _ in range(8)]
     off = 0
     t_train_rec = 0.0
-    for rec, m, dt, _ in _collimator_chunks(col, n_train, 1, chunk, hpp):
+    for rec, m, dt, _ in _collimator_chunks(col, n_train, 1, chunk, hpp, launch_col):
         for a, b in zip(train, rec):
             a[off:off + m] = b
         off += m
@@ -297,16 +300,18 @@ def bench_exitplane(args):
     t_up = t_ph = 0.0
     done = 0
     holdout = None
-    for rec, m, dt, _ in _collimator_chunks(col, n_total, 7, chunk, hpp):
+    launch_ct = None
+    for rec, m, dt, _ in _collimator_chunks(col, n_total, 7, chunk, hpp, launch_col):
         if holdout is None:
             holdout = [a[:4_000_000].clone() for a in rec]
         if not ct.sims:
             ct.add_simulation(php, phsp_source(rec), talp)
+            launch_ct, _ = ct.autotune([min(m, 20_000_000) // 10])
         else:
             ct.set_source(0, phsp_source(rec))
         torch.cuda.synchronize()
         t = time.perf_counter()
-        ct.run([m], 31, first_history=[done])
+        ct.run([m], 31, first_history=[done], **launch_ct)
         torch.cuda.synchronize()
         t_ph += time.perf_counter() - t
         t_up += dt
@@ -331,7 +336,7 @@ def bench_exitplane(args):
         ct.set_source(0, phsp_source(rec))
         torch.cuda.synchronize()
         t = time.perf_counter()
-        ct.run([m], 32, first_history=[done])
+        ct.run([m], 32, first_history=[done], **launch_ct)
         torch.cuda.synchronize()
         t_ph2 += time.perf_counter() - t
         done += m
@@ -357,6 +362,7 @@ def bench_exitplane(args):
                    "phantom_transport_photons_per_s": n_total / t_ph2,
                    "gain_upstream": t_up / t_gen, "gain_end_to_end": (t_up + t_ph) / (t_gen + t_ph2),
                    "collimator_histories_per_exit_photon": hpp,
+                   "launch": {"collimator": launch_col, "phantom": launch_ct},
                    "exit_plane": plane, "dose": dose},
     }
     col.close()
@@ -400,9 +406,14 @@ def bench_spect(args):
     for ph_, src, tal, _, _ in sims:
         eng.add_simulation(ph_, src, tal)
     hist = [h for _, _, _, h, _ in sims]
+    # launch parameters of the batched phantom launch from the autotuner (trial histories
+    # leave tallies and recorders as they were), as in the batch64 leg
+    t = time.perf_counter()
+    launch, _ = eng.autotune([max(1, h // 50) for h in hist])
+    t_tune = time.perf_counter() - t
     torch.cuda.synchronize()
     t = time.perf_counter()
-    c = eng.run(hist, 4242)
+    c = eng.run(hist, 4242, **launch)
     torch.cuda.synchronize()
     t_transport = time.perf_counter() - t
     _log(f"phantoms: {sum(hist)} histories in {t_transport:.2f} s")
@@ -436,6 +447,7 @@ def bench_spect(args):
                    "form": "batched phantom transport + learned detector response (4x128 MLP) on 128 heads",
                    "arf_training_photons": n_train, "arf_label_s": t_label, "arf_fit_s": t_fit,
                    "phantom_transport_photons_per_s": total / t_transport,
+                   "phantom_launch": launch, "autotune_s": t_tune,
                    "pairs": pairs, "pairs_per_s": pairs / t_pairs,
                    "arf_pairs_per_s": pairs / t_arf, "plain_pairs_per_s": pairs / t_plain,
                    "arf_tflops": arf.flops_per_pair() * pairs / t_arf / 1e12,

@@ -89,6 +89,33 @@ def test_cone_from_outside_air_world_histogram_nonaligned(engine):
     assert c["world_interactions"] > 0
 
 
+def test_spectra_with_empty_bins_and_lines(engine):
+    """Spectra whose CDF has flat stretches (zero-weight bins and lines, many narrow bins in
+    one guide bin) and a histogram of a few hundred bins: the guided CDF search of the
+    kernel must pick the same bin as the oracle's plain binary search."""
+    T = tables(e_max=0.21)
+    ph = small_phantom(24, 3.0)
+    tal = aligned_tally(ph)
+    # lines: zero weights first, in the middle and last
+    ev, ew, cdf = spectrum_lines([0.02, 0.05, 0.08, 0.1405, 0.16, 0.2], [0.0, 1.0, 0.0, 3.0, 1e-6, 0.0])
+    src = SourceSpec(pos_trans=centre(ph), ene_type="spectrum_discrete", energy_value=ev, energy_cdf=cdf)
+    c, g, rc, r = run_both(engine, T, ph, src, tal, 100_000, 5)
+    compare(c, rc, g, r)
+    # histogram: 400 bins, a third of them empty, one bin holding most of the weight
+    rng = np.random.default_rng(8)
+    e = np.linspace(0.010, 0.200, 401)
+    w = rng.uniform(0.0, 1.0, 400)
+    w[rng.uniform(size=400) < 0.33] = 0.0
+    w[0] = 0.0
+    w[-1] = 0.0
+    w[123] = 500.0
+    ev, ew, cdf = spectrum_histogram(e, w)
+    src = SourceSpec(pos_trans=centre(ph), ene_type="spectrum_histogram", energy_value=ev, energy_width=ew,
+                     energy_cdf=cdf)
+    c, g, rc, r = run_both(engine, T, ph, src, tal, 100_000, 6)
+    compare(c, rc, g, r)
+
+
 def test_restricted_phi_and_sphere_source(engine):
     T = tables()
     ph = small_phantom(20, 2.5)
