@@ -293,6 +293,7 @@ class SimulationEngine:
         if first0 < 0:
             raise ValueError("first_history must be >= 0")
         first = first0
+        self.launch = self._launch_parameters(runs)
         self.executed = []  # (SourceSpec, histories, first history) in launch order
         self.inputs = (tables, ph, tal)
         t0 = time.time()
@@ -306,7 +307,7 @@ class SimulationEngine:
                     chunk = self._goal_chunk(first - first0, todo)
                 if self.spect is not None:
                     chunk = min(chunk, tal.record_capacity)
-                c = self.engine.run([chunk], seed, first_history=[first], **sim.launch)
+                c = self.engine.run([chunk], seed, first_history=[first], **self.launch)
                 if self.spect is not None:
                     rec, m = self.engine.get_phsp(sid, device=True)
                     self.spect.consume(rec, min(m, tal.record_capacity))
@@ -323,6 +324,23 @@ class SimulationEngine:
         self.seed = seed
         self.finish(vol, ph, tal, names, tables, counters, n_events, duration, t_init)
         return counters
+
+    def _launch_parameters(self, runs):
+        """Launch parameters of the persistent kernel: sim.launch as given (a dict of
+        gh_launch fields, {} for the library defaults), or "auto" to let gh_autotune pick
+        them on the first source, with trial runs of 1/1000 of the histories per candidate
+        (20 candidates: about 2 % of the run; at least 2e5 and at most 2e7 histories each).
+        The trials leave the tallies as they were."""
+        launch = self.sim.launch
+        if isinstance(launch, str):
+            if launch != "auto":
+                raise ValueError(f"Simulation.launch must be a dict or 'auto', not {launch!r}")
+            n = sum(k for _, _, k in runs)
+            trial = int(min(max(n // 1000, 200_000), 20_000_000))
+            kw, _ = self.engine.autotune([trial])
+            self.autotuned = kw
+            return kw
+        return dict(launch)
 
     def _run_arf_training(self, seed, t_init):
         """ARFTrainingDatasetActor simulation: head model only, no phantom."""

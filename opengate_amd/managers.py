@@ -139,6 +139,8 @@ class Simulation:
         self.device = 0
         # first history id of this run (a job of a split campaign; see opengate_amd.jobs)
         self.first_history = 0
+        # persistent-kernel launch parameters: {} (library defaults), a dict of gh_launch
+        # fields, or "auto" (gh_autotune on a trial of the run)
         self.launch = {}
         self.warnings = []
         self.last_run = None

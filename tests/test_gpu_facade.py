@@ -226,3 +226,22 @@ def test_phase_space_actor_file_feeds_phase_space_source(tmp_path):
     e_out = c["e_deposited"] + c["e_outside_tally"] + c["e_escaped"] + c["e_world"]
     assert e_out == pytest.approx(e_in, rel=1e-6)
     assert c["e_deposited"] > 0.01 * e_in
+
+
+def test_autotuned_launch_gives_the_same_result(tmp_path):
+    """Simulation.launch = "auto" picks launch parameters with gh_autotune; results do not
+    depend on the launch shape (integer counters equal, FP64 tallies up to the order of
+    additions) and the trial runs leave no trace in the tallies."""
+    outs = []
+    for j, launch in enumerate(({}, "auto")):
+        sim, dose, stats = build_sim(tmp_path / f"l{j}", seed=123)
+        sim.sources["tc99m"].n = 300_000
+        sim.launch = launch
+        print(f"[gpu case] launch {launch!r}", flush=True)
+        c = sim.run()
+        outs.append((c, dose.edep.image.array.copy(), dose.edep_squared.image.array.copy()))
+    (c0, e0, q0), (c1, e1, q1) = outs
+    for k in ("histories", "steps", "deposits", "compton", "rayleigh", "photo", "escaped"):
+        assert c0[k] == c1[k], k
+    assert np.max(np.abs(e0 - e1)) <= 1e-9 * e0.max()
+    assert np.max(np.abs(q0 - q1)) <= 1e-9 * q0.max()
