@@ -573,7 +573,11 @@ transport_kernel(Launch L) {
       uint64_t dead = __ballot(mode == M_DEAD);
       const uint32_t n_step = (uint32_t)__popcll(__ballot(mode == M_STEP));
       if (have_work && dead != 0ull && ((uint32_t)__popcll(dead) >= refill_min || n_step == 0u)) {
-        while (dead != 0ull && have_work) {
+        // one sampling pass per refill: lanes whose new photon dies at birth (a source
+        // photon that misses the phantom) wait for the next refill instead of a second,
+        // nearly empty pass through the source sampling
+        bool sampled = false;
+        while (dead != 0ull && have_work && !sampled) {
           if (cur_next >= cur_end) {
             unsigned long long c = 0;
             if (lane_id() == 0) c = atomicAdd(L.next_chunk, 1ull);
@@ -636,6 +640,7 @@ transport_kernel(Launch L) {
             }
           }
           cur_next += take;
+          sampled = take != 0u;
           dead = __ballot(mode == M_DEAD);
         }
       }
