@@ -67,8 +67,8 @@ def main():
     ap.add_argument("--histories", type=float, default=1e9)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--refill", type=int, default=64)
-    ap.add_argument("--slow-min", type=int, default=56)
-    ap.add_argument("--refill-min", type=int, default=48)
+    ap.add_argument("--slow-min", type=int, default=52)
+    ap.add_argument("--refill-min", type=int, default=16)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--workload", default="ct256", choices=("ct256", "batch64", "exitplane", "spect16", "mlp"))
     ap.add_argument("--arf-train-photons", type=float, default=4e7,

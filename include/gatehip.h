@@ -144,8 +144,8 @@ typedef struct gh_launch {
   int32_t grid;              /* workgroups; 0 = automatic */
   int32_t refill_interval;   /* max step-phase iterations (2 Woodcock steps each) per loop */
   int32_t chunk;             /* histories per work-queue chunk */
-  int32_t slow_min;          /* parked lanes (collision / exit / world / dead) that end a step phase */
-  int32_t refill_min;        /* dead lanes that trigger a refill (clamped to slow_min) */
+  int32_t slow_min;          /* parked lanes (collision / exit / world / dead) that end a step phase (0: 52) */
+  int32_t refill_min;        /* dead lanes that trigger a refill (clamped to slow_min; 0: 12) */
 } gh_launch;
 
 typedef struct gh_counters {

@@ -1390,7 +1390,7 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   if (!n_histories) FAIL(GH_EINVAL, "gh_run: n_histories is null");
   HIPCHK(hipSetDevice(ctx->device));
   const int ns = (int)ctx->sims.size();
-  gh_launch lp{256, 0, 64, 4096, 16, 16};
+  gh_launch lp{256, 0, 64, 4096, 52, 12};
   if (launch) {
     if (launch->block > 0) lp.block = launch->block;
     if (launch->grid > 0) lp.grid = launch->grid;
@@ -1547,14 +1547,14 @@ int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *
     if (h.dev.rec_count) HIPCHK(hipMemcpy(&rc, h.dev.rec_count, sizeof(rc), hipMemcpyDeviceToHost));
     rec_counts.push_back(rc);
   }
-  // candidates: workgroup shape (automatic staging / 256 threads) x slow-phase and
-  // refill thresholds x step iterations
+  // candidates: workgroup shape (automatic staging / 256 threads) x slow-phase threshold
+  // x refill threshold (small, frequent refills keep more lanes stepping)
   struct Cand { int block, slow, refill, iter; };
   std::vector<Cand> cands;
   for (int block : {0, 256})
-    for (int slow : {8, 16, 32, 48, 56})
-      for (int iter : {16, 64})
-        cands.push_back({block, slow, slow == 56 ? 48 : slow, iter});
+    for (int slow : {16, 32, 48, 52, 56})
+      for (int refill : {8, 16, 24})
+        if (refill <= slow) cands.push_back({block, slow, refill, 64});
   double best_t = 1e300;
   gh_launch best_lp{};
   int rc = GH_OK;
