@@ -112,15 +112,18 @@ struct Hot {
   const uint8_t *labels;
   int nx, ny, nz;
   float isx, isy, isz;
-  int l4;  // labels packed two per byte (low nibble = even voxel)
+  uint32_t l4;  // 1: labels packed two per byte (low nibble = even voxel); 0: one per byte
   __device__ __forceinline__ void load(const SimDev &s) {
     labels = s.labels; nx = s.nx; ny = s.ny; nz = s.nz;
     isx = s.inv_s[0]; isy = s.inv_s[1]; isz = s.inv_s[2];
-    l4 = s.lbits == 4;
+    l4 = s.lbits == 4 ? 1u : 0u;
   }
+  // one global load and one bit-field extract for both layouts (no branch on the
+  // layout): byte vox >> l4, then its low or high nibble (l4 = 1) or the whole byte
   __device__ __forceinline__ int label(int vox) const {
-    if (l4) return (int)((labels[vox >> 1] >> ((vox & 1) << 2)) & 15u);
-    return labels[vox];
+    const auto *g = (const __attribute__((address_space(1))) uint8_t *)labels;
+    const uint32_t b = g[(uint32_t)vox >> l4];
+    return (int)__builtin_amdgcn_ubfe(b, ((uint32_t)vox & l4) << 2, 8u >> l4);
   }
 };
 
