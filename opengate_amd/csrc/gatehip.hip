@@ -222,22 +222,32 @@ struct LaneStats {
   __device__ __forceinline__ void add_e(int k, double v) { unsafeAtomicAdd(en + k * stride, v); }
 };
 
-__device__ __forceinline__ void flush_slot(const SimDev &s, int vox, float sum, float dsum, uint32_t cnt,
+// tally arrays of a simulation, read into registers before the first atomic of a flush
+// (a read of the descriptor issued after an atomic would wait for that atomic)
+struct TallyOut {
+  double *edep, *edep2, *counts, *dose, *dose2;
+};
+__device__ __forceinline__ TallyOut tally_out(const SimDev &s) {
+  return TallyOut{s.edep, s.edep2, s.counts, s.dose, s.dose2};
+}
+
+__device__ __forceinline__ void flush_slot(const TallyOut &o, int vox, float sum, float dsum, uint32_t cnt,
                                            LaneStats &st) {
-  gatomic_add(s.edep + vox, (double)sum);
-  if (s.edep2) gatomic_add(s.edep2 + vox, (double)sum * (double)sum);
-  if (s.counts) gatomic_add(s.counts + vox, (double)cnt);
-  if (s.dose) gatomic_add(s.dose + vox, (double)dsum);
-  if (s.dose2) gatomic_add(s.dose2 + vox, (double)dsum * (double)dsum);
+  gatomic_add(o.edep + vox, (double)sum);
+  if (o.edep2) gatomic_add(o.edep2 + vox, (double)sum * (double)sum);
+  if (o.counts) gatomic_add(o.counts + vox, (double)cnt);
+  if (o.dose) gatomic_add(o.dose + vox, (double)dsum);
+  if (o.dose2) gatomic_add(o.dose2 + vox, (double)dsum * (double)dsum);
   st.inc(C_DEPOSITS);
   st.add_e(E_DEP, (double)sum);
 }
 
 __device__ __forceinline__ void sq_flush_all(const SimDev &s, SqCache &c, LaneStats &st) {
-  if (c.n > 0) flush_slot(s, c.v0, c.s0, c.d0, c.c0, st);
-  if (c.n > 1) flush_slot(s, c.v1, c.s1, c.d1, c.c1, st);
-  if (c.n > 2) flush_slot(s, c.v2, c.s2, c.d2, c.c2, st);
-  if (c.n > 3) flush_slot(s, c.v3, c.s3, c.d3, c.c3, st);
+  const TallyOut o = tally_out(s);
+  if (c.n > 0) flush_slot(o, c.v0, c.s0, c.d0, c.c0, st);
+  if (c.n > 1) flush_slot(o, c.v1, c.s1, c.d1, c.c1, st);
+  if (c.n > 2) flush_slot(o, c.v2, c.s2, c.d2, c.c2, st);
+  if (c.n > 3) flush_slot(o, c.v3, c.s3, c.d3, c.c3, st);
   c.n = 0;
 }
 
@@ -261,7 +271,7 @@ __device__ __forceinline__ void deposit(const SimDev &s, SqCache &c, int tv, flo
     return;
   }
   if (c.n == 4) {
-    flush_slot(s, c.v0, c.s0, c.d0, c.c0, st);
+    flush_slot(tally_out(s), c.v0, c.s0, c.d0, c.c0, st);
     c.v0 = c.v1; c.s0 = c.s1; c.d0 = c.d1; c.c0 = c.c1;
     c.v1 = c.v2; c.s1 = c.s2; c.d1 = c.d2; c.c1 = c.c2;
     c.v2 = c.v3; c.s2 = c.s3; c.d2 = c.d3; c.c2 = c.c3;
@@ -502,9 +512,11 @@ transport_kernel(Launch L) {
   st.steps = 0u; st.rej = 0u; st.cnt = s_cnt; st.en = s_en + threadIdx.x; st.stride = BLOCK;
 
   Hot hot;
-  if (!BATCHED) hot.load(L.sims[0]);
-  const float tle_emax = TLE ? L.sims[0].tle_emax : 0.0f;
-  const float tle_emin = TLE ? L.sims[0].tle_emin : 0.0f;
+  // descriptor of simulation i (always 0 in the single-simulation forms)
+  auto sim = [&](int i) -> const SimDev & { return L.sims[BATCHED ? i : 0]; };
+  if (!BATCHED) hot.load(sim(0));
+  const float tle_emax = TLE ? sim(0).tle_emax : 0.0f;
+  const float tle_emin = TLE ? sim(0).tle_emin : 0.0f;
 
   Photon ph;
   ph.mode = MODE_BOX; ph.sim = 0; ph.hist = 0;
@@ -592,7 +604,8 @@ transport_kernel(Launch L) {
             const uint64_t local = (c - L.chunk_offsets[s]) * (uint64_t)L.chunk;
             cur_next = local;
             const uint64_t e = local + L.chunk;
-            cur_end = e < L.sims[s].n_hist ? e : L.sims[s].n_hist;
+            const uint64_t nh = sim(s).n_hist;
+            cur_end = e < nh ? e : nh;
             continue;
           }
           const uint64_t avail = cur_end - cur_next;
@@ -601,7 +614,7 @@ transport_kernel(Launch L) {
           if (mode == M_DEAD) {
             const uint32_t rank = mbcnt(dead);
             if (rank < take) {
-              const SimDev &s = L.sims[BATCHED ? cur_sim : 0];
+              const SimDev &s = sim(cur_sim);
               if (BATCHED) hot.load(s);
               ph.sim = BATCHED ? cur_sim : 0;
               ph.hist = (uint32_t)(cur_next + rank);
@@ -676,7 +689,7 @@ transport_kernel(Launch L) {
               const int lab = hot.label(vox);
               const float mu = lerp_node(mu_tab + lab * NB1, ec);
               if (TLE && ph.E <= tle_emax) {
-                const SimDev &s0 = L.sims[0];
+                const SimDev &s0 = sim(0);
                 const float x = (ph.E * ph.wt) * (lerp_node(P.muen + lab * NB1, ec) * ec.inv_maj);
                 deposit(s0, sq, tally_voxel(s0, hot, vox, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
               }
@@ -697,7 +710,7 @@ transport_kernel(Launch L) {
     // collision with the world material and no tally voxel (cvox = -1), so box and
     // world collisions share one copy of the interaction code.
     if (mode == M_COLLIDE || mode == M_EXIT || mode == M_WORLD) {
-      const SimDev &s = L.sims[BATCHED ? (int)rng.c2 : 0];
+      const SimDev &s = sim(BATCHED ? (int)rng.c2 : 0);
       bool kill = false;
       // one set of divisions for the box exit and the world transport that follows it
       InvDir iv = {0.0f, 0.0f, 0.0f};
@@ -792,8 +805,10 @@ transport_kernel(Launch L) {
         if (!kill && r < pc) ecache_set(ec, ph.E, maj_tab);
       }
       if (kill) {
-        sq_flush_all(s, sq, st);
+        // the event record goes first: a read of the descriptor placed after the flush's
+        // atomics would wait for them
         if (s.event_edep) s.event_edep[history_of(rng) - s.first_hist] = ev_edep;
+        sq_flush_all(s, sq, st);
         mode = M_DEAD;
       }
     }
