@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--slow-min", type=int, default=52)
     ap.add_argument("--refill-min", type=int, default=16)
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0,
+                    help="ct256: workgroup size (0: automatic staging, all tables in LDS for one simulation)")
     ap.add_argument("--workload", default="ct256", choices=("ct256", "batch64", "exitplane", "spect16", "mlp"))
     ap.add_argument("--arf-train-photons", type=float, default=4e7,
                     help="spect16: photons through the head model that label the ARF training set")
@@ -114,7 +116,7 @@ def main():
     eng.set_physics(tables)
     eng.add_simulation(ph, src, tal)
     n = int(args.histories)
-    launch = {"refill_interval": args.refill, "grid": args.grid, "slow_min": args.slow_min,
+    launch = {"block": args.block, "refill_interval": args.refill, "grid": args.grid, "slow_min": args.slow_min,
               "refill_min": args.refill_min}
     if args.autotune:
         launch, _ = eng.autotune([min(n, 20_000_000)])

@@ -61,9 +61,12 @@ def test_box_source_lines_and_launch_shapes(engine):
     src = SourceSpec(pos_type="box", pos_param=(20.0, 30.0, 10.0), pos_trans=centre(ph), ene_type="spectrum_discrete",
                      energy_value=ev, energy_cdf=cdf)
     tal = aligned_tally(ph)
+    # block 0: automatic staging (all tables in LDS, one 1024-thread workgroup per CU)
     for launch in ({"block": 256, "refill_interval": 16, "chunk": 4096},
                    {"block": 64, "grid": 7, "refill_interval": 1, "chunk": 100},
-                   {"block": 128, "grid": 301, "refill_interval": 64, "chunk": 333}):
+                   {"block": 128, "grid": 301, "refill_interval": 64, "chunk": 333},
+                   {"block": 0, "slow_min": 52, "refill_min": 16},
+                   {"block": 1024, "grid": 5, "refill_interval": 8, "chunk": 1000, "slow_min": 8, "refill_min": 4}):
         c, g, rc, r = run_both(engine, T, ph, src, tal, 150_000, 77, **launch)
         compare(c, rc, g, r)
 
