@@ -140,7 +140,9 @@ typedef struct gh_tally_desc {
 } gh_tally_desc;
 
 typedef struct gh_launch {
-  int32_t block;             /* threads per workgroup (multiple of 64, <= 256) */
+  int32_t block;             /* threads per workgroup: a multiple of 64 up to 256, or 1024;
+                                0 = automatic (one simulation whose tables fit in LDS: all
+                                tables in LDS, 1024 threads; else 256 threads) */
   int32_t grid;              /* workgroups; 0 = automatic */
   int32_t refill_interval;   /* max step-phase iterations (2 Woodcock steps each) per loop */
   int32_t chunk;             /* histories per work-queue chunk */
