@@ -723,16 +723,27 @@ transport_kernel(Launch L) {
         const float t = fmaxf(tf, 0.0f);
         const float ex = fmaf(t, ph.u, ph.x), ey = fmaf(t, ph.v, ph.y), ez = fmaf(t, ph.w, ph.z);
         if (s.rec_face >= 0 && (face == s.rec_face || s.rec_face == GH_FACE_ALL)) {
-          const uint64_t m = __ballot(1);
-          const uint32_t leader = __ffsll((unsigned long long)m) - 1;
-          unsigned long long base = 0;
-          if (lane_id() == leader) base = atomicAdd(s.rec_count, (unsigned long long)__popcll(m));
-          base = __shfl(base, leader, 64);
-          const uint64_t slot = base + mbcnt(m);
-          if (slot < s.rec_cap) {
-            s.rec[0][slot] = ex; s.rec[1][slot] = ey; s.rec[2][slot] = ez;
-            s.rec[3][slot] = ph.u; s.rec[4][slot] = ph.v; s.rec[5][slot] = ph.w;
-            s.rec[6][slot] = ph.E; s.rec[7][slot] = ph.wt;
+          // one slot reservation per simulation among the recording lanes: around a chunk
+          // boundary the lanes of a batched wave hold photons of two simulations, each with
+          // its own record buffer and counter (one pass in the single-simulation forms)
+          const int my_sim = BATCHED ? (int)rng.c2 : 0;
+          uint64_t pending = __ballot(1);
+          while (pending != 0ull) {
+            const uint32_t leader = __ffsll((unsigned long long)pending) - 1;
+            const int lsim = BATCHED ? __shfl(my_sim, leader, 64) : 0;
+            const uint64_t m = __ballot(my_sim == lsim) & pending;
+            unsigned long long base = 0;
+            if (lane_id() == leader) base = atomicAdd(s.rec_count, (unsigned long long)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (my_sim == lsim) {
+              const uint64_t slot = base + mbcnt(m);
+              if (slot < s.rec_cap) {
+                s.rec[0][slot] = ex; s.rec[1][slot] = ey; s.rec[2][slot] = ez;
+                s.rec[3][slot] = ph.u; s.rec[4][slot] = ph.v; s.rec[5][slot] = ph.w;
+                s.rec[6][slot] = ph.E; s.rec[7][slot] = ph.wt;
+              }
+            }
+            pending &= ~m;
           }
           st.inc(C_RECORDED);
           st.add_e(E_ESC, (double)(ph.E * ph.wt));
@@ -1471,8 +1482,11 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   L.counters = ctx->d_counters;
   L.energies = ctx->d_energies;
   // staging level and block: all tables in LDS shared by one 1024-thread workgroup per CU
-  // when they fit, else mu + majorant in LDS with 256-thread workgroups, else none.
-  const size_t budget_big = 150 * 1024, budget_small = 36 * 1024;
+  // when they fit, else mu + majorant in LDS with 256-thread workgroups, else none. The
+  // budgets leave room for the kernel's static LDS (per-lane energy slots and counters:
+  // 32 KiB at 1024 threads, 8 KiB at 256) within the 160 KiB of a CU.
+  const size_t static_big = (size_t)N_ENERGIES * 1024 * sizeof(double) + N_COUNTERS * sizeof(unsigned long long);
+  const size_t budget_big = 160 * 1024 - static_big - 512, budget_small = 36 * 1024;
   int stage, big;
   if (launch && launch->block > 0) {
     big = lp.block > 256;
