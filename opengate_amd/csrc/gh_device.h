@@ -126,18 +126,10 @@ GH_DEV void normalize3(float &x, float &y, float &z) {
   x = x * inv; y = y * inv; z = z * inv;
 }
 
-// (cos phi, sin phi) for phi uniform in [0, 2pi): point uniform in the unit disc
+// (cos phi, sin phi) for phi = 2 pi u, u uniform in (0, 1): one draw and no rejection
+// loop (a rejection loop runs as long as its slowest lane in a wave)
 GH_DEV void azimuth(Rng &g, float &cp, float &sp) {
-  float a, b, r2;
-  int it = 0;
-  do {
-    a = fmaf(2.0f, u01(g), -1.0f);
-    b = fmaf(2.0f, u01(g), -1.0f);
-    r2 = fmaf(b, b, a * a);
-  } while ((r2 > 1.0f || r2 == 0.0f) && ++it < MAX_REJECT);
-  const float inv = 1.0f / r2;
-  cp = fmaf(a, a, -(b * b)) * inv;
-  sp = ((a + a) * b) * inv;
+  gsincos(6.28318530717958647692f * u01(g), sp, cp);
 }
 
 // new direction after scattering by polar cosine c and azimuth (cp, sp)

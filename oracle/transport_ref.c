@@ -146,19 +146,9 @@ static void unit(float v[3]) {
 #define MEC2 0x1.05a1a8p-1f
 #define KPM 0x1.fac5e8p+8f
 
-static void disc_azimuth(stream_t *s, float *cp, float *sp) {
-  float a, b, r2;
-  int tries = 0;
-  for (;;) {
-    a = fmaf(2.0f, uni(s), -1.0f);
-    b = fmaf(2.0f, uni(s), -1.0f);
-    r2 = fmaf(b, b, a * a);
-    ++tries;
-    if (!(r2 > 1.0f || r2 == 0.0f) || tries >= CAP) break;
-  }
-  float inv = 1.0f / r2;
-  *cp = fmaf(a, a, -(b * b)) * inv;
-  *sp = ((a + a) * b) * inv;
+/* (cos phi, sin phi) for phi = 2 pi u: the kernel's azimuth (gh_device.h) */
+static void ref_azimuth(stream_t *s, float *cp, float *sp) {
+  ref_sincos(6.28318530717958647692f * uni(s), sp, cp);
 }
 
 static void scatter_dir(float d[3], float c, float cp, float sp) {
@@ -502,7 +492,7 @@ static void emit(const sim_t *S, stream_t *rs, uint64_t h, photon_t *ph) {
       float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
       float cp, sp;
       if (src->phi_full) {
-        disc_azimuth(rs, &cp, &sp);
+        ref_azimuth(rs, &cp, &sp);
       } else {
         float phi = fmaf(uni(rs), S->d_phi, src->phi_min);
         ref_sincos(phi, &sp, &cp);
@@ -528,7 +518,7 @@ static void emit(const sim_t *S, stream_t *rs, uint64_t h, photon_t *ph) {
     float ct = fmaf(2.0f, uni(rs), -1.0f);
     float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
     float cp, sp;
-    disc_azimuth(rs, &cp, &sp);
+    ref_azimuth(rs, &cp, &sp);
     ph->d[0] = st * cp; ph->d[1] = st * sp; ph->d[2] = ct;
   } else if (src->dir_type == GH_DIR_FOCUSED && src->pos_type != GH_POS_PHSP) {
     for (int i = 0; i < 3; ++i) ph->d[i] = src->focus[i] - ph->p[i];
@@ -634,7 +624,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
             if (is_det) ev += x;
           }
           float cp, sp;
-          disc_azimuth(&rs, &cp, &sp);
+          ref_azimuth(&rs, &cp, &sp);
           scatter_dir(ph.d, c, cp, sp);
           ph.E = e1;
           if (ph.E < d->e_cut || (tle && ph.E <= tle_emin)) {
@@ -649,7 +639,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
           st->c[K_RAYL]++;
           float c = rayleigh_sample(&rs, P, lab, ph.E);
           float cp, sp;
-          disc_azimuth(&rs, &cp, &sp);
+          ref_azimuth(&rs, &cp, &sp);
           scatter_dir(ph.d, c, cp, sp);
         } else {
           st->c[K_PHOTO]++;
@@ -727,7 +717,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         float e1 = ph.E / qq;
         st->e[EN_WORLD] += (double)((ph.E - e1) * ph.w);
         float cp, sp;
-        disc_azimuth(&rs, &cp, &sp);
+        ref_azimuth(&rs, &cp, &sp);
         scatter_dir(ph.d, c, cp, sp);
         ph.E = e1;
         if (ph.E < d->e_cut) {
@@ -739,7 +729,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
       } else if (r < pr) {
         float c = rayleigh_sample(&rs, P, wm, ph.E);
         float cp, sp;
-        disc_azimuth(&rs, &cp, &sp);
+        ref_azimuth(&rs, &cp, &sp);
         scatter_dir(ph.d, c, cp, sp);
       } else {
         st->e[EN_WORLD] += (double)(ph.E * ph.w);

@@ -153,7 +153,9 @@ def test_exact_squared_tally():
     # history revisits an evicted voxel (bounded by the eviction count)
     diff = np.abs(out["edep2"] - out["exact_edep2"]).sum()
     assert diff <= 1e-9 * out["exact_edep2"].sum() or c["sq_evictions"] > 0
-    assert np.all(out["edep2"] <= out["exact_edep2"] * (1 + 1e-12) + 1e-300)
+    # splitting a voxel's history sum (evictions) can only lower sum^2; the slot sums are
+    # FP32 (each add rounds by <= 2^-24 relative, squared doubles it), the exact ones FP64
+    assert np.all(out["edep2"] <= out["exact_edep2"] * (1 + 1e-6) + 1e-300)
 
 
 def test_woodcock_majorant_independence():
