@@ -292,18 +292,15 @@ GH_DEV void sample_compton(Rng &g, const Tabs &T, int nx, int lab, float E, floa
   int it = 0;
   for (; it < MAX_REJECT; ++it) {
     const float r1 = u01(g), r2 = u01(g), r3 = u01(g);
-    bool acc;
-    if (r1 <= thr) {
-      q = fmaf(a2, r2, 1.0f);
-      const float iq = 1.0f / q;
-      acc = r3 <= 4.0f * (iq - iq * iq);
-    } else {
-      q = onep / fmaf(a2, r2, 1.0f);
-      const float cc = 1.0f - (q - 1.0f) / alpha;
-      acc = r3 <= 0.5f * fmaf(cc, cc, 1.0f / q);
-    }
-    if (!acc) { rejections++; continue; }
+    // both branches of Kahn's method as selects over shared terms (a wave would run both
+    // branches anyway): q = t or (1 + 2 alpha) / t, then 1/q and cos theta once
+    const float t = fmaf(a2, r2, 1.0f);
+    const bool low = r1 <= thr;
+    q = low ? t : onep / t;
+    const float iq = 1.0f / q;
     c = 1.0f - (q - 1.0f) / alpha;
+    const bool acc = low ? (r3 <= 4.0f * (iq - iq * iq)) : (r3 <= 0.5f * fmaf(c, c, iq));
+    if (!acc) { rejections++; continue; }
     const float x = two_k2 * (1.0f - c);
     const float s = qgrid_value(srow, nx, x);
     if (u01(g) <= s) break;
