@@ -284,6 +284,18 @@ def test_batched_launch_equals_separate_runs(engine):
         assert c[k] == tot[k], k
 
 
+@pytest.mark.parametrize("launch", [{"block": 256}, {"block": 0}], ids=["wg256", "staged"])
+def test_tle_kerma_estimator_launch_shapes(engine, launch):
+    """Track-length (collision kerma) dose tally: counters and tallies equal the oracle's for
+    the 256-thread form and the staged form (all tables in LDS, 1024-thread workgroups)."""
+    T = tables(e_max=0.21)
+    ph = small_phantom(24, 2.0)
+    src = SourceSpec(pos_trans=centre(ph), energy=0.1405)
+    tal = aligned_tally(ph, tle=True, tle_energy_min=0.003, tle_energy_max=0.15)
+    c, g, rc, r = run_both(engine, T, ph, src, tal, 60_000, 21, **launch)
+    compare(c, rc, g, r)
+
+
 def test_batched_recording_per_simulation(engine):
     """Exit-face recording in a batched launch: with small chunks the lanes of a wave hold
     photons of different simulations, and each record must land in the buffer of its own
