@@ -223,6 +223,37 @@ def test_label_packing_odd_sizes(engine, n_mat):
     compare(c, rc, g_, r)
 
 
+@pytest.mark.parametrize("launch", [{"block": 0}, {"block": 256}], ids=["auto", "wg256"])
+def test_many_materials_partial_and_no_staging(engine, launch):
+    """40 materials: the full table set (about 290 KiB) exceeds the LDS of a CU, so the
+    automatic launch stages only mu and the majorant, with 1024-thread workgroups; with
+    256-thread workgroups even those exceed the per-workgroup budget and every table is
+    read from global memory. Both equal the oracle."""
+    from opengate_amd.physics.materials import MaterialDatabase
+    from opengate_amd.physics import xs
+    from opengate_amd.units import g4_units
+
+    db = MaterialDatabase()
+    names = ["G4_WATER", "G4_BONE_CORTICAL_ICRP", "G4_LUNG_ICRP", "G4_AIR", "G4_TISSUE_SOFT_ICRP", "G4_Al",
+             "G4_MUSCLE_SKELETAL_ICRP", "G4_ADIPOSE_TISSUE_ICRP", "G4_BONE_COMPACT_ICRU", "G4_PLEXIGLASS"]
+    for k in range(30):
+        db.add_material_weights(f"WATER_{k}", ["H", "O"], [0.111894, 0.888106],
+                                (0.9 + 0.02 * k) * g4_units.g / g4_units.cm3)
+        names.append(f"WATER_{k}")
+    T = xs.build_tables([db.get(n) for n in names], 0.1405)
+    nb1, nx, m = T.n_buckets + 1, len(T.ray_x), len(names)
+    lds1 = 4 * (m * nb1 + nb1)
+    lds2 = lds1 + 4 * (2 * m * nb1 + 2 * m * nx + nx)
+    assert lds2 > 160 * 1024 and 36 * 1024 < lds1 < 100 * 1024
+    g = np.random.default_rng(40)
+    lab = g.integers(0, m, size=(13, 11, 9)).astype(np.uint8)
+    ph = PhantomSpec(lab, (2.0, 2.5, 3.0))
+    src = SourceSpec(pos_type="box", pos_param=(18.0, 18.0, 18.0), pos_trans=centre(ph), energy=0.1)
+    tal = aligned_tally(ph)
+    c, g_, rc, r = run_both(engine, T, ph, src, tal, 50_000, 4, **launch)
+    compare(c, rc, g_, r)
+
+
 def test_degenerate_runs(engine):
     """Zero histories, a one-voxel phantom, and a source that never reaches the phantom."""
     T = tables()
