@@ -533,6 +533,7 @@ transport_kernel(Launch L) {
   float ev_edep = 0.0f;
   int mode = M_DEAD;
   int cvox = 0;                  // voxel of a pending collision (-1: in the world material)
+  int clab = 0;                  // its label (kept from the step: no second global read)
 
   // wave-uniform work state
   int cur_sim = 0;
@@ -699,6 +700,7 @@ transport_kernel(Launch L) {
               if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
                 mode = M_COLLIDE;
                 cvox = vox;
+                clab = lab;
               }
             } else {
               mode = M_EXIT;
@@ -763,8 +765,7 @@ transport_kernel(Launch L) {
       if (mode == M_WORLD && !kill) world_step(s, kill, iv);
       if (mode == M_COLLIDE && !kill) {
         const bool in_world = cvox < 0;
-        // the label is read again rather than kept in a register from the step phase
-        const int lab = in_world ? s.world_mat : hot.label(cvox);
+        const int lab = in_world ? s.world_mat : clab;
         const float r = u01(rng);
         float pc, pr;
         branching(T, NB1, lab, ec, pc, pr);
