@@ -1588,7 +1588,7 @@ int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *
   struct Cand { int block, slow, refill, iter; };
   std::vector<Cand> cands;
   for (int block : {0, 256})
-    for (int slow : {16, 32, 48, 52, 56})
+    for (int slow : {16, 32, 48, 52, 56, 60, 62})
       for (int refill : {8, 16, 24})
         if (refill <= slow) cands.push_back({block, slow, refill, 64});
   double best_t = 1e300;

@@ -329,7 +329,7 @@ class SimulationEngine:
         """Launch parameters of the persistent kernel: sim.launch as given (a dict of
         gh_launch fields, {} for the library defaults), or "auto" to let gh_autotune pick
         them on the first source, with trial runs of 1/1000 of the histories per candidate
-        (28 candidates: about 3 % of the run; at least 2e5 and at most 2e7 histories each).
+        (40 candidates: about 4 % of the run; at least 2e5 and at most 2e7 histories each).
         The trials leave the tallies as they were."""
         launch = self.sim.launch
         if isinstance(launch, str):
