@@ -185,7 +185,8 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
  * Runs n_trial[s] histories per simulation (RNG seed `seed`, keep it different from the
  * production seed) for each candidate launch (workgroup shape, slow-phase threshold,
  * refill threshold, step iterations), keeps the fastest in *best (kernel ms in *best_ms)
- * and restores every tally and recorder count to its state before the call. */
+ * and restores every tally and recorder count to its state before the call. Per-history
+ * detector energies (gh_get_event_edep) of earlier runs are not kept. */
 int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *best, double *best_ms);
 
 /* Copies a tally to host. Any pointer may be NULL. counts/dose images only if enabled. */

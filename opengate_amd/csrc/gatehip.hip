@@ -38,6 +38,7 @@ struct SimDev {
   int lbits;
   int nx, ny, nz, world_mat;
   float inv_s[3];
+  float ts_ph[3];  // voxel spacing
   float L[3];
   float wlo[3], whi[3];
   // source
@@ -61,7 +62,7 @@ struct SimDev {
   // tally
   double *edep, *edep2, *dose, *dose2, *counts;
   int aligned, tnx, tny, tnz;
-  float tcorner[3], tinv[3];
+  float tcorner[3], tinv[3], ts[3];  // tally grid (aligned: the phantom grid, corner 0)
   // recorder / detector
   int rec_face;
   uint32_t rec_cap;
@@ -289,13 +290,15 @@ __device__ __forceinline__ void deposit(const SimDev &s, SqCache &c, int tv, flo
   c.n += 1;
 }
 
-// tally voxel of a deposit at (x,y,z) in the phantom frame; vox = phantom voxel
-__device__ __forceinline__ int tally_voxel(const SimDev &s, const Hot &h, int vox, float x, float y,
-                                           float z) {
-  if (s.aligned) return vox;
-  const int ix = (int)floorf((x - s.tcorner[0]) * s.tinv[0]);
-  const int iy = (int)floorf((y - s.tcorner[1]) * s.tinv[1]);
-  const int iz = (int)floorf((z - s.tcorner[2]) * s.tinv[2]);
+// Tally voxel of a deposit at (x,y,z) in the phantom frame, -1 outside the tally grid.
+// The index follows the ITK contract of GateDoseActor::GetVoxelPosition
+// (GateDoseActor.cpp:144-178, TransformPhysicalPointToIndex) exactly (cell_index), also on
+// aligned grids, where the Woodcock step's voxel (a navigation analog) can differ from it
+// for points within an ulp of a face.
+__device__ __forceinline__ int tally_voxel(const SimDev &s, float x, float y, float z) {
+  const int ix = cell_index(x, s.tcorner[0], s.ts[0], s.tinv[0]);
+  const int iy = cell_index(y, s.tcorner[1], s.ts[1], s.tinv[1]);
+  const int iz = cell_index(z, s.tcorner[2], s.ts[2], s.tinv[2]);
   if ((unsigned)ix >= (unsigned)s.tnx || (unsigned)iy >= (unsigned)s.tny || (unsigned)iz >= (unsigned)s.tnz)
     return -1;
   return ix + s.tnx * (iy + s.tny * iz);
@@ -334,7 +337,6 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
         }
         break;
       }
-      case GH_POS_DISC:
       case GH_POS_VOXELS: {
         // Walker alias table: entry i uniform (32-bit multiply-shift), kept with
         // probability prob[i], else its alias; then uniform inside the voxel
@@ -348,6 +350,7 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
         z = ((float)iz + u01(g)) * s.vox_s[2];
         break;
       }
+      case GH_POS_DISC:
       case GH_POS_CYLINDER: {
         float a, b, r2;
         int it = 0;
@@ -413,9 +416,13 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
   p.wt = wt;
 }
 
-__device__ __forceinline__ bool inside_box(const Hot &h, float x, float y, float z) {
-  const int ix = (int)floorf(x * h.isx), iy = (int)floorf(y * h.isy), iz = (int)floorf(z * h.isz);
-  return (unsigned)ix < (unsigned)h.nx && (unsigned)iy < (unsigned)h.ny && (unsigned)iz < (unsigned)h.nz;
+// phantom voxel of a birth point (exact cell index, as the tally's), -1 outside the box
+__device__ __forceinline__ int birth_voxel(const SimDev &s, const Hot &h, float x, float y, float z) {
+  const int ix = cell_index(x, 0.0f, s.ts_ph[0], h.isx), iy = cell_index(y, 0.0f, s.ts_ph[1], h.isy),
+            iz = cell_index(z, 0.0f, s.ts_ph[2], h.isz);
+  if ((unsigned)ix < (unsigned)h.nx && (unsigned)iy < (unsigned)h.ny && (unsigned)iz < (unsigned)h.nz)
+    return ix + h.nx * (iy + h.ny * iz);
+  return -1;
 }
 
 // ---------------------------------------------------------------------- kernel
@@ -628,16 +635,14 @@ transport_kernel(Launch L) {
               st.inc(C_HIST);
               ev_edep = 0.0f;
               sq.n = 0;
-              const bool inside = inside_box(hot, ph.x, ph.y, ph.z);
+              const int bvox = birth_voxel(s, hot, ph.x, ph.y, ph.z);
+              const bool inside = bvox >= 0;
               if (ph.E < P.e_cut || (TLE && ph.E <= tle_emin)) {
                 // below the tracking cut from the start: deposit locally (inside) and stop
                 if (inside) {
-                  const int ix = (int)floorf(ph.x * hot.isx), iy = (int)floorf(ph.y * hot.isy),
-                            iz = (int)floorf(ph.z * hot.isz);
-                  const int vox = ix + hot.nx * (iy + hot.ny * iz);
-                  const int lab = hot.label(vox);
+                  const int lab = hot.label(bvox);
                   const float x = ph.E * ph.wt;
-                  deposit(s, sq, tally_voxel(s, hot, vox, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
+                  deposit(s, sq, tally_voxel(s, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
                   if (lab >= s.det_label_min && s.det_label_min >= 0) ev_edep += x;
                   sq_flush_all(s, sq, st);
                 } else {
@@ -695,7 +700,7 @@ transport_kernel(Launch L) {
               if (TLE && ph.E <= tle_emax) {
                 const SimDev &s0 = sim(0);
                 const float x = (ph.E * ph.wt) * (lerp_node(P.muen + lab * NB1, ec) * ec.inv_maj);
-                deposit(s0, sq, tally_voxel(s0, hot, vox, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
+                deposit(s0, sq, tally_voxel(s0, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
               }
               if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
                 mode = M_COLLIDE;
@@ -769,7 +774,7 @@ transport_kernel(Launch L) {
         const float r = u01(rng);
         float pc, pr;
         branching(T, NB1, lab, ec, pc, pr);
-        const int tv = in_world ? -1 : tally_voxel(s, hot, cvox, ph.x, ph.y, ph.z);
+        const int tv = in_world ? -1 : tally_voxel(s, ph.x, ph.y, ph.z);
         const float irho = P.inv_rho[lab];
         const bool det = !in_world && s.det_label_min >= 0 && lab >= s.det_label_min;
         // electrons of photons scored by the kerma estimator give nothing locally
@@ -911,6 +916,7 @@ struct SimHost {
   int flags = 0;
   int det = -1;
   uint64_t event_cap = 0;
+  uint64_t event_n = 0;  // histories of the last run (valid per-history detector energies)
 };
 
 }  // namespace
@@ -966,6 +972,8 @@ namespace {
 
 int validate_source(gh_ctx *ctx, const gh_source_desc *src) {
   if (src->pos_type < GH_POS_POINT || src->pos_type > GH_POS_VOXELS) FAIL(GH_EINVAL, "invalid pos_type");
+  if (src->pos_type != GH_POS_VOXELS && (src->vox_n != 0 || src->vox_prob || src->vox_alias || src->vox_index))
+    FAIL(GH_EINVAL, "voxel-source fields are set but pos_type is not GH_POS_VOXELS");
   if (src->pos_type == GH_POS_VOXELS) {
     if (src->vox_n <= 0 || src->vox_n > 0x7FFFFFFF || !src->vox_prob || !src->vox_alias || !src->vox_index)
       FAIL(GH_EINVAL, "voxel source needs a non-empty alias table");
@@ -1317,6 +1325,7 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
   d.world_mat = ph->world_material;
   for (int i = 0; i < 3; ++i) {
     d.inv_s[i] = 1.0f / ph->spacing[i];
+    d.ts_ph[i] = ph->spacing[i];
     const int n = i == 0 ? ph->nx : (i == 1 ? ph->ny : ph->nz);
     d.L[i] = (float)((double)n * (double)ph->spacing[i]);
     d.wlo[i] = ph->world_lo[i];
@@ -1331,8 +1340,10 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
   d.aligned = (tal->flags & GH_TALLY_ALIGNED) ? 1 : 0;
   d.tnx = tal->nx; d.tny = tal->ny; d.tnz = tal->nz;
   for (int i = 0; i < 3; ++i) {
-    d.tcorner[i] = tal->corner[i];
-    d.tinv[i] = 1.0f / tal->spacing[i];
+    // an aligned tally is the phantom grid: corner at the origin of the phantom frame
+    d.tcorner[i] = d.aligned ? 0.0f : tal->corner[i];
+    d.ts[i] = d.aligned ? ph->spacing[i] : tal->spacing[i];
+    d.tinv[i] = 1.0f / d.ts[i];
   }
   double *dd;
   if ((rc = dev_upload<double>(ctx, nullptr, ntal, &dd, h.owned))) { free_all(h.owned); return rc; }
@@ -1449,6 +1460,7 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
         h.dev.event_edep = t;
         h.event_cap = n_histories[s];
       }
+      h.event_n = n_histories[s];
     }
     devs[s] = h.dev;
     offs[s + 1] = offs[s] + (n_histories[s] + lp.chunk - 1) / lp.chunk;
@@ -1620,6 +1632,8 @@ int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *
                            hipMemcpyHostToDevice, ctx->stream);
   (void)hipStreamSynchronize(ctx->stream);
   free_all(tmp);
+  // the trials overwrote the per-history detector energies of the last run
+  for (auto &h : ctx->sims) h.event_n = 0;
   if (rc != GH_OK) return rc;
   *best = best_lp;
   if (best_ms) *best_ms = best_t;
@@ -1668,7 +1682,7 @@ int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n) {
   if (sim_id < 0 || sim_id >= (int32_t)ctx->sims.size()) FAIL(GH_EINVAL, "gh_get_event_edep: bad sim id");
   SimHost &h = ctx->sims[sim_id];
   if (h.det < 0 || !h.dev.event_edep) FAIL(GH_ESTATE, "gh_get_event_edep: detector scoring not enabled");
-  if (n > h.event_cap) FAIL(GH_ERANGE, "gh_get_event_edep: n beyond the last run");
+  if (n > h.event_n) FAIL(GH_ERANGE, "gh_get_event_edep: n beyond the histories of the last run");
   HIPCHK(hipSetDevice(ctx->device));
   if (n) HIPCHK(hipMemcpy(out, h.dev.event_edep, n * sizeof(float), hipMemcpyDefault));  // host or device
   return GH_OK;

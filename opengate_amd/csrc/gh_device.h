@@ -209,6 +209,29 @@ GH_DEV int cdf_index(const float *cdf, int n, float u) {
   return lo;
 }
 
+// ---------------------------------------------------------------- voxel index
+// Index of the cell of a grid (lower corner c, spacing s > 0, inv ~ 1/s) that holds the
+// point p: floor((p - c) / s) evaluated exactly for the real values of the FP32 inputs.
+// With the image origin o = c + s/2 at the centre of voxel 0 this is ITK's
+// TransformPhysicalPointToIndex, RoundHalfIntegerUp((p - o) / s): a point on a face
+// belongs to the upper voxel. floorf((p - c) * inv) alone is off by one for points
+// within about an ulp of a face (the rounding of 1/s and of the product); the estimate
+// is corrected with the exact residual p - c - k s: TwoSum gives p - c = d + e exactly,
+// and fmaf(-k, s, d) is exact wherever it is small enough to decide the sign.
+GH_DEV int cell_index(float p, float c, float s, float inv) {
+  const float d = p - c;
+  const float bv = d - p;
+  const float av = d - bv;
+  const float e = (p - av) + (-c - bv);
+  float k = floorf(d * inv);
+  if (fmaf(-k, s, d) + e < 0.0f) {
+    k -= 1.0f;
+  } else if (fmaf(-(k + 1.0f), s, d) + e >= 0.0f) {
+    k += 1.0f;
+  }
+  return (int)k;
+}
+
 // ---------------------------------------------------------------- energy grid
 struct ECache {
   int b;         // bucket

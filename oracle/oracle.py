@@ -15,11 +15,14 @@ Contents
       - FP64 Beer-Lambert line integral through a label volume (Siddon's method),
         the oracle of the uncollided primary fluence.
 
-Pinning (see tests/test_oracle_*.py): Philox against the Random123 known-answer
-vectors; digitize against numpy.digitize; uncertainty against the closed form of a
-known per-history distribution; Beer-Lambert against exp(-mu L) in homogeneous
-boxes; the transport samplers against Klein-Nishina / Thomson closed forms; energy
-conservation of whole runs.
+Pinning (tests/test_oracle_*.py, test_voxel_index.py, test_gpu_index.py): Philox against
+the Random123 known-answer vectors; digitize against numpy.digitize; uncertainty against
+the closed form of a known per-history distribution; Siddon chords against the clipped
+segment length and Beer-Lambert against exp(-mu L) in homogeneous boxes and across a
+plane interface; the table lookup mu_at against the table nodes and the FP64 mixture;
+dose_gy against a hand-worked unit conversion; the cell index against exact rational
+floor((p - c)/s) and the FP64 ITK formula on every face of small grids; the transport
+samplers against Klein-Nishina / Thomson closed forms; energy conservation of whole runs.
 """
 import ctypes as C
 import subprocess
@@ -62,6 +65,8 @@ def lib():
                                           A.FP]
         L.ref_mu_at.restype = C.c_float
         L.ref_mu_at.argtypes = [C.POINTER(A.PhysicsDesc), C.c_int, C.c_float]
+        L.ref_cell_index.restype = C.c_int
+        L.ref_cell_index.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float]
         L.ref_hu_to_labels.argtypes = [C.POINTER(C.c_int16), C.c_int64, DP, C.c_int, C.POINTER(C.c_uint16),
                                        C.POINTER(C.c_uint16)]
         _lib = L
@@ -216,10 +221,22 @@ def label_image(hu, material, voxel_materials):
     return ls[np.digitize(hu, bins=bins)], lut
 
 
+def cell_index(p, corner, spacing):
+    """The C restatement's voxel index of an FP32 coordinate: floor((p - corner) / spacing)
+    for the real values of the FP32 inputs (ref_cell_index, the kernel's cell_index)."""
+    p, c, s = np.float32(p), np.float32(corner), np.float32(spacing)
+    return lib().ref_cell_index(float(p), float(c), float(s), float(np.float32(1.0) / s))
+
+
 def itk_index(point, origin, spacing, size):
-    """itk::ImageBase::TransformPhysicalPointToIndex for an identity direction:
-    idx = RoundHalfIntegerUp((p - origin) / spacing) = floor(c + 0.5); inside iff 0 <= idx < size."""
-    c = (np.asarray(point, dtype=np.float64) - np.asarray(origin)) / np.asarray(spacing)
+    """itk::ImageBase::TransformPhysicalPointToIndex (ITK 5.4, itkImageBase.h) for an identity
+    direction, in FP64 as ITK computes it: c = PhysicalPointToIndex (p - origin) with the
+    matrix inverse(diag(spacing)) = diag(1 / spacing), idx = RoundHalfIntegerUp(c) =
+    floor(c + 0.5); inside iff 0 <= idx < size. The point and origin are those of the
+    dose image (origin = centre of voxel 0), as GateDoseActor::GetVoxelPosition passes them
+    (GateDoseActor.cpp:144-178)."""
+    c = (np.asarray(point, dtype=np.float64) - np.asarray(origin, dtype=np.float64)) * (
+        1.0 / np.asarray(spacing, dtype=np.float64))
     idx = np.floor(c + 0.5).astype(np.int64)
     inside = np.all((idx >= 0) & (idx < np.asarray(size)), axis=-1)
     return idx, inside

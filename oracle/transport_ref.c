@@ -392,14 +392,47 @@ static int voxel_of(const sim_t *S, const float p[3]) {
   return ix + S->ph->nx * (iy + S->ph->ny * iz);
 }
 
-static int tally_index(const sim_t *S, int vox, const float p[3]) {
-  if (S->aligned) return vox;
+/* floor((p - c) / s) for the real values of FP32 p, c, s: the ITK index rule
+ * RoundHalfIntegerUp((p - o) / s) with the origin o = c + s/2 at the centre of voxel 0
+ * (GateDoseActor.cpp:144-178, TransformPhysicalPointToIndex). Estimate from the rounded
+ * quotient, corrected by the sign of the exact residual p - c - k s (TwoSum of p - c,
+ * then an fmaf that is exact where the residual is small). */
+int ref_cell_index(float p, float c, float s, float inv) {
+  const float d = p - c;
+  const float bv = d - p;
+  const float av = d - bv;
+  const float e = (p - av) + (-c - bv);
+  float k = floorf(d * inv);
+  if (fmaf(-k, s, d) + e < 0.0f) {
+    k -= 1.0f;
+  } else if (fmaf(-(k + 1.0f), s, d) + e >= 0.0f) {
+    k += 1.0f;
+  }
+  return (int)k;
+}
+
+/* phantom voxel of a birth point by the exact rule (the Woodcock steps use voxel_of) */
+static int birth_voxel(const sim_t *S, const float p[3]) {
+  int ix = ref_cell_index(p[0], 0.0f, S->ph->spacing[0], S->inv_s[0]);
+  int iy = ref_cell_index(p[1], 0.0f, S->ph->spacing[1], S->inv_s[1]);
+  int iz = ref_cell_index(p[2], 0.0f, S->ph->spacing[2], S->inv_s[2]);
+  if ((unsigned)ix >= (unsigned)S->ph->nx || (unsigned)iy >= (unsigned)S->ph->ny || (unsigned)iz >= (unsigned)S->ph->nz)
+    return -1;
+  return ix + S->ph->nx * (iy + S->ph->ny * iz);
+}
+
+/* tally voxel of a deposit at p, -1 outside; an aligned tally is the phantom grid */
+static int tally_index(const sim_t *S, const float p[3]) {
   const gh_tally_desc *t = S->tal;
-  int ix = (int)floorf((p[0] - t->corner[0]) * S->tinv[0]);
-  int iy = (int)floorf((p[1] - t->corner[1]) * S->tinv[1]);
-  int iz = (int)floorf((p[2] - t->corner[2]) * S->tinv[2]);
-  if ((unsigned)ix >= (unsigned)t->nx || (unsigned)iy >= (unsigned)t->ny || (unsigned)iz >= (unsigned)t->nz) return -1;
-  return ix + t->nx * (iy + t->ny * iz);
+  int idx[3];
+  for (int i = 0; i < 3; ++i) {
+    const float c = S->aligned ? 0.0f : t->corner[i];
+    const float sp = S->aligned ? S->ph->spacing[i] : t->spacing[i];
+    idx[i] = ref_cell_index(p[i], c, sp, S->tinv[i]);
+  }
+  if ((unsigned)idx[0] >= (unsigned)t->nx || (unsigned)idx[1] >= (unsigned)t->ny || (unsigned)idx[2] >= (unsigned)t->nz)
+    return -1;
+  return idx[0] + t->nx * (idx[1] + t->ny * idx[2]);
 }
 
 static void box_hit(const sim_t *S, const float p[3], const float d[3], float *tnear, float *tfar, int *face) {
@@ -555,13 +588,13 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
   /* TLE: Woodcock collision estimator of kerma (see the kernel's TLE variant) */
   const int tle = (S->tal->flags & GH_TALLY_TLE) != 0;
   const float tle_emin = S->tal->tle_energy_min, tle_emax = S->tal->tle_energy_max;
-  int in_box = voxel_of(S, ph.p) >= 0;
+  const int bvox = birth_voxel(S, ph.p);
+  int in_box = bvox >= 0;
   if (ph.E < d->e_cut || (tle && ph.E <= tle_emin)) {
     if (in_box) {
-      int vox = voxel_of(S, ph.p);
-      int lab = S->ph->labels[vox];
+      int lab = S->ph->labels[bvox];
       float x = ph.E * ph.w;
-      score(t, &sl, tally_index(S, vox, ph.p), x, x * P->inv_rho[lab], st);
+      score(t, &sl, tally_index(S, ph.p), x, x * P->inv_rho[lab], st);
       if (det >= 0 && lab >= det) ev += x;
       history_end(t, &sl, st);
     } else {
@@ -600,7 +633,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         float mu = node_interp(d->mu_total + (size_t)lab * P->nb1, &ex);
         if (tle && ph.E <= tle_emax) {
           const float x = (ph.E * ph.w) * (node_interp(d->mu_en + (size_t)lab * P->nb1, &ex) * ex.inv_maj);
-          score(t, &sl, tally_index(S, vox, ph.p), x, x * P->inv_rho[lab], st);
+          score(t, &sl, tally_index(S, ph.p), x, x * P->inv_rho[lab], st);
         }
         if ((float)((w[2 * k + 1] >> 8) | 1u) * 0x1p-24f * ex.maj < mu) { outcome = 1; break; }
       }
@@ -609,7 +642,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         float r = uni(&rs);
         float pc = node_interp(d->p_compton + (size_t)lab * P->nb1, &ex);
         float pr = node_interp(d->p_rayleigh + (size_t)lab * P->nb1, &ex);
-        int tv = tally_index(S, vox, ph.p);
+        int tv = tally_index(S, ph.p);
         float irho = P->inv_rho[lab];
         int is_det = det >= 0 && lab >= det;
         const int tle_on = tle && ph.E <= tle_emax;
@@ -780,14 +813,14 @@ int ref_run(const gh_physics_desc *phys, const gh_phantom_desc *ph, const gh_sou
   S.src = src;
   S.tal = tal;
   int nn[3] = {ph->nx, ph->ny, ph->nz};
+  S.aligned = (tal->flags & GH_TALLY_ALIGNED) != 0;
   for (int i = 0; i < 3; ++i) {
     S.inv_s[i] = 1.0f / ph->spacing[i];
     S.L[i] = (float)((double)nn[i] * (double)ph->spacing[i]);
-    S.tinv[i] = 1.0f / tal->spacing[i];
+    S.tinv[i] = 1.0f / (S.aligned ? ph->spacing[i] : tal->spacing[i]);
   }
   S.d_ct = src->cos_theta_min - src->cos_theta_max;
   S.d_phi = src->phi_max - src->phi_min;
-  S.aligned = (tal->flags & GH_TALLY_ALIGNED) != 0;
   size_t ntal = (size_t)tal->nx * tal->ny * tal->nz;
   work_t *W = (work_t *)calloc((size_t)n_threads, sizeof(work_t));
   pthread_t *T = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));

@@ -7,7 +7,6 @@ import pytest
 import opengate_amd as gate
 from oracle import oracle as O
 from opengate_amd import phantoms as P
-from opengate_amd.actors.dose import GY_FACTOR
 
 pytestmark = pytest.mark.gpu
 
@@ -79,7 +78,7 @@ def test_user_script_matches_oracle(tmp_path, two_sources):
     assert np.max(np.abs(dose.edep_squared.image.array - edep2)) <= 1e-9 * edep2.max()
     # dose = edep / rho / V in Gy; uncertainty = the reference formula
     rho = tables.density.astype(np.float64)[ph.labels]
-    exp_dose = np.divide(edep, rho, out=np.zeros_like(edep), where=rho > 0) * GY_FACTOR / 8.0
+    exp_dose = O.dose_gy(edep, rho, 8.0)
     assert np.allclose(dose.dose.image.array, exp_dose, rtol=1e-9, atol=1e-30)
     np.testing.assert_allclose(dose.edep_uncertainty.image.array, O.uncertainty(edep, edep2, n_tot),
                                rtol=1e-6, atol=1e-12)
