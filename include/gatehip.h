@@ -145,7 +145,8 @@ typedef struct gh_launch {
                                 tables in LDS, 1024 threads; else 256 threads) */
   int32_t grid;              /* workgroups; 0 = automatic */
   int32_t refill_interval;   /* max step-phase iterations (2 Woodcock steps each) per loop */
-  int32_t chunk;             /* histories per work-queue chunk */
+  int32_t chunk;             /* histories per work-queue chunk (0: automatic, about 4 chunks per wave,
+                                64..4096) */
   int32_t slow_min;          /* parked lanes (collision / exit / world / dead) that end a step phase (0: 52) */
   int32_t refill_min;        /* dead lanes that trigger a refill (clamped to slow_min; 0: 12) */
 } gh_launch;

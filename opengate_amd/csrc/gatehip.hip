@@ -1434,7 +1434,7 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   if (!n_histories) FAIL(GH_EINVAL, "gh_run: n_histories is null");
   HIPCHK(hipSetDevice(ctx->device));
   const int ns = (int)ctx->sims.size();
-  gh_launch lp{256, 0, 64, 4096, 52, 12};
+  gh_launch lp{256, 0, 64, 0, 52, 12};
   if (launch) {
     if (launch->block > 0) lp.block = launch->block;
     if (launch->grid > 0) lp.grid = launch->grid;
@@ -1463,7 +1463,6 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
       h.event_n = n_histories[s];
     }
     devs[s] = h.dev;
-    offs[s + 1] = offs[s] + (n_histories[s] + lp.chunk - 1) / lp.chunk;
   }
   if (!ctx->d_sims || ctx->d_sims_cap < (size_t)ns) {
     if (ctx->d_sims) { (void)hipFree(ctx->d_sims); ctx->d_sims = nullptr; }
@@ -1477,18 +1476,11 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
     HIPCHK(hipMalloc(&ctx->d_counters, 12 * sizeof(unsigned long long)));
     HIPCHK(hipMalloc(&ctx->d_energies, 4 * sizeof(double)));
   }
-  HIPCHK(hipMemcpyAsync(ctx->d_sims, devs.data(), ns * sizeof(SimDev), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemcpyAsync(ctx->d_chunk_off, offs.data(), (ns + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemsetAsync(ctx->d_next, 0, sizeof(unsigned long long), ctx->stream));
-  HIPCHK(hipMemsetAsync(ctx->d_counters, 0, 12 * sizeof(unsigned long long), ctx->stream));
-  HIPCHK(hipMemsetAsync(ctx->d_energies, 0, 4 * sizeof(double), ctx->stream));
 
   Launch L;
   L.sims = ctx->d_sims;
   L.n_sims = ns;
   L.chunk_offsets = ctx->d_chunk_off;
-  L.total_chunks = offs[ns];
-  L.chunk = (uint32_t)lp.chunk;
   L.next_chunk = ctx->d_next;
   L.seed = seed;
   L.max_iter = lp.refill_interval;
@@ -1546,6 +1538,25 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, lp.block, shmem));
     lp.grid = std::max(1, ctx->n_cu) * std::max(1, per_cu);
   }
+  // work-queue chunk: automatic = about four chunks per wave of the launch (64 to 4096
+  // histories, a multiple of 64), so that a short run still spreads over every wave; which
+  // wave runs a history does not change its result
+  if (lp.chunk <= 0) {
+    uint64_t total = 0;
+    for (int s = 0; s < ns; ++s) total += n_histories[s];
+    const uint64_t waves = (uint64_t)lp.grid * (uint64_t)(lp.block / 64);
+    uint64_t c = total / std::max<uint64_t>(1, 4 * waves);
+    c = std::min<uint64_t>(4096, std::max<uint64_t>(64, c & ~(uint64_t)63));
+    lp.chunk = (int32_t)c;
+  }
+  for (int s = 0; s < ns; ++s) offs[s + 1] = offs[s] + (n_histories[s] + lp.chunk - 1) / lp.chunk;
+  L.total_chunks = offs[ns];
+  L.chunk = (uint32_t)lp.chunk;
+  HIPCHK(hipMemcpyAsync(ctx->d_sims, devs.data(), ns * sizeof(SimDev), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(ctx->d_chunk_off, offs.data(), (ns + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_next, 0, sizeof(unsigned long long), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_counters, 0, 12 * sizeof(unsigned long long), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_energies, 0, 4 * sizeof(double), ctx->stream));
   ctx->last_launch = lp;
   ctx->last_stage = stage;
   HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
@@ -1608,7 +1619,7 @@ int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *
   int rc = GH_OK;
   std::vector<uint64_t> first(ctx->sims.size(), 0);
   for (const Cand &c : cands) {
-    gh_launch lp{c.block, 0, c.iter, 4096, c.slow, c.refill};
+    gh_launch lp{c.block, 0, c.iter, 0, c.slow, c.refill};
     gh_counters out{};
     rc = gh_run(ctx, n_trial, first.data(), seed, &lp, &out);
     if (rc != GH_OK) break;
@@ -1617,6 +1628,7 @@ int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *
       best_lp = ctx->last_launch;
       best_lp.block = c.block;
       best_lp.grid = 0;
+      best_lp.chunk = 0;  // automatic: sized by the production run's own history count
     }
   }
   for (auto &pr : saved) {

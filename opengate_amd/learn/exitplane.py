@@ -192,9 +192,11 @@ class ExitPlaneGenerator:
         return self.history
 
     # ------------------------------------------------------------------ generation
-    def generate(self, n, seed, first=0, out=None, chunk=1 << 22, weight=1.0):
+    def generate(self, n, seed, first=0, out=None, chunk=1 << 22, weight=1.0, events=None):
         """n exit-plane photons as 8 device float32 arrays (x y z u v w E weight) in the
-        exit-plane frame. Photon i depends only on (seed, first + i)."""
+        exit-plane frame. Photon i depends only on (seed, first + i). events: optional list
+        that receives a (start, end) pair of CUDA events recorded around each network forward
+        (on the current stream, where the forward runs)."""
         torch = _torch()
         if self.qmap is None:
             raise RuntimeError("generator not trained or loaded")
@@ -214,10 +216,16 @@ class ExitPlaneGenerator:
         for c0 in range(0, n, m):
             k = min(m, n - c0)
             self.noise(k, seed, first=first + c0, out=X[:k])
+            if events is not None:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
             if fused:
                 self.mlp.infer_fused(X[:k], Z[:k])
             else:
                 self.mlp.infer(X[:k], Z[:k], bufs)
+            if events is not None:
+                e1.record()
+                events.append((e0, e1))
             ptrs = (A.FP * 8)(*[C.cast(C.c_void_p(a.data_ptr() + 4 * c0), A.FP) for a in out])
             rc = self.lib.gh_exitplane_decode(_ptr(Z), Z.shape[1], k, _ptr(self.qmap.qtab), self.qmap.n_knots,
                                               self.qmap.g_max, focal, self.plane_z, zero, float(weight), ptrs,
