@@ -221,10 +221,25 @@ enum { GH_EPI_STORE = 0,      /* C32 = acc */
        GH_EPI_DSILU = 3 };    /* C32 = acc * silu'(zprev) (optional), C16 = bf16(same) */
 /* In GH_EPI_STORE mode a long reduction over few output tiles (weight gradients) is split
  * across workgroups and the slices are summed in a fixed order (library-owned workspace,
- * allocated on first use). */
+ * allocated on first use, one per device and stream). */
 int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
                  int32_t M, int32_t N, int32_t K, const float *bias, const float *zprev, int64_t ldz,
                  float *C32, void *C16, int64_t ldc, int32_t mode, void *stream);
+
+/* Allocation-free variant of gh_gemm_bf16 (same arguments, kernels and results). No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It calls no
+ * hipMalloc or hipFree: the split-K slices of weight-gradient shapes go to a static device
+ * array of GH_GEMM_NOALLOC_WS_BYTES that each GPU holds until the process ends. A split into
+ * S slices is only made for fewer than 512 output tiles of 64 x 64 and with S <= 1024 / tiles,
+ * so the slices (S x M x N x 4 bytes) never exceed 1024 x 64 x 64 x 4 bytes = 16 MiB, the
+ * array's size; a larger request would fail with GH_ERANGE. Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other host threads) race.
+ * gh_last_error(NULL) returns the message of this host thread's last failed call. */
+#define GH_GEMM_NOALLOC_WS_BYTES (16u << 20)
+int gh_gemm_bf16_noalloc(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
+                         int32_t M, int32_t N, int32_t K, const float *bias, const float *zprev, int64_t ldz,
+                         float *C32, void *C16, int64_t ldc, int32_t mode, void *stream);
 
 /* Inference of a whole dense network in one kernel (activations stay in LDS): layers
  * Z = X W_l + b_l, SiLU between layers, FP32 output of the last layer. X: M rows of bf16

@@ -12,6 +12,7 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libgatehip.so"
 
 GH_OK, GH_EINVAL, GH_EHIP, GH_ESTATE, GH_ERANGE = 0, 1, 2, 3, 4
+GEMM_NOALLOC_WS_BYTES = 16 << 20  # GH_GEMM_NOALLOC_WS_BYTES
 POS = {"point": 0, "box": 1, "sphere": 2, "ellipsoid": 3, "disc": 4, "cylinder": 5, "phsp": 6, "voxels": 7}
 DIR = {"iso": 0, "momentum": 1, "focused": 2}
 ENE = {"mono": 0, "spectrum_discrete": 1, "spectrum_histogram": 2, "range": 3}
@@ -143,6 +144,9 @@ SIGNATURES = {
     "gh_gemm_bf16": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_int32,
                                C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                C.c_int64, C.c_int32, C.c_void_p]),
+    "gh_gemm_bf16_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                                       C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                                       C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
     "gh_gaussian_bf16": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
                                    C.c_void_p]),
     "gh_exitplane_decode": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int32, C.c_float, FP,

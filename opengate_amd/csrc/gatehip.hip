@@ -1164,7 +1164,10 @@ void gh_destroy(gh_ctx *ctx) {
   delete ctx;
 }
 
-const char *gh_last_error(const gh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+// message of the last failed call of a context-free entry point of this host thread (mlp.hip)
+extern "C" __attribute__((visibility("hidden"))) const char *gh_free_call_error();
+
+const char *gh_last_error(const gh_ctx *ctx) { return ctx ? ctx->err.c_str() : gh_free_call_error(); }
 
 int gh_device_info(gh_ctx *ctx, int32_t *n_cu, int32_t *lds_bytes, char *name, int32_t name_len) {
   if (!ctx) return GH_EINVAL;

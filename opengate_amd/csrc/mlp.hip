@@ -26,7 +26,9 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <string>
 
 #include "../../include/gatehip.h"
 
@@ -199,16 +201,23 @@ __global__ void __launch_bounds__(256) splitk_reduce_kernel(const float *__restr
   }
 }
 
-// split-K workspace, grown on demand (one per process; the calls are stream-ordered)
-std::mutex g_ws_mutex;
-float *g_ws = nullptr;
-size_t g_ws_bytes = 0;
+// split-K plan of a GEMM: S slices of kc (a multiple of BK) reduction elements each;
+// S = 1 means no split. Weight gradients (few output tiles, long reduction) are split so
+// that the launch fills the chip; the slices are then summed in a fixed order.
+void splitk_plan(int M, int N, int K, int mode, int &S, int &kc) {
+  const int tiles = ((N + BN - 1) / BN) * ((M + BM - 1) / BM);
+  S = 1;
+  kc = ((K + BK - 1) / BK) * BK;
+  if (mode == GH_EPI_STORE && tiles < 512 && K >= 2048) {
+    S = std::max(1, std::min((K + 511) / 512, std::max(1, 1024 / tiles)));
+    kc = (K + S - 1) / S;
+    kc = (kc + BK - 1) / BK * BK;
+    S = (K + kc - 1) / kc;
+  }
+}
 
-}  // namespace
-
-extern "C" int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
-                            int32_t M, int32_t N, int32_t K, const float *bias, const float *zprev, int64_t ldz,
-                            float *C32, void *C16, int64_t ldc, int32_t mode, void *stream) {
+int check_gemm_args(const void *A, const void *B, int M, int N, int K, const float *bias, const float *zprev,
+                    float *C32, void *C16, int mode) {
   if (!A || !B || M < 0 || N < 0 || K < 0) return GH_EINVAL;
   if (mode < GH_EPI_STORE || mode > GH_EPI_DSILU) return GH_EINVAL;
   if (mode == GH_EPI_STORE && !C32) return GH_EINVAL;
@@ -216,44 +225,151 @@ extern "C" int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void 
   if ((mode == GH_EPI_BIAS_SILU || mode == GH_EPI_DSILU) && !C16) return GH_EINVAL;
   if (mode == GH_EPI_BIAS && !C16 && !C32) return GH_EINVAL;
   if (mode == GH_EPI_DSILU && !zprev) return GH_EINVAL;
-  if (M == 0 || N == 0) return GH_OK;
-  const int tiles = ((N + BN - 1) / BN) * ((M + BM - 1) / BM);
-  // weight gradients (few output tiles, long reduction): split the reduction so that the
-  // launch fills the chip, then sum the slices in a fixed order
-  int S = 1;
-  if (mode == GH_EPI_STORE && tiles < 512 && K >= 2048) {
-    S = std::min((K + 511) / 512, std::max(1, 1024 / tiles));
-    S = std::max(S, 1);
-  }
+  return GH_OK;
+}
+
+// Launches the GEMM of a plan; ws holds S*M*N floats when S > 1 (unused otherwise).
+int launch_gemm(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn, int M, int N,
+                int K, const float *bias, const float *zprev, int64_t ldz, float *C32, void *C16, int64_t ldc,
+                int mode, int S, int kc, float *ws, hipStream_t stream) {
   if (S > 1) {
-    int kc = (K + S - 1) / S;
-    kc = (kc + BK - 1) / BK * BK;
-    S = (K + kc - 1) / kc;
-    const size_t need = (size_t)S * M * N * sizeof(float);
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if (g_ws_bytes < need) {
-      if (g_ws) (void)hipFree(g_ws);
-      g_ws = nullptr;
-      g_ws_bytes = 0;
-      if (hipMalloc(&g_ws, need) != hipSuccess) return GH_EHIP;
-      g_ws_bytes = need;
-    }
     dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, S);
-    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)A, sam, sak,
-                       (const __bf16 *)B, sbk, sbn, M, N, K, bias, zprev, ldz, g_ws, (__bf16 *)nullptr,
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, stream, (const __bf16 *)A, sam, sak,
+                       (const __bf16 *)B, sbk, sbn, M, N, K, bias, zprev, ldz, ws, (__bf16 *)nullptr,
                        (int64_t)N, mode, kc, (int64_t)M * N);
     if (hipGetLastError() != hipSuccess) return GH_EHIP;
     const int64_t total = (int64_t)M * N;
     const int rgrid = (int)std::min<int64_t>((total + 255) / 256, 4096);
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rgrid), dim3(256), 0, (hipStream_t)stream, g_ws, S,
-                       (int64_t)M * N, M, N, C32, ldc);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rgrid), dim3(256), 0, stream, ws, S, (int64_t)M * N, M, N, C32,
+                       ldc);
     return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
   }
   dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
-  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)A, sam, sak,
-                     (const __bf16 *)B, sbk, sbn, M, N, K, bias, zprev, ldz, C32, (__bf16 *)C16, ldc, mode,
-                     ((K + BK - 1) / BK) * BK, (int64_t)0);
+  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, stream, (const __bf16 *)A, sam, sak, (const __bf16 *)B,
+                     sbk, sbn, M, N, K, bias, zprev, ldz, C32, (__bf16 *)C16, ldc, mode, kc, (int64_t)0);
   return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
+}
+
+// Split-K workspaces of gh_gemm_bf16, one per (device, stream) and grown on demand: calls
+// on one stream are ordered, so a workspace is never used by two kernels at once; before
+// a workspace is replaced its stream is drained, so no queued kernel still reads it.
+struct WsKey {
+  int device;
+  hipStream_t stream;
+  bool operator<(const WsKey &o) const {
+    return device != o.device ? device < o.device : (uintptr_t)stream < (uintptr_t)o.stream;
+  }
+};
+struct WsBuf {
+  float *p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::map<WsKey, WsBuf> g_ws;
+
+// message of the last failed call of the allocation-free entry point (per host thread)
+thread_local std::string g_free_err;
+
+}  // namespace
+
+// Scratch memory of gh_gemm_bf16_noalloc: the split-K slices live in this array, which
+// every GPU the library runs on holds once, at its own address, until the process ends.
+static __device__ __align__(256) float scratch[GH_GEMM_NOALLOC_WS_BYTES / sizeof(float)];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+float *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (hipGetSymbolAddress once per device,
+// with that device current; the host-side name of the array is not a device address).
+int scratch_address(float **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    g_free_err = "gh_gemm_bf16_noalloc: hipGetDevice failed";
+    return GH_EHIP;
+  }
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    g_free_err = "gh_gemm_bf16_noalloc: device ordinal beyond the library's limit";
+    return GH_ERANGE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_free_err = std::string("gh_gemm_bf16_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return GH_EHIP;
+    }
+    g_scratch_ptr[dev] = (float *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GH_OK;
+}
+}  // namespace
+
+// internal to the library (not part of the C-ABI): read by gh_last_error(NULL)
+extern "C" __attribute__((visibility("hidden"))) const char *gh_free_call_error() { return g_free_err.c_str(); }
+
+extern "C" int gh_gemm_bf16(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
+                            int32_t M, int32_t N, int32_t K, const float *bias, const float *zprev, int64_t ldz,
+                            float *C32, void *C16, int64_t ldc, int32_t mode, void *stream) {
+  const int rc = check_gemm_args(A, B, M, N, K, bias, zprev, C32, C16, mode);
+  if (rc != GH_OK) return rc;
+  if (M == 0 || N == 0) return GH_OK;
+  int S, kc;
+  splitk_plan(M, N, K, mode, S, kc);
+  float *ws = nullptr;
+  if (S > 1) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return GH_EHIP;
+    const size_t need = (size_t)S * M * N * sizeof(float);
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    WsBuf &b = g_ws[WsKey{dev, (hipStream_t)stream}];
+    if (b.bytes < need) {
+      if (b.p) {
+        if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return GH_EHIP;
+        (void)hipFree(b.p);
+      }
+      b.p = nullptr;
+      b.bytes = 0;
+      if (hipMalloc(&b.p, need) != hipSuccess) return GH_EHIP;
+      b.bytes = need;
+    }
+    ws = b.p;
+  }
+  return launch_gemm(A, sam, sak, B, sbk, sbn, M, N, K, bias, zprev, ldz, C32, C16, ldc, mode, S, kc, ws,
+                     (hipStream_t)stream);
+}
+
+extern "C" int gh_gemm_bf16_noalloc(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk,
+                                    int64_t sbn, int32_t M, int32_t N, int32_t K, const float *bias,
+                                    const float *zprev, int64_t ldz, float *C32, void *C16, int64_t ldc,
+                                    int32_t mode, void *stream) {
+  const int rc = check_gemm_args(A, B, M, N, K, bias, zprev, C32, C16, mode);
+  if (rc != GH_OK) {
+    g_free_err = "gh_gemm_bf16_noalloc: invalid argument";
+    return rc;
+  }
+  if (M == 0 || N == 0) return GH_OK;
+  int S, kc;
+  splitk_plan(M, N, K, mode, S, kc);
+  float *ws = nullptr;
+  if (S > 1) {
+    const size_t need = (size_t)S * M * N * sizeof(float);
+    if (need > (size_t)GH_GEMM_NOALLOC_WS_BYTES) {
+      g_free_err = "gh_gemm_bf16_noalloc: the split-K workspace of this shape (" + std::to_string(need) +
+                   " bytes) exceeds the library's static scratch (" +
+                   std::to_string((size_t)GH_GEMM_NOALLOC_WS_BYTES) + " bytes)";
+      return GH_ERANGE;
+    }
+    const int r = scratch_address(&ws);
+    if (r != GH_OK) return r;
+  }
+  const int r = launch_gemm(A, sam, sak, B, sbk, sbn, M, N, K, bias, zprev, ldz, C32, C16, ldc, mode, S, kc, ws,
+                            (hipStream_t)stream);
+  if (r != GH_OK) g_free_err = "gh_gemm_bf16_noalloc: kernel launch failed";
+  return r;
 }
 
 // ---------------------------------------------------------------------------------------
