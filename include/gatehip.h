@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GH_ABI_VERSION 4
+#define GH_ABI_VERSION 5
 
 enum {
   GH_OK = 0,
@@ -149,6 +149,8 @@ typedef struct gh_launch {
                                 64..4096) */
   int32_t slow_min;          /* parked lanes (collision / exit / world / dead) that end a step phase (0: 52) */
   int32_t refill_min;        /* dead lanes that trigger a refill (clamped to slow_min; 0: 12) */
+  int32_t scatter_tries;     /* Compton / Rayleigh sampler tries of a photon per slow phase before it
+                                waits for the next one (0: 2); results do not depend on it */
 } gh_launch;
 
 typedef struct gh_counters {

@@ -82,7 +82,7 @@ class Blur(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid", C.c_int32), ("refill_interval", C.c_int32), ("chunk", C.c_int32),
-                ("slow_min", C.c_int32), ("refill_min", C.c_int32)]
+                ("slow_min", C.c_int32), ("refill_min", C.c_int32), ("scatter_tries", C.c_int32)]
 
 
 class Counters(C.Structure):
@@ -190,7 +190,7 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.gh_abi_version() != 4:
+    if lib.gh_abi_version() != 5:
         raise RuntimeError("libgatehip.so ABI version mismatch")
     if path is None:
         _LIB = lib

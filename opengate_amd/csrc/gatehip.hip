@@ -88,6 +88,7 @@ struct Launch {
   int max_iter;   // step-phase iterations (2 tentative steps each) between slow phases
   int slow_min;   // lanes waiting for slow work that end a step phase
   int refill_min; // dead lanes that trigger a refill (<= slow_min)
+  int scatter_tries; // Compton / Rayleigh sampler tries per slow phase before a photon waits
   Phys P;
   unsigned long long *counters;   // [12]
   double *energies;               // [4]
@@ -416,18 +417,10 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
   p.wt = wt;
 }
 
-// phantom voxel of a birth point (exact cell index, as the tally's), -1 outside the box
-__device__ __forceinline__ int birth_voxel(const SimDev &s, const Hot &h, float x, float y, float z) {
-  const int ix = cell_index(x, 0.0f, s.ts_ph[0], h.isx), iy = cell_index(y, 0.0f, s.ts_ph[1], h.isy),
-            iz = cell_index(z, 0.0f, s.ts_ph[2], h.isz);
-  if ((unsigned)ix < (unsigned)h.nx && (unsigned)iy < (unsigned)h.ny && (unsigned)iz < (unsigned)h.nz)
-    return ix + h.nx * (iy + h.ny * iz);
-  return -1;
-}
-
 // ---------------------------------------------------------------------- kernel
 // lane modes
-enum { M_DEAD = 0, M_STEP = 1, M_COLLIDE = 2, M_EXIT = 3, M_WORLD = 4 };
+// (every mode from M_COLLIDE on is slow work)
+enum { M_DEAD = 0, M_STEP = 1, M_COLLIDE = 2, M_EXIT = 3, M_WORLD = 4, M_COMPTON = 5, M_RAYLEIGH = 6 };
 
 // global history id of the lane's current history (event-stream counter words)
 __device__ __forceinline__ uint64_t history_of(const Rng &g) { return ((uint64_t)g.c1 << 32) | g.c0; }
@@ -437,31 +430,67 @@ __device__ __forceinline__ float u01w(uint32_t w) { return (float)((w >> 8) | 1u
 
 // Persistent transport kernel. The loop of a wave has three phases, each entered by
 // all 64 lanes together (the conditions are ballots, so they are wave-uniform):
-//   refill  dead lanes get new histories (source sampling) once at least refill_min
-//           lanes are dead, or when no lane can step;
+//   refill  dead lanes take photons from the wave's birth queue once at least refill_min
+//           lanes are dead, or when no lane can step. The queue (64 entries per wave in
+//           LDS) is filled by a birth pass of all 64 lanes at once: each lane samples the
+//           source for one history of the wave's work chunk, and a photon born outside the
+//           phantom is carried to it through the world; photons that stop at birth (below
+//           the tracking cut, or missing the phantom) are finished in the pass and never
+//           take a lane. So the source sampling runs on full waves however few lanes wait.
 //   step    every lane in M_STEP takes up to two Woodcock steps per iteration from one
 //           Philox block of its step stream; a real collision (M_COLLIDE) or a box exit
 //           (M_EXIT) parks the lane; the phase ends when slow_min lanes are parked;
 //   slow    parked lanes process their collision, exit or world event together, using
-//           the event stream of their history.
+//           the event stream of their history. The Compton and Rayleigh samplers are
+//           rejection methods: a lane runs at most `scatter_tries` tries per slow phase and
+//           keeps its remaining tries (M_COMPTON / M_RAYLEIGH) for the next one, so a wave
+//           does not run as many tries as its slowest lane needs.
 // Batching the slow work this way keeps the wave converged: a collision in one lane no
 // longer stalls 63 lanes that only want to step.
-// LDS layout (floats) of the staged tables; STAGE 1 stages mu + majorant, STAGE 2 all
+// After the birth of a history (source sampling, and the world step to the phantom for a
+// photon born outside it) its event stream continues at the next unused Philox block, so
+// that a queue entry carries the block counter instead of buffered words.
+// LDS layout (floats) of the staged tables; STAGE 1 stages mu + majorant, STAGE 2 also the
+// form-factor tables of the samplers (the branching ratios, read once per collision, stay
+// in global memory, cached)
 struct LdsLayout {
-  int mu, maj, pcpr, sinc, rcdf, rx, total;
+  int mu, maj, sinc, rcdf, rx, total;
 };
 __host__ __device__ inline LdsLayout lds_layout(int n_mat, int nb1, int nx, int stage) {
   LdsLayout o;
   o.mu = 0;
   o.maj = n_mat * nb1;
-  int end = o.maj + nb1;
-  o.pcpr = (end + 1) & ~1;
-  o.sinc = o.pcpr + 2 * n_mat * nb1;
+  const int end = o.maj + nb1;
+  o.sinc = end;
   o.rcdf = o.sinc + n_mat * nx;
   o.rx = o.rcdf + n_mat * nx;
   o.total = stage >= 2 ? o.rx + nx : end;
   if (stage == 0) o.total = 0;
   return o;
+}
+
+// birth queue entry: 10 words per photon (x y z u v w E weight, history, meta) in LDS,
+// [word][workgroup lane] so that the 64 slots of a wave are consecutive
+constexpr int QWORDS = 10;
+constexpr uint32_t META_COLLIDE = 1u << 16;  // born outside, interacts in the world first
+constexpr int META_SIM_SHIFT = 17;           // simulation id in bits 17..31 (c3 in 0..15)
+constexpr int MAX_SIMS = 1 << 15;
+
+// label of voxel `vox` of simulation s (both label layouts)
+__device__ __forceinline__ int sim_label(const SimDev &s, int vox) {
+  const uint32_t l4 = s.lbits == 4 ? 1u : 0u;
+  const auto *g = (const __attribute__((address_space(1))) uint8_t *)s.labels;
+  const uint32_t b = g[(uint32_t)vox >> l4];
+  return (int)__builtin_amdgcn_ubfe(b, ((uint32_t)vox & l4) << 2, 8u >> l4);
+}
+
+// phantom voxel of a point (exact cell index, as the tally's), -1 outside the box
+__device__ __forceinline__ int sim_voxel(const SimDev &s, float x, float y, float z) {
+  const int ix = cell_index(x, 0.0f, s.ts_ph[0], s.inv_s[0]), iy = cell_index(y, 0.0f, s.ts_ph[1], s.inv_s[1]),
+            iz = cell_index(z, 0.0f, s.ts_ph[2], s.inv_s[2]);
+  if ((unsigned)ix < (unsigned)s.nx && (unsigned)iy < (unsigned)s.ny && (unsigned)iz < (unsigned)s.nz)
+    return ix + s.nx * (iy + s.ny * iz);
+  return -1;
 }
 
 // TLE: dose by the Woodcock collision estimator of kerma instead of analog deposits
@@ -473,8 +502,7 @@ __host__ __device__ inline LdsLayout lds_layout(int n_mat, int nb1, int nx, int 
 // cut) deposit their energy where they are and stop. Single-simulation launches only.
 // The batched form keeps a few more per-lane values (its simulation's grid) and the
 // kerma-estimator form a few more live values in the step loop; the waves-per-EU bound
-// holds both at 128 VGPRs (4 waves per SIMD), which they meet without spilling, as the
-// analog single-simulation forms do by themselves.
+// holds both at 128 VGPRs (4 waves per SIMD).
 template <bool BATCHED, int STAGE, int BLOCK, bool TLE>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu((BATCHED || TLE) ? 4 : 1)))
 transport_kernel(Launch L) {
@@ -485,21 +513,21 @@ transport_kernel(Launch L) {
   T.mu = P.mu; T.maj = P.maj; T.pcpr = P.pcpr; T.rx = P.rx; T.rcdf = P.rcdf; T.sinc = P.sinc;
   T.rguide = P.rguide;
   if (STAGE >= 1) {
+    // staged by the threads of the launch (a launch may use fewer than BLOCK)
+    const int nt = (int)blockDim.x;
     const LdsLayout o = lds_layout(P.n_mat, NB1, P.nx, STAGE);
     const int n_mu = P.n_mat * NB1;
-    for (int i = threadIdx.x; i < n_mu; i += BLOCK) lds[o.mu + i] = P.mu[i];
-    for (int i = threadIdx.x; i < NB1; i += BLOCK) lds[o.maj + i] = P.maj[i];
+    for (int i = threadIdx.x; i < n_mu; i += nt) lds[o.mu + i] = P.mu[i];
+    for (int i = threadIdx.x; i < NB1; i += nt) lds[o.maj + i] = P.maj[i];
     T.mu = lds + o.mu;
     T.maj = lds + o.maj;
     if (STAGE >= 2) {
-      for (int i = threadIdx.x; i < 2 * n_mu; i += BLOCK) lds[o.pcpr + i] = P.pcpr[i];
       const int n_x = P.n_mat * P.nx;
-      for (int i = threadIdx.x; i < n_x; i += BLOCK) {
+      for (int i = threadIdx.x; i < n_x; i += nt) {
         lds[o.sinc + i] = P.sinc[i];
         lds[o.rcdf + i] = P.rcdf[i];
       }
-      for (int i = threadIdx.x; i < P.nx; i += BLOCK) lds[o.rx + i] = P.rx[i];
-      T.pcpr = lds + o.pcpr;
+      for (int i = threadIdx.x; i < P.nx; i += nt) lds[o.rx + i] = P.rx[i];
       T.sinc = lds + o.sinc;
       T.rcdf = lds + o.rcdf;
       T.rx = lds + o.rx;
@@ -512,11 +540,14 @@ transport_kernel(Launch L) {
   // lanes of a wave never add to one LDS address at once (64 same-address FP64 atomics
   // serialise in the LDS unit and hold up the table reads of every wave of the CU)
   __shared__ double s_en[N_ENERGIES * BLOCK];
+  // birth queues: wave w owns columns [64 w, 64 w + 64)
+  __shared__ uint32_t s_q[QWORDS * BLOCK];
   if (threadIdx.x < N_COUNTERS) s_cnt[threadIdx.x] = 0ull;
   for (int k = 0; k < N_ENERGIES; ++k) s_en[k * BLOCK + threadIdx.x] = 0.0;
   __syncthreads();
   LaneStats st;
   st.steps = 0u; st.rej = 0u; st.cnt = s_cnt; st.en = s_en + threadIdx.x; st.stride = BLOCK;
+  const uint32_t qbase = threadIdx.x & ~63u;
 
   Hot hot;
   // descriptor of simulation i (always 0 in the single-simulation forms)
@@ -532,151 +563,194 @@ transport_kernel(Launch L) {
   ec.b = 0; ec.frac = 0.0f; ec.maj = 1.0f; ec.inv_maj = 1.0f;
   Rng rng;
   // the key is the run's seed in every lane from the start, so the compiler sees one
-  // wave-uniform key (scalar registers) instead of a per-lane value: 20 fewer VGPRs, which
-  // keeps the batched form within its 128 without spilling
+  // wave-uniform key (scalar registers) instead of a per-lane value
   rng_init(rng, L.seed, 0, 0);
   uint32_t sblock = 0;           // step-stream block counter of the current history
   SqCache sq = {};
   float ev_edep = 0.0f;
   int mode = M_DEAD;
   int cvox = 0;                  // voxel of a pending collision (-1: in the world material)
-  int clab = 0;                  // its label (kept from the step: no second global read)
+  int clab = 0;                  // its material (kept from the step: no second global read)
+  int ntry = 0;                  // rejected tries of a pending Compton / Rayleigh sampling
 
   // wave-uniform work state
   int cur_sim = 0;
   uint64_t cur_next = 0, cur_end = 0;
   bool have_work = true;
+  uint32_t qn = 0;               // photons in this wave's birth queue
   const uint32_t slow_min = (uint32_t)L.slow_min;
   const uint32_t refill_min = (uint32_t)L.refill_min;
+  const int scatter_tries = L.scatter_tries;
 
-  // world transport outside the phantom box (homogeneous material or vacuum); run at
-  // refill for photons born outside the box (all new lanes together) and in the slow
-  // phase for photons that left it. The event stream of the history supplies the
-  // interaction distance either way, so where it runs does not change the result.
-  // The single-simulation forms compute the reciprocal direction once per slow event and
-  // share it between slab tests; the batched form, which has no registers to spare,
-  // recomputes the same values at each use.
-  auto recip = [&](const InvDir &iv) { return BATCHED ? inv_dir(ph.u, ph.v, ph.w) : iv; };
-  auto world_step = [&](const SimDev &s, bool &kill, const InvDir &iv) {
-        // world outside the phantom box (homogeneous material or vacuum)
-        float tn, tf;
-        int face;
-        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, recip(iv), s.L[0], s.L[1], s.L[2], tn, tf, face);
-        const bool hit = (tn < tf) && (tf > EXIT_EPS);
-        const float tin = fmaxf(tn, 0.0f);
-        if (s.world_mat < 0) {
-          if (hit) {
-            ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
-            mode = M_STEP;
-          } else {
-            st.inc(C_ESCAPED);
-            st.add_e(E_ESC, (double)(ph.E * ph.wt));
-            kill = true;
-          }
+  // world transport outside the phantom box (homogeneous material or vacuum) of photon p
+  // with energy cache e; run in the birth pass for photons born outside the box and in the
+  // slow phase for photons that left it. Returns the new mode: M_STEP (entered the box),
+  // M_COLLIDE (interaction in the world material, p at the interaction point) or M_DEAD
+  // (escaped). The single-simulation forms pass the reciprocal direction; the batched
+  // form, which has no registers to spare, recomputes it.
+  auto world_step = [&](const SimDev &s, Photon &p, Rng &g, const ECache &e, const InvDir &iv0) -> int {
+    auto iv = [&]() { return BATCHED ? inv_dir(p.u, p.v, p.w) : iv0; };
+    float tn, tf;
+    int face;
+    ray_box(p.x, p.y, p.z, p.u, p.v, p.w, iv(), s.L[0], s.L[1], s.L[2], tn, tf, face);
+    const bool hit = (tn < tf) && (tf > EXIT_EPS);
+    const float tin = fmaxf(tn, 0.0f);
+    if (s.world_mat < 0) {
+      if (hit) {
+        p.x = fmaf(tin, p.u, p.x); p.y = fmaf(tin, p.v, p.y); p.z = fmaf(tin, p.w, p.z);
+        return M_STEP;
+      }
+      st.inc(C_ESCAPED);
+      st.add_e(E_ESC, (double)(p.E * p.wt));
+      return M_DEAD;
+    }
+    const int wm = s.world_mat;
+    const float muw = lerp_node(mu_tab + wm * NB1, e);
+    const float tw = world_exit(s, p.x, p.y, p.z, p.u, p.v, p.w, iv());
+    const float d = (muw > 0.0f) ? (-glog(u01(g)) / muw) : INFINITY;
+    if (hit && d >= tin && tin < tw) {
+      p.x = fmaf(tin, p.u, p.x); p.y = fmaf(tin, p.v, p.y); p.z = fmaf(tin, p.w, p.z);
+      return M_STEP;
+    }
+    if (d >= tw) {
+      st.inc(C_ESCAPED);
+      st.add_e(E_ESC, (double)(p.E * p.wt));
+      return M_DEAD;
+    }
+    p.x = fmaf(d, p.u, p.x); p.y = fmaf(d, p.v, p.y); p.z = fmaf(d, p.w, p.z);
+    st.inc(C_WORLD);
+    return M_COLLIDE;
+  };
+
+  // Birth pass: all 64 lanes sample one history each from the wave's chunk; photons that
+  // still need transport go into the wave's queue (empty when the pass starts).
+  auto birth_pass = [&]() {
+    while (have_work && cur_next >= cur_end) {
+      unsigned long long c = 0;
+      if (lane_id() == 0) c = atomicAdd(L.next_chunk, 1ull);
+      c = __shfl(c, 0, 64);
+      if (c >= L.total_chunks) { have_work = false; break; }
+      int s = 0;
+      while (s + 1 < L.n_sims && L.chunk_offsets[s + 1] <= c) ++s;
+      cur_sim = s;
+      const uint64_t local = (c - L.chunk_offsets[s]) * (uint64_t)L.chunk;
+      cur_next = local;
+      const uint64_t e = local + L.chunk;
+      const uint64_t nh = sim(s).n_hist;
+      cur_end = e < nh ? e : nh;
+    }
+    if (!have_work) return;
+    const uint64_t left = cur_end - cur_next;
+    const uint32_t avail = (uint32_t)(left < 64u ? left : 64u);
+    const uint32_t lane = lane_id();
+    const SimDev &s = sim(cur_sim);
+    int bm = M_DEAD;
+    Photon b;
+    b.x = b.y = b.z = b.u = b.v = b.w = b.E = b.wt = 0.0f;
+    uint32_t bh = 0, bc3 = 0;
+    if (lane < avail) {
+      bh = (uint32_t)(cur_next + lane);
+      Rng g;
+      rng_init(g, L.seed, s.first_hist + bh, (uint32_t)(BATCHED ? cur_sim : 0));
+      sample_source(s, g, bh, b);
+      st.inc(C_HIST);
+      const int bvox = sim_voxel(s, b.x, b.y, b.z);
+      if (b.E < P.e_cut || (TLE && b.E <= tle_emin)) {
+        // below the tracking cut from the start: deposit locally (inside) and stop; the
+        // history's only deposit is its (voxel, sum) entry
+        float ev = 0.0f;
+        if (bvox >= 0) {
+          const int lab = sim_label(s, bvox);
+          const float x = b.E * b.wt;
+          const int tv = tally_voxel(s, b.x, b.y, b.z);
+          if (lab >= s.det_label_min && s.det_label_min >= 0) ev = x;
+          if (s.event_edep) s.event_edep[bh] = ev;
+          if (tv >= 0) flush_slot(tally_out(s), tv, x, x * P.inv_rho[lab], 1u, st);
+          else st.add_e(E_OUT, (double)x);
         } else {
-          const int wm = s.world_mat;
-          const float muw = lerp_node(mu_tab + wm * NB1, ec);
-          const float tw = world_exit(s, ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, recip(iv));
-          const float d = (muw > 0.0f) ? (-glog(u01(rng)) / muw) : INFINITY;
-          if (hit && d >= tin && tin < tw) {
-            ph.x = fmaf(tin, ph.u, ph.x); ph.y = fmaf(tin, ph.v, ph.y); ph.z = fmaf(tin, ph.w, ph.z);
-            mode = M_STEP;
-          } else if (d >= tw) {
-            st.inc(C_ESCAPED);
-            st.add_e(E_ESC, (double)(ph.E * ph.wt));
-            kill = true;
-          } else {
-            ph.x = fmaf(d, ph.u, ph.x); ph.y = fmaf(d, ph.v, ph.y); ph.z = fmaf(d, ph.w, ph.z);
-            st.inc(C_WORLD);
-            mode = M_COLLIDE;
-            cvox = -1;
-          }
+          if (s.event_edep) s.event_edep[bh] = ev;
+          st.add_e(E_WORLD, (double)(b.E * b.wt));
         }
-      };
+        st.inc(C_CUT);
+      } else if (bvox >= 0) {
+        bm = M_STEP;
+      } else {
+        ECache be;
+        ecache_set(be, b.E, maj_tab);
+        bm = world_step(s, b, g, be, BATCHED ? InvDir{0.0f, 0.0f, 0.0f} : inv_dir(b.u, b.v, b.w));
+        if (bm == M_DEAD && s.event_edep) s.event_edep[bh] = 0.0f;
+      }
+      bc3 = g.c3;  // the event stream goes on at the next unused block
+    }
+    cur_next += avail;
+    const bool keep = bm != M_DEAD;
+    const uint64_t m = __ballot(keep);
+    if (keep) {
+      const uint32_t col = qbase + qn + mbcnt(m);
+      const float f[8] = {b.x, b.y, b.z, b.u, b.v, b.w, b.E, b.wt};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s_q[k * BLOCK + col] = __float_as_uint(f[k]);
+      s_q[8 * BLOCK + col] = bh;
+      s_q[9 * BLOCK + col] = (bc3 & 0xFFFFu) | (bm == M_COLLIDE ? META_COLLIDE : 0u) |
+                             ((uint32_t)(BATCHED ? cur_sim : 0) << META_SIM_SHIFT);
+    }
+    qn += (uint32_t)__popcll(m);
+    __builtin_amdgcn_wave_barrier();
+  };
+
   for (;;) {
     // ------------------------------------------------------------- refill
     {
       uint64_t dead = __ballot(mode == M_DEAD);
       const uint32_t n_step = (uint32_t)__popcll(__ballot(mode == M_STEP));
-      if (have_work && dead != 0ull && ((uint32_t)__popcll(dead) >= refill_min || n_step == 0u)) {
-        // one sampling pass per refill: lanes whose new photon dies at birth (a source
-        // photon that misses the phantom) wait for the next refill instead of a second,
-        // nearly empty pass through the source sampling
-        bool sampled = false;
-        while (dead != 0ull && have_work && !sampled) {
-          if (cur_next >= cur_end) {
-            unsigned long long c = 0;
-            if (lane_id() == 0) c = atomicAdd(L.next_chunk, 1ull);
-            c = __shfl(c, 0, 64);
-            if (c >= L.total_chunks) { have_work = false; break; }
-            int s = 0;
-            while (s + 1 < L.n_sims && L.chunk_offsets[s + 1] <= c) ++s;
-            cur_sim = s;
-            const uint64_t local = (c - L.chunk_offsets[s]) * (uint64_t)L.chunk;
-            cur_next = local;
-            const uint64_t e = local + L.chunk;
-            const uint64_t nh = sim(s).n_hist;
-            cur_end = e < nh ? e : nh;
+      if (dead != 0ull && (qn > 0u || have_work) &&
+          ((uint32_t)__popcll(dead) >= refill_min || n_step == 0u)) {
+        while (dead != 0ull && (qn > 0u || have_work)) {
+          if (qn == 0u) {
+            birth_pass();
             continue;
           }
-          const uint64_t avail = cur_end - cur_next;
           const uint32_t nd = (uint32_t)__popcll(dead);
-          const uint32_t take = (uint32_t)(avail < nd ? avail : nd);
+          const uint32_t take = qn < nd ? qn : nd;
           if (mode == M_DEAD) {
             const uint32_t rank = mbcnt(dead);
             if (rank < take) {
-              const SimDev &s = sim(cur_sim);
+              const uint32_t col = qbase + qn - 1u - rank;
+              ph.x = __uint_as_float(s_q[0 * BLOCK + col]); ph.y = __uint_as_float(s_q[1 * BLOCK + col]);
+              ph.z = __uint_as_float(s_q[2 * BLOCK + col]); ph.u = __uint_as_float(s_q[3 * BLOCK + col]);
+              ph.v = __uint_as_float(s_q[4 * BLOCK + col]); ph.w = __uint_as_float(s_q[5 * BLOCK + col]);
+              ph.E = __uint_as_float(s_q[6 * BLOCK + col]); ph.wt = __uint_as_float(s_q[7 * BLOCK + col]);
+              ph.hist = s_q[8 * BLOCK + col];
+              const uint32_t meta = s_q[9 * BLOCK + col];
+              ph.sim = BATCHED ? (int)(meta >> META_SIM_SHIFT) : 0;
+              const SimDev &s = sim(ph.sim);
               if (BATCHED) hot.load(s);
-              ph.sim = BATCHED ? cur_sim : 0;
-              ph.hist = (uint32_t)(cur_next + rank);
               rng_init(rng, L.seed, s.first_hist + ph.hist, (uint32_t)ph.sim);
+              rng.c3 = meta & 0xFFFFu;
               sblock = 0u;
-              sample_source(s, rng, ph.hist, ph);
-              st.inc(C_HIST);
               ev_edep = 0.0f;
               sq.n = 0;
-              const int bvox = birth_voxel(s, hot, ph.x, ph.y, ph.z);
-              const bool inside = bvox >= 0;
-              if (ph.E < P.e_cut || (TLE && ph.E <= tle_emin)) {
-                // below the tracking cut from the start: deposit locally (inside) and stop
-                if (inside) {
-                  const int lab = hot.label(bvox);
-                  const float x = ph.E * ph.wt;
-                  deposit(s, sq, tally_voxel(s, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
-                  if (lab >= s.det_label_min && s.det_label_min >= 0) ev_edep += x;
-                  sq_flush_all(s, sq, st);
-                } else {
-                  st.add_e(E_WORLD, (double)(ph.E * ph.wt));
-                }
-                st.inc(C_CUT);
-                if (s.event_edep) s.event_edep[ph.hist] = ev_edep;
+              ecache_set(ec, ph.E, maj_tab);
+              if (meta & META_COLLIDE) {
+                mode = M_COLLIDE;
+                cvox = -1;
               } else {
-                ecache_set(ec, ph.E, maj_tab);
-                mode = inside ? M_STEP : M_WORLD;
-                if (mode == M_WORLD) {
-                  bool kill = false;
-                  world_step(s, kill, BATCHED ? InvDir{0.0f, 0.0f, 0.0f} : inv_dir(ph.u, ph.v, ph.w));
-                  if (kill) {
-                    if (s.event_edep) s.event_edep[ph.hist] = 0.0f;
-                    mode = M_DEAD;
-                  }
-                }
+                mode = M_STEP;
               }
             }
           }
-          cur_next += take;
-          sampled = take != 0u;
+          qn -= take;
           dead = __ballot(mode == M_DEAD);
         }
       }
-      if (!have_work && __ballot(mode != M_DEAD) == 0ull) break;
+      if (!have_work && qn == 0u && __ballot(mode != M_DEAD) == 0ull) break;
     }
 
     // ------------------------------------------------------------- step phase
     for (int it = 0; it < L.max_iter; ++it) {
       const uint32_t n_step = (uint32_t)__popcll(__ballot(mode == M_STEP));
       const uint32_t n_slow = (uint32_t)__popcll(
-          __ballot(mode == M_COLLIDE || mode == M_EXIT || mode == M_WORLD || (have_work && mode == M_DEAD)));
+          __ballot(mode >= M_COLLIDE || ((have_work || qn > 0u) && mode == M_DEAD)));
       if (n_step == 0u || n_slow >= slow_min) break;
       if (mode == M_STEP) {
         // the event stream's counter words hold the global history id and the simulation
@@ -716,20 +790,21 @@ transport_kernel(Launch L) {
     }
 
     // ------------------------------------------------------------- slow phase
-    // order: box exit -> world event -> collision. A world interaction becomes a
-    // collision with the world material and no tally voxel (cvox = -1), so box and
-    // world collisions share one copy of the interaction code.
-    if (mode == M_COLLIDE || mode == M_EXIT || mode == M_WORLD) {
+    // order: box exit -> world event -> collision (branching) -> scattering samplers. A
+    // world interaction becomes a collision with the world material and no tally voxel
+    // (cvox = -1), so box and world collisions share one copy of the interaction code.
+    if (mode >= M_COLLIDE) {
       const SimDev &s = sim(BATCHED ? (int)rng.c2 : 0);
       bool kill = false;
       // one set of divisions for the box exit and the world transport that follows it
       InvDir iv = {0.0f, 0.0f, 0.0f};
-      if (!BATCHED && mode != M_COLLIDE) iv = inv_dir(ph.u, ph.v, ph.w);
+      if (!BATCHED && (mode == M_EXIT || mode == M_WORLD)) iv = inv_dir(ph.u, ph.v, ph.w);
       if (mode == M_EXIT) {
         // leaves the phantom box: exit point from the last position inside
         float tn, tf;
         int face;
-        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, recip(iv), s.L[0], s.L[1], s.L[2], tn, tf, face);
+        ray_box(ph.x, ph.y, ph.z, ph.u, ph.v, ph.w, BATCHED ? inv_dir(ph.u, ph.v, ph.w) : iv, s.L[0], s.L[1],
+                s.L[2], tn, tf, face);
         const float t = fmaxf(tf, 0.0f);
         const float ex = fmaf(t, ph.u, ph.x), ey = fmaf(t, ph.v, ph.y), ez = fmaf(t, ph.w, ph.z);
         if (s.rec_face >= 0 && (face == s.rec_face || s.rec_face == GH_FACE_ALL)) {
@@ -767,41 +842,75 @@ transport_kernel(Launch L) {
           mode = M_WORLD;
         }
       }
-      if (mode == M_WORLD && !kill) world_step(s, kill, iv);
+      if (mode == M_WORLD && !kill) {
+        mode = world_step(s, ph, rng, ec, iv);
+        if (mode == M_DEAD) kill = true;
+        else if (mode == M_COLLIDE) cvox = -1;
+      }
+      // interaction: branching, then the process (photoelectric at once, the scattering
+      // samplers in bounded tries); `fin` marks a finished interaction this pass
+      bool fin = false;
+      bool comp = false;
+      float x_dep = 0.0f;   // energy given to the medium at this interaction
+      const bool in_world = cvox < 0;
       if (mode == M_COLLIDE && !kill) {
-        const bool in_world = cvox < 0;
-        const int lab = in_world ? s.world_mat : clab;
+        clab = in_world ? s.world_mat : clab;
         const float r = u01(rng);
         float pc, pr;
-        branching(T, NB1, lab, ec, pc, pr);
-        const int tv = in_world ? -1 : tally_voxel(s, ph.x, ph.y, ph.z);
-        const float irho = P.inv_rho[lab];
-        const bool det = !in_world && s.det_label_min >= 0 && lab >= s.det_label_min;
-        // electrons of photons scored by the kerma estimator give nothing locally
-        const bool tle_on = TLE && !in_world && ph.E <= tle_emax;
-        mode = in_world ? M_WORLD : M_STEP;
-        float x_dep = 0.0f;   // energy given to the medium at this collision
+        branching(T, NB1, clab, ec, pc, pr);
+        ntry = 0;
         if (r < pc) {
           if (!in_world) st.inc(C_COMPTON);
-          float q, c;
-          sample_compton(rng, T, P.nx, lab, ph.E, q, c, st.rej);
+          mode = M_COMPTON;
+        } else if (r < pr) {
+          if (!in_world) st.inc(C_RAYLEIGH);
+          mode = M_RAYLEIGH;
+        } else {
+          if (!in_world) st.inc(C_PHOTO);
+          x_dep = (TLE && !in_world && ph.E <= tle_emax) ? 0.0f : ph.E * ph.wt;
+          kill = true;
+          fin = true;
+        }
+      }
+      if (mode == M_COMPTON && !fin) {
+        float q = 1.0f, c = 1.0f;
+        bool done = false;
+        for (int t = 0; t < scatter_tries && !done; ++t) {
+          if (compton_try(rng, T, P.nx, clab, ph.E, q, c)) {
+            done = true;
+          } else {
+            st.rej++;
+            if (++ntry == MAX_REJECT) { q = 1.0f; c = 1.0f; done = true; }
+          }
+        }
+        if (done) {
+          c = fmaxf(c, -1.0f);
           const float e1 = ph.E / q;
-          x_dep = tle_on ? 0.0f : (ph.E - e1) * ph.wt;
+          // electrons of photons scored by the kerma estimator give nothing locally
+          x_dep = (TLE && !in_world && ph.E <= tle_emax) ? 0.0f : (ph.E - e1) * ph.wt;
           float cp, sp;
           azimuth(rng, cp, sp);
           rotate(ph.u, ph.v, ph.w, c, cp, sp);
           ph.E = e1;
-        } else if (r < pr) {
-          if (!in_world) st.inc(C_RAYLEIGH);
-          const float c = sample_rayleigh(rng, T, P.nx, lab, ph.E);
+          fin = true;
+          comp = true;
+        }
+      }
+      if (mode == M_RAYLEIGH && !fin) {
+        float c = 1.0f;
+        bool done = false;
+        for (int t = 0; t < scatter_tries && !done; ++t) {
+          if (rayleigh_try(rng, T, P.nx, clab, ph.E, c) || ++ntry == MAX_REJECT) done = true;
+        }
+        if (done) {
           float cp, sp;
           azimuth(rng, cp, sp);
           rotate(ph.u, ph.v, ph.w, c, cp, sp);
-        } else {
-          if (!in_world) st.inc(C_PHOTO);
-          x_dep = tle_on ? 0.0f : ph.E * ph.wt;
-          kill = true;
+          fin = true;
         }
+      }
+      if (fin) {
+        if (!kill) mode = in_world ? M_WORLD : M_STEP;
         // local deposition: electron energy, and the photon itself below the tracking cut
         float x_cut = 0.0f;
         if (!kill && (ph.E < P.e_cut || (TLE && !in_world && ph.E <= tle_emin))) {
@@ -813,6 +922,9 @@ transport_kernel(Launch L) {
           if (x_dep != 0.0f) st.add_e(E_WORLD, (double)x_dep);
           if (x_cut != 0.0f) st.add_e(E_WORLD, (double)x_cut);
         } else {
+          const int tv = tally_voxel(s, ph.x, ph.y, ph.z);
+          const float irho = P.inv_rho[clab];
+          const bool det = s.det_label_min >= 0 && clab >= s.det_label_min;
           if (x_dep != 0.0f) {
             deposit(s, sq, tv, x_dep, x_dep * irho, st);
             if (det) ev_edep += x_dep;
@@ -822,7 +934,7 @@ transport_kernel(Launch L) {
             if (det) ev_edep += x_cut;
           }
         }
-        if (!kill && r < pc) ecache_set(ec, ph.E, maj_tab);
+        if (!kill && comp) ecache_set(ec, ph.E, maj_tab);
       }
       if (kill) {
         // the event record goes first: a read of the descriptor placed after the flush's
@@ -1437,7 +1549,7 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   if (!n_histories) FAIL(GH_EINVAL, "gh_run: n_histories is null");
   HIPCHK(hipSetDevice(ctx->device));
   const int ns = (int)ctx->sims.size();
-  gh_launch lp{256, 0, 64, 0, 52, 12};
+  gh_launch lp{256, 0, 64, 0, 52, 12, 2};
   if (launch) {
     if (launch->block > 0) lp.block = launch->block;
     if (launch->grid > 0) lp.grid = launch->grid;
@@ -1445,7 +1557,10 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
     if (launch->chunk > 0) lp.chunk = launch->chunk;
     if (launch->slow_min > 0) lp.slow_min = launch->slow_min;
     if (launch->refill_min > 0) lp.refill_min = launch->refill_min;
+    if (launch->scatter_tries > 0) lp.scatter_tries = launch->scatter_tries;
   }
+  if (ns > MAX_SIMS) FAIL(GH_ERANGE, "gh_run: more than 32768 simulations in one launch");
+  if (lp.scatter_tries > MAX_REJECT) lp.scatter_tries = MAX_REJECT;
 
   std::vector<uint64_t> offs(ns + 1, 0);
   std::vector<SimDev> devs(ns);
@@ -1489,15 +1604,21 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   L.max_iter = lp.refill_interval;
   L.slow_min = std::max(1, std::min(64, lp.slow_min));
   L.refill_min = std::max(1, std::min(L.slow_min, lp.refill_min));
+  L.scatter_tries = lp.scatter_tries;
   L.P = ctx->P;
   L.counters = ctx->d_counters;
   L.energies = ctx->d_energies;
   // staging level and block: all tables in LDS shared by one 1024-thread workgroup per CU
   // when they fit, else mu + majorant in LDS with 256-thread workgroups, else none. The
-  // budgets leave room for the kernel's static LDS (per-lane energy slots and counters:
-  // 32 KiB at 1024 threads, 8 KiB at 256) within the 160 KiB of a CU.
-  const size_t static_big = (size_t)N_ENERGIES * 1024 * sizeof(double) + N_COUNTERS * sizeof(unsigned long long);
-  const size_t budget_big = 160 * 1024 - static_big - 512, budget_small = 36 * 1024;
+  // budgets leave room for the kernel's static LDS (per-lane energy slots, birth queues and
+  // counters: 72 KiB at 1024 threads, 18 KiB at 256) within the 160 KiB of a CU; a
+  // 256-thread workgroup keeps to 40 KiB so that four of them share a CU.
+  auto static_lds = [](size_t block) {
+    return (size_t)N_ENERGIES * block * sizeof(double) + (size_t)QWORDS * block * sizeof(uint32_t) +
+           N_COUNTERS * sizeof(unsigned long long);
+  };
+  const size_t budget_big = 160 * 1024 - static_lds(1024) - 512;
+  const size_t budget_small = 40 * 1024 - static_lds(256) - 256;
   int stage, big;
   if (launch && launch->block > 0) {
     big = lp.block > 256;
@@ -1622,7 +1743,7 @@ int gh_autotune(gh_ctx *ctx, const uint64_t *n_trial, uint64_t seed, gh_launch *
   int rc = GH_OK;
   std::vector<uint64_t> first(ctx->sims.size(), 0);
   for (const Cand &c : cands) {
-    gh_launch lp{c.block, 0, c.iter, 0, c.slow, c.refill};
+    gh_launch lp{c.block, 0, c.iter, 0, c.slow, c.refill, 0};
     gh_counters out{};
     rc = gh_run(ctx, n_trial, first.data(), seed, &lp, &out);
     if (rc != GH_OK) break;

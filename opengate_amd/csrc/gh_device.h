@@ -300,6 +300,47 @@ GH_DEV void branching(const Tabs &T, int nb1, int lab, const ECache &c, float &p
   pr = fmaf(c.frac, b.y - a.y, a.y);
 }
 
+// One try of the Compton sampler below (Kahn's method, then acceptance with S(q)/Z): the
+// transport kernel runs a bounded number of tries per pass and resumes a photon whose
+// tries were all rejected in a later pass, so that a wave does not wait for its slowest
+// lane. The draws of a photon are the same either way. Returns true if accepted.
+GH_DEV bool compton_try(Rng &g, const Tabs &T, int nx, int lab, float E, float &q, float &c) {
+  const float alpha = E / MEC2;
+  const float a2 = alpha + alpha;
+  const float onep = 1.0f + a2;
+  const float thr = onep / (9.0f + a2);
+  const float r1 = u01(g), r2 = u01(g), r3 = u01(g);
+  const float t = fmaf(a2, r2, 1.0f);
+  const bool low = r1 <= thr;
+  q = low ? t : onep / t;
+  const float iq = 1.0f / q;
+  c = 1.0f - (q - 1.0f) / alpha;
+  const bool acc = low ? (r3 <= 4.0f * (iq - iq * iq)) : (r3 <= 0.5f * fmaf(c, c, iq));
+  if (!acc) return false;
+  const float kk = E * KPM;
+  const float two_k2 = (2.0f * kk) * kk;
+  const float x = two_k2 * (1.0f - c);
+  const float s = qgrid_value(T.sinc + lab * nx, nx, x);
+  return u01(g) <= s;
+}
+
+// One try of the Rayleigh sampler below: returns true if accepted; c is the try's cosine.
+GH_DEV bool rayleigh_try(Rng &g, const Tabs &T, int nx, int lab, float E, float &c) {
+  const float kk = E * KPM;
+  const float two_k2 = (2.0f * kk) * kk;
+  const float xmax = two_k2 + two_k2;
+  const float *G = T.rcdf + lab * nx;
+  const float gmax = qgrid_value(G, nx, xmax);
+  const float t = u01(g) * gmax;
+  const int j = bsearch_le_guided(G, T.rguide + lab * (RAY_GUIDE + 1), nx, t);
+  const float g0 = G[j], g1 = G[j + 1];
+  const float dg = g1 - g0;
+  const float x0 = T.rx[j], x1 = T.rx[j + 1];
+  const float x = (dg > 0.0f) ? fmaf((t - g0) / dg, x1 - x0, x0) : x0;
+  c = fmaxf(1.0f - x / two_k2, -1.0f);
+  return u01(g) <= 0.5f * fmaf(c, c, 1.0f);
+}
+
 // Klein-Nishina by Kahn's rejection method (H. Kahn, "Applications of Monte Carlo",
 // RM-1237-AEC, 1956) followed by acceptance with S(q)/Z. Returns q = E/E' and cos(theta).
 GH_DEV void sample_compton(Rng &g, const Tabs &T, int nx, int lab, float E, float &q_out, float &c_out,

@@ -335,12 +335,13 @@ class GpuEngine:
         self._check(self.lib.gh_reset_tally(self.ctx, int(sim_id)), "gh_reset_tally")
 
     def run(self, n_histories, seed, first_history=None, block=0, grid=0, refill_interval=64, chunk=0,
-            slow_min=52, refill_min=12):
+            slow_min=52, refill_min=12, scatter_tries=0):
         n = np.ascontiguousarray(np.atleast_1d(n_histories), dtype=np.uint64)
         if len(n) != len(self.sims):
             raise GhError("one history count per simulation is required")
         f = np.zeros_like(n) if first_history is None else np.ascontiguousarray(np.atleast_1d(first_history), dtype=np.uint64)
-        lp = A.Launch(int(block), int(grid), int(refill_interval), int(chunk), int(slow_min), int(refill_min))
+        lp = A.Launch(int(block), int(grid), int(refill_interval), int(chunk), int(slow_min), int(refill_min),
+                      int(scatter_tries))
         out = A.Counters()
         self._check(self.lib.gh_run(self.ctx, n.ctypes.data_as(C.POINTER(C.c_uint64)),
                                     f.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_uint64(int(seed)),
@@ -358,7 +359,7 @@ class GpuEngine:
         self._check(self.lib.gh_autotune(self.ctx, n.ctypes.data_as(C.POINTER(C.c_uint64)), C.c_uint64(int(seed)),
                                          C.byref(best), C.byref(ms)), "gh_autotune")
         kw = {"block": best.block, "grid": best.grid, "refill_interval": best.refill_interval, "chunk": best.chunk,
-              "slow_min": best.slow_min, "refill_min": best.refill_min}
+              "slow_min": best.slow_min, "refill_min": best.refill_min, "scatter_tries": best.scatter_tries}
         return kw, ms.value
 
     def get_tally(self, sim_id=0):

@@ -590,6 +590,11 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
   const float tle_emin = S->tal->tle_energy_min, tle_emax = S->tal->tle_energy_max;
   const int bvox = birth_voxel(S, ph.p);
   int in_box = bvox >= 0;
+  /* After the birth (source sampling, and for a photon born outside the phantom its
+   * first world step) the event stream continues at the next unused Philox block: the
+   * words left in the current block are dropped (the GPU kernel hands a born photon to a
+   * lane with only the block counter). */
+  int birth = !in_box;
   if (ph.E < d->e_cut || (tle && ph.E <= tle_emin)) {
     if (in_box) {
       int lab = S->ph->labels[bvox];
@@ -604,6 +609,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
     if (o->event_edep) o->event_edep[h] = ev;
     return;
   }
+  if (in_box) rs.pos = 4;
   eidx_t ex;
   energy_index(P, ph.E, &ex);
   const int wm = S->ph->world_material;
@@ -724,11 +730,13 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         }
         for (int i = 0; i < 3; ++i) ph.p[i] = fmaf(tin, ph.d[i], ph.p[i]);
         in_box = 1;
+        if (birth) { rs.pos = 4; birth = 0; }
         continue;
       }
       float muw = node_interp(d->mu_total + (size_t)wm * P->nb1, &ex);
       float tw = world_distance(S, ph.p, ph.d);
       float dist = (muw > 0.0f) ? (-ref_log(uni(&rs)) / muw) : INFINITY;
+      if (birth) { rs.pos = 4; birth = 0; }
       if (hit && dist >= tin && tin < tw) {
         for (int i = 0; i < 3; ++i) ph.p[i] = fmaf(tin, ph.d[i], ph.p[i]);
         in_box = 1;

@@ -71,6 +71,22 @@ def test_box_source_lines_and_launch_shapes(engine):
         compare(c, rc, g, r)
 
 
+def test_small_workgroups_stage_every_table_entry(engine):
+    """A launch with fewer threads than its kernel's workgroup size (64 of 256) stages the
+    whole table: right after a full-size launch has left another table set in LDS, its
+    results still equal the oracle's (each launch must write every entry it reads)."""
+    ph = small_phantom(24, 3.0)
+    src = SourceSpec(pos_trans=centre(ph), energy=0.1405)
+    tal = aligned_tally(ph)
+    other = tables(("G4_LUNG_ICRP", "G4_BONE_CORTICAL_ICRP"), 0.1405)
+    c, g, rc, r = run_both(engine, other, ph, src, tal, 50_000, 5, block=256)
+    compare(c, rc, g, r)
+    T = tables(("G4_WATER", "G4_BONE_COMPACT_ICRU"), 0.1405)
+    for launch in ({"block": 64, "grid": 256}, {"block": 192, "grid": 300}):
+        c, g, rc, r = run_both(engine, T, ph, src, tal, 50_000, 6, **launch)
+        compare(c, rc, g, r)
+
+
 def test_cone_from_outside_air_world_histogram_nonaligned(engine):
     T = tables(("G4_WATER", "G4_BONE_CORTICAL_ICRP", "G4_AIR"), 0.125)
     ph = small_phantom(32, 2.0)
