@@ -218,9 +218,9 @@ int gh_hu_to_labels(gh_ctx *ctx, const int16_t *hu, int64_t n_voxels, const doub
  * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]; C row pitch ldc. Device pointers,
  * `stream` is a hipStream_t (NULL = default stream). Epilogue modes: */
 enum { GH_EPI_STORE = 0,      /* C32 = acc */
-       GH_EPI_BIAS_SILU = 1,  /* C32 = acc + bias (optional), C16 = silu(acc + bias) */
+       GH_EPI_BIAS_RELU = 1,  /* C32 = acc + bias (optional), C16 = relu(acc + bias) */
        GH_EPI_BIAS = 2,       /* C32 = acc + bias and/or C16 = bf16(acc + bias) */
-       GH_EPI_DSILU = 3 };    /* C32 = acc * silu'(zprev) (optional), C16 = bf16(same) */
+       GH_EPI_DRELU = 3 };    /* C32 = acc * relu'(zprev) (optional), C16 = bf16(same) */
 /* In GH_EPI_STORE mode a long reduction over few output tiles (weight gradients) is split
  * across workgroups and the slices are summed in a fixed order (library-owned workspace,
  * allocated on first use, one per device and stream). */
@@ -244,7 +244,7 @@ int gh_gemm_bf16_noalloc(const void *A, int64_t sam, int64_t sak, const void *B,
                          float *C32, void *C16, int64_t ldc, int32_t mode, void *stream);
 
 /* Inference of a whole dense network in one kernel (activations stay in LDS): layers
- * Z = X W_l + b_l, SiLU between layers, FP32 output of the last layer. X: M rows of bf16
+ * Z = X W_l + b_l, ReLU between layers, FP32 output of the last layer. X: M rows of bf16
  * (pitch ldx, first k_in <= 32 columns used); wt[l]: device bf16 weights transposed,
  * [N_l][K_l] with K contiguous, the first layer padded to K = 32 and the last to N = 16
  * (zeros); bias[l]: device FP32 [N_l]; hidden widths all equal to `hidden` (128 or 256);

@@ -1,13 +1,13 @@
 """Dense MLP trained and evaluated with the library's BF16 MFMA GEMM (gh_gemm_bf16).
 
 Every matrix product of the forward and backward passes runs in the hand-written
-kernel (csrc/mlp.hip) with FP32 accumulation; bias, SiLU and the SiLU derivative are
+kernel (csrc/mlp.hip) with FP32 accumulation; bias, ReLU and the ReLU derivative are
 fused into its epilogues. PyTorch holds the tensors (device memory, the FP32 master
 weights, the Adam state and the loss), nothing more.
 
-Layer l: Z_l = X_l W_l + b_l (FP32), X_{l+1} = bf16(silu(Z_l)) for hidden layers,
+Layer l: Z_l = X_l W_l + b_l (FP32), X_{l+1} = bf16(relu(Z_l)) for hidden layers,
 the last layer is linear. Backward: G_L = dLoss/dZ_L, dW_l = X_l^T G_l,
-db_l = sum_m G_l, G_{l-1} = (G_l W_l^T) * silu'(Z_{l-1}).
+db_l = sum_m G_l, G_{l-1} = (G_l W_l^T) * relu'(Z_{l-1}).
 """
 import ctypes as C
 
@@ -15,7 +15,7 @@ import numpy as np
 
 from .. import _abi as A
 
-EPI_STORE, EPI_BIAS_SILU, EPI_BIAS, EPI_DSILU = 0, 1, 2, 3
+EPI_STORE, EPI_BIAS_RELU, EPI_BIAS, EPI_DRELU = 0, 1, 2, 3
 
 
 def _torch():
@@ -91,7 +91,7 @@ class MLP:
             if l < L - 1:
                 Y = torch.empty(M, N, device=self.dev, dtype=torch.bfloat16)
                 self._gemm(xs[-1], K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c32=Z, c16=Y, ldc=N,
-                           mode=EPI_BIAS_SILU)
+                           mode=EPI_BIAS_RELU)
                 xs.append(Y)
             else:
                 self._gemm(xs[-1], K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c32=Z, ldc=N, mode=EPI_BIAS)
@@ -111,7 +111,7 @@ class MLP:
             K, N = self.psizes[l], self.psizes[l + 1]
             if l < L - 1:
                 Y = bufs[l % 2].view(-1)[: M * N].view(M, N)
-                self._gemm(cur, K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c16=Y, ldc=N, mode=EPI_BIAS_SILU)
+                self._gemm(cur, K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c16=Y, ldc=N, mode=EPI_BIAS_RELU)
                 cur = Y
             else:
                 self._gemm(cur, K, 1, self.W16[l], N, 1, M, N, K, bias=self.b[l], c32=out, ldc=out.stride(0),
@@ -183,9 +183,9 @@ class MLP:
             if l > 0:
                 Gp32 = torch.empty(M, K, device=self.dev, dtype=torch.float32)
                 Gp16 = torch.empty(M, K, device=self.dev, dtype=torch.bfloat16)
-                # G_{l-1} = (G W^T) * silu'(Z_{l-1}) : (M x N) (N x K)
+                # G_{l-1} = (G W^T) * relu'(Z_{l-1}) : (M x N) (N x K)
                 self._gemm(G16, N, 1, self.W16[l], 1, N, M, K, N, zprev=self._zs[l - 1], ldz=K, c32=Gp32, c16=Gp16,
-                           ldc=K, mode=EPI_DSILU)
+                           ldc=K, mode=EPI_DRELU)
                 G32, G16 = Gp32, Gp16
         return dW, db
 

@@ -1,7 +1,7 @@
 """FP64 numpy restatement of the dense MLP (TEST INFRASTRUCTURE ONLY).
 
-Mirrors opengate_amd/learn/mlp.py: Z_l = X_l W_l + b_l, X_{l+1} = silu(Z_l), last layer
-linear; backward dW_l = X_l^T G_l, db_l = sum G_l, G_{l-1} = (G_l W_l^T) silu'(Z_{l-1}).
+Mirrors opengate_amd/learn/mlp.py: Z_l = X_l W_l + b_l, X_{l+1} = relu(Z_l), last layer
+linear; backward dW_l = X_l^T G_l, db_l = sum G_l, G_{l-1} = (G_l W_l^T) relu'(Z_{l-1}).
 With bf16=True the values the GPU rounds to bfloat16 (layer inputs, weights, the
 gradients fed to the GEMMs) are rounded the same way (round to nearest even).
 Pinned in tests/test_mlp_oracle.py by central finite differences of the FP64 model.
@@ -16,13 +16,12 @@ def to_bf16(a):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def relu(z):
+    return np.maximum(z, 0.0)
 
 
-def dsilu(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def drelu(z):
+    return (np.asarray(z) > 0.0).astype(np.float64)
 
 
 def forward(Ws, bs, x, bf16=True):
@@ -32,7 +31,7 @@ def forward(Ws, bs, x, bf16=True):
         z = xs[-1] @ r(W) + b
         zs.append(z)
         if l < len(Ws) - 1:
-            xs.append(r(silu(z)))
+            xs.append(r(relu(z)))
     return zs[-1], xs, zs
 
 
@@ -46,5 +45,5 @@ def backward(Ws, xs, zs, g_out, bf16=True):
         dW[l] = xs[l].T @ g16
         db[l] = g32.sum(0)
         if l > 0:
-            g32 = (g16 @ r(Ws[l]).T) * dsilu(zs[l - 1])
+            g32 = (g16 @ r(Ws[l]).T) * drelu(zs[l - 1])
     return dW, db

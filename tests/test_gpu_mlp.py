@@ -17,7 +17,7 @@ def _torch():
                                    (128, 136, 8192)])
 def test_gemm_modes_and_layouts(M, N, K):
     torch = _torch()
-    from opengate_amd.learn.mlp import EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU, EPI_STORE, MLP
+    from opengate_amd.learn.mlp import EPI_BIAS, EPI_BIAS_RELU, EPI_DRELU, EPI_STORE, MLP
 
     print(f"[gpu case] gemm M={M} N={N} K={K}", flush=True)
     net = MLP([8, 8])  # only for the GEMM entry point
@@ -46,17 +46,17 @@ def test_gemm_modes_and_layouts(M, N, K):
     # fused epilogues
     z = torch.zeros(M, N, device="cuda")
     y = torch.zeros(M, N, device="cuda", dtype=torch.bfloat16)
-    net._gemm(a, K, 1, bkn, N, 1, M, N, K, bias=bias, c32=z, c16=y, ldc=N, mode=EPI_BIAS_SILU)
+    net._gemm(a, K, 1, bkn, N, 1, M, N, K, bias=bias, c32=z, c16=y, ldc=N, mode=EPI_BIAS_RELU)
     torch.cuda.synchronize()
     zr = ref + bias.double().cpu().numpy()
     assert np.all(np.abs(z.cpu().numpy() - zr) <= tol)
-    assert np.allclose(y.double().cpu().numpy(), R.silu(zr), rtol=1e-2, atol=1e-2)
+    assert np.allclose(y.double().cpu().numpy(), R.relu(zr), rtol=1e-2, atol=1e-2)
     net._gemm(a, K, 1, bkn, N, 1, M, N, K, bias=bias, c32=z, ldc=N, mode=EPI_BIAS)
     torch.cuda.synchronize()
     assert np.all(np.abs(z.cpu().numpy() - zr) <= tol)
-    net._gemm(a, K, 1, bkn, N, 1, M, N, K, zprev=zp, ldz=N, c32=z, c16=y, ldc=N, mode=EPI_DSILU)
+    net._gemm(a, K, 1, bkn, N, 1, M, N, K, zprev=zp, ldz=N, c32=z, c16=y, ldc=N, mode=EPI_DRELU)
     torch.cuda.synchronize()
-    dr = ref * R.dsilu(zp.double().cpu().numpy())
+    dr = ref * R.drelu(zp.double().cpu().numpy())
     assert np.all(np.abs(z.cpu().numpy() - dr) <= tol * 1.2)
 
 

@@ -13,6 +13,9 @@ def test_backprop_matches_finite_differences():
     t = rng.normal(size=(4, sizes[-1]))
     loss = lambda Ws, bs: 0.5 * np.sum((R.forward(Ws, bs, x, bf16=False)[0] - t) ** 2)  # noqa: E731
     out, xs, zs = R.forward(Ws, bs, x, bf16=False)
+    # ReLU kinks: no hidden pre-activation near 0, so that the central differences (step
+    # 1e-6) stay on one side of every kink
+    assert min(np.abs(z).min() for z in zs[:-1]) > 1e-3
     dW, db = R.backward(Ws, xs, zs, out - t, bf16=False)
     h = 1e-6
     for l in range(len(Ws)):

@@ -9,7 +9,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-EPI_STORE, EPI_BIAS_SILU, EPI_BIAS, EPI_DSILU = 0, 1, 2, 3
+EPI_STORE, EPI_BIAS_RELU, EPI_BIAS, EPI_DRELU = 0, 1, 2, 3
 
 
 def _lib():
@@ -43,10 +43,10 @@ def _both(torch, lib, M, N, K, mode, seed):
     for fn in (lib.gh_gemm_bf16, lib.gh_gemm_bf16_noalloc):
         c32 = torch.full((M, N), float("nan"), device="cuda")
         c16 = torch.zeros(M, N, device="cuda", dtype=torch.bfloat16)
-        use16 = mode in (EPI_BIAS_SILU, EPI_BIAS, EPI_DSILU)
+        use16 = mode in (EPI_BIAS_RELU, EPI_BIAS, EPI_DRELU)
         rc = _call(fn, a, b, M, N, K, mode, c32=c32, c16=c16 if use16 else None,
-                   bias=bias if mode in (EPI_BIAS_SILU, EPI_BIAS) else None,
-                   zprev=zp if mode == EPI_DSILU else None)
+                   bias=bias if mode in (EPI_BIAS_RELU, EPI_BIAS) else None,
+                   zprev=zp if mode == EPI_DRELU else None)
         assert rc == 0, lib.gh_last_error(None)
         torch.cuda.synchronize()
         outs.append((c32.cpu().numpy(), c16.float().cpu().numpy()))
@@ -57,7 +57,7 @@ def _both(torch, lib, M, N, K, mode, seed):
     return n32
 
 
-@pytest.mark.parametrize("mode", [EPI_STORE, EPI_BIAS_SILU, EPI_BIAS, EPI_DSILU])
+@pytest.mark.parametrize("mode", [EPI_STORE, EPI_BIAS_RELU, EPI_BIAS, EPI_DRELU])
 def test_noalloc_equals_allocating_entry_point_two_calls(mode):
     import torch
 
