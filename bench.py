@@ -231,6 +231,31 @@ def leg_ct256(args, steps, cpu=False):
     return leg
 
 
+def leg_ct256_schneider(args, steps=2, tol=0.05):
+    """ct256 through the reference's Schneider HU conversion (75 materials at 0.05 g/cm3,
+    sharing 27 composition rows): plain transport, `steps` launches of --histories."""
+    from opengate_amd import workloads
+    from opengate_amd.transport import GpuEngine
+
+    eng = GpuEngine(0)
+    tables, ph, src, tal, info = workloads.ct256_schneider(eng, tol)
+    eng.set_physics(tables)
+    eng.add_simulation(ph, src, tal)
+    n = int(args.histories)
+    eng.run([n // 10], 7)
+    _sync()
+    t0 = time.perf_counter()
+    cs = [eng.run([n], 12345, first_history=[k * n]) for k in range(steps)]
+    _sync()
+    elapsed = time.perf_counter() - t0
+    kms = sum(c["kernel_ms"] for c in cs) / steps
+    eng.close()
+    return {"workload": info["desc"], "n_materials": info["n_materials"], "table_rows": info["n_rows"],
+            "label_bits": info["label_bits"], "histories_per_step": n, "steps": steps,
+            "photons_per_s": steps * n / elapsed, "kernel_ms": kms,
+            "steps_per_history": cs[0]["steps"] / n}
+
+
 def leg_config1(args, cpu=True):
     """Config 1: 64^3 water phantom with three bone spheres, 140.5 keV point source, runs of
     1e6 histories (the tally accumulating), GPU against the CPU restatement."""
@@ -276,10 +301,13 @@ def bench_headline(args):
     _log(f"exitplane leg done in {time.perf_counter() - t_start:.0f} s")
     ct = leg_ct256(args, args.ct256_steps, cpu=False)
     _log(f"ct256 leg done: {ct['photons_per_s']:.3e} photons/s")
+    cs = leg_ct256_schneider(args)
+    _log(f"ct256-schneider leg done: {cs['photons_per_s']:.3e} photons/s")
     c1 = leg_config1(args, cpu=not args.no_cpu_baseline)
     _log(f"config1 leg done: {c1['photons_per_s']:.3e} photons/s")
     cfg = dict(ep["config"])
-    cfg["legs"] = {"ct256_plain": ct, "config1": {k: v for k, v in c1.items() if k != "cpu_baseline"}}
+    cfg["legs"] = {"ct256_plain": ct, "ct256_schneider": cs,
+                   "config1": {k: v for k, v in c1.items() if k != "cpu_baseline"}}
     line = dict(ep)
     line["config"] = cfg
     line["roofline_transport"] = ct["roofline"]

@@ -64,36 +64,47 @@ enum { GH_FACE_ALL = 6 };
 
 typedef struct gh_ctx gh_ctx;
 
-/* Per-material FP32 tables on the float-bit energy grid (node j has the FP32 energy
- * with bits (bucket_base + j) << 17, bucket_base = 7488, 64 buckets per octave from
- * 2^-10 MeV) and on the q^2 grid ray_x (ascending, ray_x[0] = 0). */
+/* FP32 tables on the float-bit energy grid (node j has the FP32 energy with bits
+ * (bucket_base + j) << 17, bucket_base = 7488, 64 buckets per octave from 2^-10 MeV) and on
+ * the q^2 grid ray_x (ascending, ray_x[0] = 0).
+ * Two layouts. n_compositions = 0: the tables ("[T]" below) are per material, mu_total and
+ * mu_en in 1/mm. n_compositions = C > 0: materials that share an elemental composition
+ * share its tables (the Schneider HU conversion makes many materials of one composition at
+ * different densities, HounsfieldUnit_to_material, geometry/materials.py:149-230): the
+ * tables are per composition, mu_total and mu_en per unit density (1/mm per g/cm3), and
+ * material m has mu = density[m] * mu_total[composition[m]] (FP32 product). Up to 65535
+ * materials. */
 typedef struct gh_physics_desc {
   int32_t n_materials;
-  int32_t n_buckets;         /* nodes per material = n_buckets + 1 */
+  int32_t n_buckets;         /* nodes per table row = n_buckets + 1 */
   int32_t n_x_nodes;
-  int32_t reserved0;
+  int32_t n_compositions;    /* 0 or C (see above) */
   const float *density;      /* [M] g/cm3 */
-  const float *mu_total;     /* [M][NB+1] 1/mm */
-  const float *mu_majorant;  /* [NB+1]   1/mm, >= every mu_total node */
-  const float *p_compton;    /* [M][NB+1] sigma_C / mu */
-  const float *p_rayleigh;   /* [M][NB+1] (sigma_C + sigma_R) / mu */
-  const float *mu_en;        /* [M][NB+1] 1/mm */
+  const int32_t *composition;/* [M] composition of each material (n_compositions > 0), else NULL */
+  const float *mu_total;     /* [T][NB+1] */
+  const float *mu_majorant;  /* [NB+1]   1/mm, >= the mu of every material at every node */
+  const float *p_compton;    /* [T][NB+1] sigma_C / mu */
+  const float *p_rayleigh;   /* [T][NB+1] (sigma_C + sigma_R) / mu */
+  const float *mu_en;        /* [T][NB+1] (optional) */
   const float *ray_x;        /* [NX] q^2 nodes, 1/Angstrom^2 */
-  const float *ray_cdf;      /* [M][NX] normalised int_0^x F^2 */
-  const float *incoh_s;      /* [M][NX] S(q)/Z */
+  const float *ray_cdf;      /* [T][NX] normalised int_0^x F^2 */
+  const float *incoh_s;      /* [T][NX] S(q)/Z */
   float e_cut;               /* photons below e_cut deposit locally and stop */
   int32_t reserved1;
 } gh_physics_desc;
 
 /* Voxel phantom. Transport runs in the phantom frame: the box spans [0, n*spacing)
- * per axis, labels are x-fastest (index = x + nx*(y + ny*z)). */
+ * per axis, labels are x-fastest (index = x + nx*(y + ny*z)). The reference's label image
+ * is uint16 (ImageVolume.create_label_image, geometry/volumes.py:1237): label_bytes = 2
+ * takes it as it is; 1 takes uint8 labels. */
 typedef struct gh_phantom_desc {
   int32_t nx, ny, nz;
   int32_t world_material;    /* material index of the world outside the box, -1 = vacuum */
   float spacing[3];
   float world_lo[3];         /* world box in the phantom frame (ignored for vacuum) */
   float world_hi[3];
-  const uint8_t *labels;     /* material index per voxel */
+  int32_t label_bytes;       /* 1 (uint8) or 2 (uint16) */
+  const void *labels;        /* material index per voxel */
 } gh_phantom_desc;
 
 typedef struct gh_source_desc {

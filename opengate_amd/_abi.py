@@ -24,8 +24,9 @@ DP = C.POINTER(C.c_double)
 
 class PhysicsDesc(C.Structure):
     _fields_ = [
-        ("n_materials", C.c_int32), ("n_buckets", C.c_int32), ("n_x_nodes", C.c_int32), ("reserved0", C.c_int32),
-        ("density", FP), ("mu_total", FP), ("mu_majorant", FP), ("p_compton", FP), ("p_rayleigh", FP),
+        ("n_materials", C.c_int32), ("n_buckets", C.c_int32), ("n_x_nodes", C.c_int32), ("n_compositions", C.c_int32),
+        ("density", FP), ("composition", C.POINTER(C.c_int32)), ("mu_total", FP), ("mu_majorant", FP),
+        ("p_compton", FP), ("p_rayleigh", FP),
         ("mu_en", FP), ("ray_x", FP), ("ray_cdf", FP), ("incoh_s", FP),
         ("e_cut", C.c_float), ("reserved1", C.c_int32),
     ]
@@ -35,7 +36,7 @@ class PhantomDesc(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("world_material", C.c_int32),
         ("spacing", C.c_float * 3), ("world_lo", C.c_float * 3), ("world_hi", C.c_float * 3),
-        ("labels", C.POINTER(C.c_uint8)),
+        ("label_bytes", C.c_int32), ("labels", C.c_void_p),
     ]
 
 

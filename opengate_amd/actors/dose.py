@@ -42,6 +42,40 @@ def variance_or_uncertainty(value, squared, n, which):
     return out
 
 
+def mean_of_highest_values(values, count):
+    """Mean of the `count` largest values (GateDoseActor::GetMeanOfHighestNValues,
+    GateDoseActor.cpp:401-428; 0 for an empty image)."""
+    v = np.asarray(values, dtype=np.float64).ravel()
+    k = min(int(count), v.size)
+    if k <= 0:
+        return 0.0
+    return float(np.partition(v, v.size - k)[v.size - k:].sum() / k)
+
+
+def dose_actor_mean_uncertainty(edep, edep2, n_events, top_voxels_count, threshold):
+    """Mean relative uncertainty of the stopping rule (GateDoseActor::ComputeMeanUncertainty,
+    GateDoseActor.cpp:281-330): n = max(events, 2); over the voxels whose edep exceeds
+    threshold x the mean of the top voxels, unc = sqrt(max(0, (S2/n - (S/n)^2) / (n - 1))) /
+    (S/n), each clamped at 1; their mean, or 1 when no voxel qualifies."""
+    n = float(max(n_events, 2))
+    e = np.asarray(edep, dtype=np.float64).ravel()
+    e2 = np.asarray(edep2, dtype=np.float64).ravel()
+    sel = e > mean_of_highest_values(e, top_voxels_count) * threshold
+    val = e[sel] / n
+    sq = e2[sel] / n
+    unc = (1.0 / (n - 1.0)) * (sq - val ** 2)
+    unc = np.sqrt(np.maximum(unc, 0.0)) / val
+    unc = np.minimum(unc, 1.0)
+    total = float(unc.sum())
+    return total / unc.size if (unc.size > 0 and total > 0) else 1.0
+
+
+def next_uncertainty_check(u, goal, n_events, overshoot):
+    """Events of the run at which the next check happens (GateDoseActor.cpp:272-274:
+    (u / goal)^2 n overshoot, truncated to an integer)."""
+    return int((u / goal) * (u / goal) * n_events * overshoot)
+
+
 def insert_suffix_before_extension(path, suffix):
     p = Path(path)
     name = p.name

@@ -34,8 +34,8 @@ namespace {
 
 struct SimDev {
   // phantom
-  const uint8_t *labels;  // uint8 per voxel, or two 4-bit labels per byte (lbits == 4)
-  int lbits;
+  const uint32_t *labels;  // 2^lb-bit labels packed in 32-bit words (lb = 2, 3, 4: 4-, 8-, 16-bit)
+  int lb;
   int nx, ny, nz, world_mat;
   float inv_s[3];
   float ts_ph[3];  // voxel spacing
@@ -110,23 +110,27 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Per-lane copy of what every Woodcock step reads (in a batched launch each lane may
 // follow another simulation); fields needed only at slow events (box size, world
 // material, tally alignment) are read from the simulation descriptor there.
+// label of voxel vox from 2^lb-bit labels packed in 32-bit words (voxel v in word
+// v >> (5 - lb), bits (v mod 2^(5-lb)) << lb): one global load and one bit-field extract
+// for every layout, no branch on it
+__device__ __forceinline__ int packed_label(const uint32_t *labels, uint32_t lb, int vox) {
+  const auto *g = (const __attribute__((address_space(1))) uint32_t *)labels;
+  const uint32_t v = (uint32_t)vox;
+  const uint32_t w = g[v >> (5u - lb)];
+  return (int)__builtin_amdgcn_ubfe(w, (v & ((1u << (5u - lb)) - 1u)) << lb, 1u << lb);
+}
+
 struct Hot {
-  const uint8_t *labels;
+  const uint32_t *labels;
   int nx, ny, nz;
   float isx, isy, isz;
-  uint32_t l4;  // 1: labels packed two per byte (low nibble = even voxel); 0: one per byte
+  uint32_t lb;
   __device__ __forceinline__ void load(const SimDev &s) {
     labels = s.labels; nx = s.nx; ny = s.ny; nz = s.nz;
     isx = s.inv_s[0]; isy = s.inv_s[1]; isz = s.inv_s[2];
-    l4 = s.lbits == 4 ? 1u : 0u;
+    lb = (uint32_t)s.lb;
   }
-  // one global load and one bit-field extract for both layouts (no branch on the
-  // layout): byte vox >> l4, then its low or high nibble (l4 = 1) or the whole byte
-  __device__ __forceinline__ int label(int vox) const {
-    const auto *g = (const __attribute__((address_space(1))) uint8_t *)labels;
-    const uint32_t b = g[(uint32_t)vox >> l4];
-    return (int)__builtin_amdgcn_ubfe(b, ((uint32_t)vox & l4) << 2, 8u >> l4);
-  }
+  __device__ __forceinline__ int label(int vox) const { return packed_label(labels, lb, vox); }
 };
 
 // reciprocal direction: the slab tests of a photon reuse one set of divisions
@@ -454,16 +458,17 @@ __device__ __forceinline__ float u01w(uint32_t w) { return (float)((w >> 8) | 1u
 // form-factor tables of the samplers (the branching ratios, read once per collision, stay
 // in global memory, cached)
 struct LdsLayout {
-  int mu, maj, sinc, rcdf, rx, total;
+  int mu, maj, mat, sinc, rcdf, rx, total;
 };
-__host__ __device__ inline LdsLayout lds_layout(int n_mat, int nb1, int nx, int stage) {
+__host__ __device__ inline LdsLayout lds_layout(int n_rows, int n_mat, int nb1, int nx, int stage) {
   LdsLayout o;
   o.mu = 0;
-  o.maj = n_mat * nb1;
-  const int end = o.maj + nb1;
+  o.maj = n_rows * nb1;
+  o.mat = (o.maj + nb1 + 1) & ~1;  // MatRow pairs, 8-byte aligned
+  const int end = o.mat + 2 * n_mat;
   o.sinc = end;
-  o.rcdf = o.sinc + n_mat * nx;
-  o.rx = o.rcdf + n_mat * nx;
+  o.rcdf = o.sinc + n_rows * nx;
+  o.rx = o.rcdf + n_rows * nx;
   o.total = stage >= 2 ? o.rx + nx : end;
   if (stage == 0) o.total = 0;
   return o;
@@ -477,12 +482,7 @@ constexpr int META_SIM_SHIFT = 17;           // simulation id in bits 17..31 (c3
 constexpr int MAX_SIMS = 1 << 15;
 
 // label of voxel `vox` of simulation s (both label layouts)
-__device__ __forceinline__ int sim_label(const SimDev &s, int vox) {
-  const uint32_t l4 = s.lbits == 4 ? 1u : 0u;
-  const auto *g = (const __attribute__((address_space(1))) uint8_t *)s.labels;
-  const uint32_t b = g[(uint32_t)vox >> l4];
-  return (int)__builtin_amdgcn_ubfe(b, ((uint32_t)vox & l4) << 2, 8u >> l4);
-}
+__device__ __forceinline__ int sim_label(const SimDev &s, int vox) { return packed_label(s.labels, (uint32_t)s.lb, vox); }
 
 // phantom voxel of a point (exact cell index, as the tally's), -1 outside the box
 __device__ __forceinline__ int sim_voxel(const SimDev &s, float x, float y, float z) {
@@ -512,17 +512,20 @@ transport_kernel(Launch L) {
   Tabs T;
   T.mu = P.mu; T.maj = P.maj; T.pcpr = P.pcpr; T.rx = P.rx; T.rcdf = P.rcdf; T.sinc = P.sinc;
   T.rguide = P.rguide;
+  const MatRow *mat_tab = P.mat;
   if (STAGE >= 1) {
     // staged by the threads of the launch (a launch may use fewer than BLOCK)
     const int nt = (int)blockDim.x;
-    const LdsLayout o = lds_layout(P.n_mat, NB1, P.nx, STAGE);
-    const int n_mu = P.n_mat * NB1;
+    const LdsLayout o = lds_layout(P.n_rows, P.n_mat, NB1, P.nx, STAGE);
+    const int n_mu = P.n_rows * NB1;
     for (int i = threadIdx.x; i < n_mu; i += nt) lds[o.mu + i] = P.mu[i];
     for (int i = threadIdx.x; i < NB1; i += nt) lds[o.maj + i] = P.maj[i];
+    for (int i = threadIdx.x; i < P.n_mat; i += nt) ((MatRow *)(lds + o.mat))[i] = P.mat[i];
     T.mu = lds + o.mu;
     T.maj = lds + o.maj;
+    mat_tab = (const MatRow *)(lds + o.mat);
     if (STAGE >= 2) {
-      const int n_x = P.n_mat * P.nx;
+      const int n_x = P.n_rows * P.nx;
       for (int i = threadIdx.x; i < n_x; i += nt) {
         lds[o.sinc + i] = P.sinc[i];
         lds[o.rcdf + i] = P.rcdf[i];
@@ -604,8 +607,8 @@ transport_kernel(Launch L) {
       st.add_e(E_ESC, (double)(p.E * p.wt));
       return M_DEAD;
     }
-    const int wm = s.world_mat;
-    const float muw = lerp_node(mu_tab + wm * NB1, e);
+    const MatRow mw = mat_tab[s.world_mat];
+    const float muw = mw.rho * lerp_node(mu_tab + mw.row * NB1, e);
     const float tw = world_exit(s, p.x, p.y, p.z, p.u, p.v, p.w, iv());
     const float d = (muw > 0.0f) ? (-glog(u01(g)) / muw) : INFINITY;
     if (hit && d >= tin && tin < tw) {
@@ -770,10 +773,11 @@ transport_kernel(Launch L) {
               st.steps++;
               const int vox = ix + hot.nx * (iy + hot.ny * iz);
               const int lab = hot.label(vox);
-              const float mu = lerp_node(mu_tab + lab * NB1, ec);
+              const MatRow mr = mat_tab[lab];
+              const float mu = mr.rho * lerp_node(mu_tab + mr.row * NB1, ec);
               if (TLE && ph.E <= tle_emax) {
                 const SimDev &s0 = sim(0);
-                const float x = (ph.E * ph.wt) * (lerp_node(P.muen + lab * NB1, ec) * ec.inv_maj);
+                const float x = (ph.E * ph.wt) * ((mr.rho * lerp_node(P.muen + mr.row * NB1, ec)) * ec.inv_maj);
                 deposit(s0, sq, tally_voxel(s0, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
               }
               if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
@@ -853,11 +857,12 @@ transport_kernel(Launch L) {
       bool comp = false;
       float x_dep = 0.0f;   // energy given to the medium at this interaction
       const bool in_world = cvox < 0;
+      if (mode == M_COLLIDE && !kill) clab = in_world ? s.world_mat : clab;
+      const int crow = mat_tab[clab].row;  // table row of the interaction's material
       if (mode == M_COLLIDE && !kill) {
-        clab = in_world ? s.world_mat : clab;
         const float r = u01(rng);
         float pc, pr;
-        branching(T, NB1, clab, ec, pc, pr);
+        branching(T, NB1, crow, ec, pc, pr);
         ntry = 0;
         if (r < pc) {
           if (!in_world) st.inc(C_COMPTON);
@@ -876,7 +881,7 @@ transport_kernel(Launch L) {
         float q = 1.0f, c = 1.0f;
         bool done = false;
         for (int t = 0; t < scatter_tries && !done; ++t) {
-          if (compton_try(rng, T, P.nx, clab, ph.E, q, c)) {
+          if (compton_try(rng, T, P.nx, crow, ph.E, q, c)) {
             done = true;
           } else {
             st.rej++;
@@ -900,7 +905,7 @@ transport_kernel(Launch L) {
         float c = 1.0f;
         bool done = false;
         for (int t = 0; t < scatter_tries && !done; ++t) {
-          if (rayleigh_try(rng, T, P.nx, clab, ph.E, c) || ++ntry == MAX_REJECT) done = true;
+          if (rayleigh_try(rng, T, P.nx, crow, ph.E, c) || ++ntry == MAX_REJECT) done = true;
         }
         if (done) {
           float cp, sp;
@@ -1294,23 +1299,41 @@ int gh_device_info(gh_ctx *ctx, int32_t *n_cu, int32_t *lds_bytes, char *name, i
 
 int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
   if (!ctx) return GH_EINVAL;
-  if (!ph || ph->n_materials <= 0 || ph->n_materials > 255 || ph->n_buckets <= 0 || ph->n_x_nodes < 2)
-    FAIL(GH_EINVAL, "gh_set_physics: invalid sizes");
+  if (!ph || ph->n_materials <= 0 || ph->n_materials > 65535 || ph->n_buckets <= 0 || ph->n_x_nodes < 2 ||
+      ph->n_compositions < 0 || ph->n_compositions > 65535)
+    FAIL(GH_EINVAL, "gh_set_physics: invalid sizes (1..65535 materials)");
   if (!ph->density || !ph->mu_total || !ph->mu_majorant || !ph->p_compton || !ph->p_rayleigh ||
       !ph->ray_x || !ph->ray_cdf || !ph->incoh_s)
     FAIL(GH_EINVAL, "gh_set_physics: missing table");
+  if (ph->n_compositions > 0 && !ph->composition) FAIL(GH_EINVAL, "gh_set_physics: composition array missing");
   if (ph->n_buckets > 64 * 24) FAIL(GH_ERANGE, "gh_set_physics: energy grid beyond 2^14 MeV");
   if (!(ph->e_cut >= 0x1p-10f) || !std::isfinite(ph->e_cut))
     FAIL(GH_EINVAL, "gh_set_physics: e_cut must be >= 2^-10 MeV (first energy node)");
+  const size_t M = ph->n_materials, NB1 = ph->n_buckets + 1, NX = ph->n_x_nodes;
+  const size_t R = ph->n_compositions > 0 ? (size_t)ph->n_compositions : M;  // table rows
+  // material m: attenuation rho * row; rho = density (shared compositions) or 1
+  std::vector<MatRow> mat(M);
+  for (size_t m = 0; m < M; ++m) {
+    if (!(ph->density[m] >= 0.0f) || !std::isfinite(ph->density[m]))
+      FAIL(GH_EINVAL, "gh_set_physics: densities must be finite and >= 0");
+    if (ph->n_compositions > 0) {
+      if (ph->composition[m] < 0 || ph->composition[m] >= ph->n_compositions)
+        FAIL(GH_EINVAL, "gh_set_physics: composition index out of range");
+      mat[m] = MatRow{ph->density[m], ph->composition[m]};
+    } else {
+      mat[m] = MatRow{1.0f, (int)m};
+    }
+  }
   {
-    const size_t M = ph->n_materials, NB1 = ph->n_buckets + 1, NX = ph->n_x_nodes;
     for (size_t j = 0; j < NB1; ++j) {
       if (!(ph->mu_majorant[j] > 0.0f) || !std::isfinite(ph->mu_majorant[j]))
         FAIL(GH_EINVAL, "gh_set_physics: majorant must be positive and finite");
+      for (size_t r = 0; r < R; ++r)
+        if (!(ph->mu_total[r * NB1 + j] >= 0.0f)) FAIL(GH_EINVAL, "gh_set_physics: mu_total must be >= 0");
+      // every material's mu, as the kernel forms it (FP32 product), within the majorant
       for (size_t m = 0; m < M; ++m) {
-        const float mu = ph->mu_total[m * NB1 + j];
-        if (!(mu >= 0.0f) || mu > ph->mu_majorant[j])
-          FAIL(GH_EINVAL, "gh_set_physics: mu_total must lie in [0, majorant]");
+        const float mu = mat[m].rho * ph->mu_total[(size_t)mat[m].row * NB1 + j];
+        if (!(mu <= ph->mu_majorant[j])) FAIL(GH_EINVAL, "gh_set_physics: a material's mu exceeds the majorant");
       }
     }
     if (ph->ray_x[0] != 0.0f) FAIL(GH_EINVAL, "gh_set_physics: ray_x[0] must be 0");
@@ -1320,60 +1343,65 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
       if (b != ((uint32_t)(X_BUCKET_BASE + (int)i - 1) << XSHIFT))
         FAIL(GH_EINVAL, "gh_set_physics: ray_x must be the q^2 bucket grid (node i >= 1 has bits (920+i-1)<<20)");
     }
-    for (size_t m = 0; m < M; ++m)
+    for (size_t r = 0; r < R; ++r)
       for (size_t i = 1; i < NX; ++i)
-        if (!(ph->ray_cdf[m * NX + i] >= ph->ray_cdf[m * NX + i - 1]))
+        if (!(ph->ray_cdf[r * NX + i] >= ph->ray_cdf[r * NX + i - 1]))
           FAIL(GH_EINVAL, "gh_set_physics: ray_cdf must be non-decreasing");
   }
   HIPCHK(hipSetDevice(ctx->device));
   if (!ctx->sims.empty()) FAIL(GH_ESTATE, "gh_set_physics: clear simulations first");
   free_all(ctx->phys_owned);
-  const size_t M = ph->n_materials, NB1 = ph->n_buckets + 1, NX = ph->n_x_nodes;
   std::vector<float> inv_rho(M);
   for (size_t i = 0; i < M; ++i) inv_rho[i] = ph->density[i] > 0.0f ? 1.0f / ph->density[i] : 0.0f;
   std::vector<float> muen_zero;
   const float *muen = ph->mu_en;
   ctx->have_muen = muen != nullptr;
-  if (!muen) { muen_zero.assign(M * NB1, 0.0f); muen = muen_zero.data(); }
+  if (!muen) { muen_zero.assign(R * NB1, 0.0f); muen = muen_zero.data(); }
   Phys P{};
   float *t;
   int rc;
-  if ((rc = dev_upload(ctx, ph->mu_total, M * NB1, &t, ctx->phys_owned))) return rc; P.mu = t;
+  {
+    MatRow *dm;
+    if ((rc = dev_upload(ctx, (const MatRow *)mat.data(), M, &dm, ctx->phys_owned))) return rc;
+    P.mat = dm;
+  }
+  if ((rc = dev_upload(ctx, ph->mu_total, R * NB1, &t, ctx->phys_owned))) return rc; P.mu = t;
   if ((rc = dev_upload(ctx, ph->mu_majorant, NB1, &t, ctx->phys_owned))) return rc; P.maj = t;
-  std::vector<float> pcpr(2 * M * NB1);
-  for (size_t i = 0; i < M * NB1; ++i) {
+  std::vector<float> pcpr(2 * R * NB1);
+  for (size_t i = 0; i < R * NB1; ++i) {
     pcpr[2 * i] = ph->p_compton[i];
     pcpr[2 * i + 1] = ph->p_rayleigh[i];
   }
-  if ((rc = dev_upload(ctx, (const float *)pcpr.data(), 2 * M * NB1, &t, ctx->phys_owned))) return rc; P.pcpr = t;
-  if ((rc = dev_upload(ctx, muen, M * NB1, &t, ctx->phys_owned))) return rc; P.muen = t;
+  if ((rc = dev_upload(ctx, (const float *)pcpr.data(), 2 * R * NB1, &t, ctx->phys_owned))) return rc; P.pcpr = t;
+  if ((rc = dev_upload(ctx, muen, R * NB1, &t, ctx->phys_owned))) return rc; P.muen = t;
   if ((rc = dev_upload(ctx, ph->ray_x, NX, &t, ctx->phys_owned))) return rc; P.rx = t;
-  if ((rc = dev_upload(ctx, ph->ray_cdf, M * NX, &t, ctx->phys_owned))) return rc; P.rcdf = t;
+  if ((rc = dev_upload(ctx, ph->ray_cdf, R * NX, &t, ctx->phys_owned))) return rc; P.rcdf = t;
   {
-    // guide[m][b] = largest i <= NX-2 with ray_cdf[m][i] <= b / RAY_GUIDE
-    std::vector<int32_t> guide(M * (RAY_GUIDE + 1));
-    for (size_t m = 0; m < M; ++m) {
-      const float *G = ph->ray_cdf + m * NX;
+    // guide[r][b] = largest i <= NX-2 with ray_cdf[r][i] <= b / RAY_GUIDE
+    std::vector<int32_t> guide(R * (RAY_GUIDE + 1));
+    for (size_t r = 0; r < R; ++r) {
+      const float *G = ph->ray_cdf + r * NX;
       int i = 0;
       for (int b = 0; b <= RAY_GUIDE; ++b) {
         const float edge = (float)b / (float)RAY_GUIDE;
         while (i + 1 <= (int)NX - 2 && G[i + 1] <= edge) ++i;
-        guide[m * (RAY_GUIDE + 1) + b] = i;
+        guide[r * (RAY_GUIDE + 1) + b] = i;
       }
     }
     int32_t *tg;
     if ((rc = dev_upload(ctx, (const int32_t *)guide.data(), guide.size(), &tg, ctx->phys_owned))) return rc;
     P.rguide = tg;
   }
-  if ((rc = dev_upload(ctx, ph->incoh_s, M * NX, &t, ctx->phys_owned))) return rc; P.sinc = t;
+  if ((rc = dev_upload(ctx, ph->incoh_s, R * NX, &t, ctx->phys_owned))) return rc; P.sinc = t;
   if ((rc = dev_upload(ctx, inv_rho.data(), M, &t, ctx->phys_owned))) return rc; P.inv_rho = t;
   P.n_mat = (int)M;
+  P.n_rows = (int)R;
   P.nb = ph->n_buckets;
   P.nx = (int)NX;
   P.e_cut = ph->e_cut;
   ctx->P = P;
-  ctx->lds1_bytes = (size_t)lds_layout((int)M, (int)NB1, (int)NX, 1).total * sizeof(float);
-  ctx->lds2_bytes = (size_t)lds_layout((int)M, (int)NB1, (int)NX, 2).total * sizeof(float);
+  ctx->lds1_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 1).total * sizeof(float);
+  ctx->lds2_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 2).total * sizeof(float);
   ctx->have_phys = true;
   return GH_OK;
 }
@@ -1399,8 +1427,13 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
   for (int i = 0; i < 3; ++i)
     if (!(ph->spacing[i] > 0.0f)) FAIL(GH_EINVAL, "gh_add_simulation: spacing must be > 0");
   if (ph->world_material >= ctx->P.n_mat) FAIL(GH_EINVAL, "gh_add_simulation: world material out of range");
-  for (int64_t i = 0; i < nvox; ++i)
-    if (ph->labels[i] >= ctx->P.n_mat) FAIL(GH_EINVAL, "gh_add_simulation: label out of range of the material table");
+  if (ph->label_bytes != 1 && ph->label_bytes != 2) FAIL(GH_EINVAL, "gh_add_simulation: label_bytes must be 1 or 2");
+  auto label_in = [&](int64_t i) -> uint32_t {
+    return ph->label_bytes == 2 ? ((const uint16_t *)ph->labels)[i] : ((const uint8_t *)ph->labels)[i];
+  };
+  uint32_t max_label = 0;
+  for (int64_t i = 0; i < nvox; ++i) max_label = std::max(max_label, label_in(i));
+  if (max_label >= (uint32_t)ctx->P.n_mat) FAIL(GH_EINVAL, "gh_add_simulation: label out of range of the material table");
   {
     const int rc_src = validate_source(ctx, src);
     if (rc_src) return rc_src;
@@ -1423,19 +1456,23 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
   SimDev &d = h.dev;
   std::memset(&d, 0, sizeof(d));
   int rc;
-  uint8_t *lab;
-  if (ctx->P.n_mat <= 16) {
-    // at most 16 materials: two labels per byte halve the footprint the Woodcock steps
-    // gather from (ct256: 4 MiB instead of 8 MiB against 4 MB of L2 per XCD)
-    std::vector<uint8_t> packed(((size_t)nvox + 1) / 2, 0);
-    for (int64_t i = 0; i < nvox; ++i) packed[i >> 1] |= (uint8_t)(ph->labels[i] << ((i & 1) * 4));
-    if ((rc = dev_upload(ctx, packed.data(), packed.size(), &lab, h.owned))) { free_all(h.owned); return rc; }
-    d.lbits = 4;
-  } else {
-    if ((rc = dev_upload(ctx, ph->labels, (size_t)nvox, &lab, h.owned))) { free_all(h.owned); return rc; }
-    d.lbits = 8;
+  {
+    // device layout: the narrowest of 4-, 8- and 16-bit labels that holds every label of
+    // the phantom, packed in 32-bit words (at most 16 materials in use: two labels per
+    // byte, ct256 4 MiB instead of 8 MiB against 4 MB of L2 per XCD)
+    const int lb = max_label < 16u ? 2 : (max_label < 256u ? 3 : 4);
+    const int per_word = 32 >> lb;
+    std::vector<uint32_t> packed(((size_t)nvox + per_word - 1) / per_word, 0u);
+    for (int64_t i = 0; i < nvox; ++i)
+      packed[(size_t)i / per_word] |= label_in(i) << (((uint32_t)i % per_word) << lb);
+    uint32_t *lab;
+    if ((rc = dev_upload(ctx, (const uint32_t *)packed.data(), packed.size(), &lab, h.owned))) {
+      free_all(h.owned);
+      return rc;
+    }
+    d.labels = lab;
+    d.lb = lb;
   }
-  d.labels = lab;
   d.nx = ph->nx; d.ny = ph->ny; d.nz = ph->nz;
   d.world_mat = ph->world_material;
   for (int i = 0; i < 3; ++i) {

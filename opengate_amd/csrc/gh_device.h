@@ -272,17 +272,25 @@ GH_DEV float qgrid_value(const float *ys, int n, float x) {
   return fmaf(f, y1 - y0, y0);
 }
 
+// material m: its attenuation is rho * (table row `row` of the mu tables); rho is the
+// density when materials share composition rows, else 1 (a row per material)
+struct MatRow {
+  float rho;
+  int row;
+};
+
 struct Phys {
-  const float *mu;       // [M][NB+1] (global copy)
-  const int32_t *rguide; // [M][RAY_GUIDE+1] guide of rcdf
+  const MatRow *mat;     // [M]
+  const float *mu;       // [R][NB+1] table rows (global copy)
+  const int32_t *rguide; // [R][RAY_GUIDE+1] guide of rcdf
   const float *maj;      // [NB+1]
-  const float *pcpr;     // [M][NB+1][2] (sigma_C/mu, (sigma_C+sigma_R)/mu) interleaved
-  const float *muen;     // [M][NB+1]
+  const float *pcpr;     // [R][NB+1][2] (sigma_C/mu, (sigma_C+sigma_R)/mu) interleaved
+  const float *muen;     // [R][NB+1]
   const float *rx;       // [NX]
-  const float *rcdf;     // [M][NX]
-  const float *sinc;     // [M][NX]
+  const float *rcdf;     // [R][NX]
+  const float *sinc;     // [R][NX]
   const float *inv_rho;  // [M]
-  int n_mat, nb, nx;
+  int n_mat, n_rows, nb, nx;
   float e_cut;
 };
 

@@ -10,6 +10,7 @@ initialize :1721, start_and_stop :1632) for the photon path:
   * the DoseActor grid becomes the tally, dose/uncertainty are derived in FP64 as
     DoseActor.EndOfRunActionMasterThread and get_variance_or_uncertainty do.
 """
+import dataclasses
 import time
 
 import numpy as np
@@ -71,8 +72,8 @@ class SimulationEngine:
             spacing = np.asarray(vol.size, dtype=np.float64)
             size = np.array([1, 1, 1])
             lab16 = np.zeros((1, 1, 1), np.uint16)
-        if len(names) > 255:
-            raise NotImplementedError("more than 255 materials in one phantom")
+        if len(names) > 65535:
+            raise ValueError("more than 65535 materials in one phantom (uint16 labels)")
         mats = [db.get(n) for n in names]
         world = self.sim.world
         world_index = -1
@@ -93,7 +94,7 @@ class SimulationEngine:
             half = np.asarray(world.size, dtype=np.float64) / 2
             wlo = tuple(-half - t + L / 2)
             whi = tuple(half - t + L / 2)
-        ph = PhantomSpec(lab16.astype(np.uint8), tuple(spacing), world_index, wlo, whi)
+        ph = PhantomSpec(np.asarray(lab16, dtype=np.uint16), tuple(spacing), world_index, wlo, whi)
         return ph, mats, names
 
     def to_phantom(self, p_world):
@@ -266,14 +267,32 @@ class SimulationEngine:
         self.engine = GpuEngine(sim.device)
         vol = self.phantom_volume()
         ph, mats, names = self.build_geometry(vol)
-        duration_s = sum((b - a) for a, b in sim.run_timing_intervals) / g4_units.s
-        runs = []
-        for s in sim.sources.values():
-            n = s.number_of_primaries(duration_s)
+        # events of each source in each run interval (GateSourceManager run loop): counts
+        # per run for number_of_primaries, Poisson counts of the activity, drawn from a
+        # generator keyed by the seed, the source and the run
+        per_run = self.events_per_run(seed)
+        specs = {}
+        for s, counts in per_run:
+            n = sum(counts)
             if n > 0:
-                runs.append((s, self.source_spec(s, n), n))
+                specs[s.name] = (s, self.source_spec(s, n))
+        # launch plan: run by run, each source's events of that run
+        runs = []
+        run_of = []  # run interval of each launch-plan entry
+        done = {}
+        for k in range(len(sim.run_timing_intervals)):
+            for s, counts in per_run:
+                if counts[k] > 0:
+                    src, spec = specs[s.name]
+                    if spec.pos_type == "phsp":
+                        # a phase-space source goes on from the row where the last run ended
+                        spec = dataclasses.replace(spec, phsp_start=spec.phsp_start + done.get(s.name, 0))
+                    runs.append((src, spec, counts[k]))
+                    run_of.append(k)
+                    done[s.name] = done.get(s.name, 0) + counts[k]
         if not runs:
-            raise ValueError("no source with primaries (set source.n or source.activity)")
+            raise ValueError("no source with primaries (set source.number_of_primaries or source.activity)")
+        self.events_per_source = {s.name: counts for s, counts in per_run}
         e_max = max(spec.max_energy() for _, spec, _ in runs)
         e_cut = max(float(sim.physics_manager.photon_tracking_cut), 2.0 ** -10)
         tables = xs.build_tables(mats, e_max * (1 + 1e-6), e_cut=e_cut)
@@ -297,14 +316,19 @@ class SimulationEngine:
         self.executed = []  # (SourceSpec, histories, first history) in launch order
         self.inputs = (tables, ph, tal)
         t0 = time.time()
+        goal = self.dose is not None and self.dose.uncertainty_goal is not None
         for k, (src, spec, n) in enumerate(runs):
             if k > 0:
                 self.engine.set_source(sid, spec)
+            if goal and (k == 0 or run_of[k] != run_of[k - 1]):
+                self._goal_begin_run(first)
+            if goal and self._goal_stopped:
+                continue  # the run was stopped at the goal: its other sources add nothing
             todo = n
             while todo > 0:
                 chunk = min(todo, 2**32 - 1)
-                if self.dose is not None and self.dose.uncertainty_goal is not None:
-                    chunk = self._goal_chunk(first - first0, todo)
+                if goal:
+                    chunk = min(chunk, self._goal_chunk(first))
                 if self.spect is not None:
                     chunk = min(chunk, tal.record_capacity)
                 c = self.engine.run([chunk], seed, first_history=[first], **self.launch)
@@ -316,14 +340,21 @@ class SimulationEngine:
                 counters = c if counters is None else {key: counters[key] + c[key] for key in c}
                 first += chunk
                 todo -= chunk
-                if (self.dose is not None and self.dose.uncertainty_goal is not None
-                        and self._goal_reached(first - first0)):
+                if goal and self._goal_reached(first):
                     todo = 0
         duration = time.time() - t0
         n_events = first - first0
         self.seed = seed
         self.finish(vol, ph, tal, names, tables, counters, n_events, duration, t_init)
         return counters
+
+    def events_per_run(self, seed):
+        """[(source, [events in run k])] for the simulation's run_timing_intervals."""
+        out = []
+        for i, s in enumerate(self.sim.sources.values()):
+            rng = np.random.Generator(np.random.PCG64([int(seed) & 0xFFFFFFFFFFFFFFFF, i, 0x5EED]))
+            out.append((s, s.events_per_run(self.sim.run_timing_intervals, rng)))
+        return out
 
     def _launch_parameters(self, runs):
         """Launch parameters of the persistent kernel: sim.launch as given (a dict of
@@ -357,28 +388,37 @@ class SimulationEngine:
                 s.write_outputs()
         return {"histories": int(n)}
 
-    # uncertainty goal: GateDoseActor::EndOfEventAction / ComputeMeanUncertainty (GateDoseActor.cpp:245-330)
-    def _goal_chunk(self, done, todo):
-        a = self.dose
-        if done == 0:
-            return int(min(todo, max(1, a.uncertainty_first_check_after_n_events)))
-        return int(min(todo, max(1, self._next_check - done)))
-
-    def _goal_reached(self, n):
-        a = self.dose
+    # uncertainty goal (GateDoseActor::EndOfEventAction / ComputeMeanUncertainty,
+    # GateDoseActor.cpp:245-330): per run (the event count and the next check restart at
+    # BeginOfRunActionMasterThread, :83-89), on the run's own part of the tallies
+    def _goal_begin_run(self, first):
         t = self.engine.get_tally(0)
-        edep, edep2 = t["edep"], t["edep2"]
-        top = np.sort(edep.ravel())[-int(a.uncertainty_top_voxels_count):]
-        thresh = a.uncertainty_voxel_edep_threshold * top.mean()
-        sel = edep > thresh
-        from .actors.dose import variance_or_uncertainty
+        self._goal_base = (t["edep"].copy(), t["edep2"].copy())
+        self._goal_first = first
+        self._goal_next = max(1, int(self.dose.uncertainty_first_check_after_n_events))
+        self._goal_stopped = False
 
-        unc = variance_or_uncertainty(edep[sel], edep2[sel], n, "uncertainty")
-        u = float(unc.mean()) if unc.size else 1.0
+    def _goal_chunk(self, first):
+        return int(max(1, self._goal_next - (first - self._goal_first)))
+
+    def _goal_reached(self, first):
+        from .actors.dose import dose_actor_mean_uncertainty, next_uncertainty_check
+
+        a = self.dose
+        n = first - self._goal_first
+        if n < self._goal_next:
+            return False
+        t = self.engine.get_tally(0)
+        edep = t["edep"] - self._goal_base[0]
+        edep2 = t["edep2"] - self._goal_base[1]
+        u = dose_actor_mean_uncertainty(edep, edep2, n, a.uncertainty_top_voxels_count,
+                                        a.uncertainty_voxel_edep_threshold)
         self.goal_uncertainty = u
+        self.goal_checks = getattr(self, "goal_checks", []) + [(n, u)]
         if u <= a.uncertainty_goal:
+            self._goal_stopped = True
             return True
-        self._next_check = int(n * (u / a.uncertainty_goal) ** 2 * a.uncertainty_overshoot_factor_N_events) + 1
+        self._goal_next = next_uncertainty_check(u, a.uncertainty_goal, n, a.uncertainty_overshoot_factor_N_events)
         return False
 
     def finish(self, vol, ph, tal, names, tables, counters, n_events, duration, t_init):

@@ -316,53 +316,99 @@ def rayleigh_x_nodes(e_max):
 
 @dataclass
 class PhysicsTables:
-    """FP32 arrays in the layout the C-ABI call gh_set_physics expects."""
+    """FP32 arrays in the layout the C-ABI call gh_set_physics expects. Table rows ("R"):
+    one per material (composition is None; mu_total, mu_en in 1/mm), or one per elemental
+    composition shared by several materials (composition[m] = row of material m; mu_total,
+    mu_en per unit density, 1/mm per g/cm3, and material m has mu = density[m] * row)."""
 
     material_names: list
     density: np.ndarray  # [M] float32 g/cm3
     n_buckets: int
-    mu_total: np.ndarray  # [M, NB+1] float32 1/mm
+    mu_total: np.ndarray  # [R, NB+1] float32
     mu_majorant: np.ndarray  # [NB+1] float32 1/mm
-    p_compton: np.ndarray  # [M, NB+1] float32 cumulative
-    p_rayleigh: np.ndarray  # [M, NB+1] float32 cumulative (compton + rayleigh)
-    mu_en: np.ndarray  # [M, NB+1] float32 1/mm
+    p_compton: np.ndarray  # [R, NB+1] float32 cumulative
+    p_rayleigh: np.ndarray  # [R, NB+1] float32 cumulative (compton + rayleigh)
+    mu_en: np.ndarray  # [R, NB+1] float32
     ray_x: np.ndarray  # [NX] float32 q^2 nodes [1/A^2]
-    ray_cdf: np.ndarray  # [M, NX] float32 normalised int F^2 dx
-    incoh_s: np.ndarray  # [M, NX] float32 S(q)/Z of the material
+    ray_cdf: np.ndarray  # [R, NX] float32 normalised int F^2 dx
+    incoh_s: np.ndarray  # [R, NX] float32 S(q)/Z of the row's composition
     e_cut: float
     e_max: float
+    composition: np.ndarray = None  # [M] int32, or None (a row per material)
 
     @property
     def n_materials(self):
         return len(self.material_names)
 
+    @property
+    def n_compositions(self):
+        return 0 if self.composition is None else int(self.mu_total.shape[0])
 
-def build_tables(materials, e_max, e_cut=None, majorant_margin=2.0**-16):
+    def material_mu(self, m):
+        """[NB+1] mu nodes (1/mm) of material m, as FP32 products when rows are shared."""
+        if self.composition is None:
+            return self.mu_total[m]
+        return np.float32(self.density[m]) * self.mu_total[self.composition[m]]
+
+    def row_of(self, m):
+        return m if self.composition is None else int(self.composition[m])
+
+
+def _composition_key(mat):
+    if mat.is_vacuum:
+        return ("vacuum",)
+    return tuple((int(z), float(w)) for z, w in zip(mat.elements, mat.fractions))
+
+
+def build_tables(materials, e_max, e_cut=None, majorant_margin=2.0**-16, share_compositions=True):
     """Build the FP32 transport tables for `materials` (list of Material) up to e_max MeV.
 
-    All arithmetic is FP64; the arrays are rounded to FP32 once at the end.
-    The majorant node values are the per-node maximum over materials, raised by
+    All arithmetic is FP64; the arrays are rounded to FP32 once at the end. When several
+    materials share an elemental composition (Schneider HU materials at different
+    densities) and share_compositions is set, the tables hold one row per composition for
+    unit density; otherwise one row per material. The majorant node values are the
+    per-node maximum of every material's mu (as the transport forms it in FP32), raised by
     `majorant_margin` (relative) so that FP32 interpolation never exceeds it.
     """
     nb = n_buckets_for(e_max)
     e_nodes = bucket_node_energies(nb).astype(np.float64)
-    m = len(materials)
-    mu_t = np.zeros((m, nb + 1))
-    pc = np.zeros((m, nb + 1))
-    pr = np.zeros((m, nb + 1))
-    mue = np.zeros((m, nb + 1))
-    for i, mat in enumerate(materials):
+    keys = [_composition_key(m) for m in materials]
+    rows, comp = {}, []
+    for k in keys:
+        comp.append(rows.setdefault(k, len(rows)))
+    shared = share_compositions and len(rows) < len(materials)
+    if shared:
+        reps = [None] * len(rows)
+        for m, c in zip(materials, comp):
+            if reps[c] is None:
+                reps[c] = m
+        row_mats = reps
+    else:
+        row_mats = list(materials)
+    r = len(row_mats)
+    mu_t = np.zeros((r, nb + 1))
+    pc = np.zeros((r, nb + 1))
+    pr = np.zeros((r, nb + 1))
+    mue = np.zeros((r, nb + 1))
+    for i, mat in enumerate(row_mats):
         mu, compt, rayl, photo, muen = partial_mass_attenuation(mat, e_nodes)
-        rho = 0.0 if mat.is_vacuum else mat.density
+        # shared rows are per unit density (the vacuum row keeps density 0)
+        rho = 0.0 if mat.is_vacuum else (1.0 if shared else mat.density)
         mu_t[i] = mu * rho / 10.0  # 1/cm -> 1/mm
         mue[i] = muen * rho / 10.0
         pc[i] = compt / mu
         pr[i] = (compt + rayl) / mu
-    maj = mu_t.max(axis=0) * (1.0 + majorant_margin)
+    density = np.array([0.0 if mm.is_vacuum else mm.density for mm in materials], dtype=np.float32)
+    mu32 = mu_t.astype(np.float32)
+    if shared:
+        per_mat = np.stack([density[m] * mu32[comp[m]] for m in range(len(materials))])  # FP32 products
+    else:
+        per_mat = mu32
+    maj = per_mat.astype(np.float64).max(axis=0) * (1.0 + majorant_margin)
     xn = rayleigh_x_nodes(float(e_nodes[-1]))
-    cdf = np.zeros((m, len(xn)))
-    sfun = np.zeros((m, len(xn)))
-    for i, mat in enumerate(materials):
+    cdf = np.zeros((r, len(xn)))
+    sfun = np.zeros((r, len(xn)))
+    for i, mat in enumerate(row_mats):
         g = np.zeros(len(xn))
         s_num = np.zeros(len(xn))
         s_den = 0.0
@@ -378,9 +424,9 @@ def build_tables(materials, e_max, e_cut=None, majorant_margin=2.0**-16):
     f32 = np.float32
     return PhysicsTables(
         material_names=[mm.name for mm in materials],
-        density=np.array([0.0 if mm.is_vacuum else mm.density for mm in materials], dtype=f32),
+        density=density,
         n_buckets=nb,
-        mu_total=mu_t.astype(f32),
+        mu_total=mu32,
         mu_majorant=maj.astype(f32),
         p_compton=pc.astype(f32),
         p_rayleigh=pr.astype(f32),
@@ -390,4 +436,5 @@ def build_tables(materials, e_max, e_cut=None, majorant_margin=2.0**-16):
         incoh_s=sfun.astype(f32),
         e_cut=float(e_cut),
         e_max=float(e_nodes[-1]),
+        composition=np.array(comp, dtype=np.int32) if shared else None,
     )

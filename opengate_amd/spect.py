@@ -244,12 +244,12 @@ def run_arf_training(sim, seed):
     if rr < 1:
         raise ValueError("russian_roulette must be >= 1")
     theta_acc = float(head.acceptance_angle)
-    duration_s = sum((b - a) for a, b in sim.run_timing_intervals) / g4_units.s
     device = f"cuda:{sim.device}"
     g = torch.Generator(device=device).manual_seed(int(derived_seed(seed, 2) & 0x7FFFFFFFFFFFFFFF))
     plans, e_lo, e_hi = [], math.inf, 0.0
-    for s in sim.sources.values():
-        n = s.number_of_primaries(duration_s)
+    for i, s in enumerate(sim.sources.values()):
+        rng = np.random.Generator(np.random.PCG64([int(seed) & 0xFFFFFFFFFFFFFFFF, i, 0x5EED]))
+        n = s.number_of_primaries_total(sim.run_timing_intervals, rng)
         if n <= 0:
             continue
         if not isinstance(s, GenericSource) or s.particle != "gamma":
@@ -263,7 +263,7 @@ def run_arf_training(sim, seed):
         e_lo, e_hi = min(e_lo, lo), max(e_hi, hi)
         plans.append((n, sample_e, (t0, t1), (float(s.direction.phi[0]), float(s.direction.phi[1]))))
     if not plans:
-        raise ValueError("no source with primaries (set source.n or source.activity)")
+        raise ValueError("no source with primaries (set source.number_of_primaries or source.activity)")
     e_range = (e_lo if e_lo < e_hi else 0.0, e_hi)
     hs = HeadSimulator(head.head_model(), device=sim.device, e_max=e_hi)
     det_seed = derived_seed(seed, 3)

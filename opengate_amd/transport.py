@@ -19,16 +19,22 @@ def _f3(v):
 
 @dataclass
 class PhantomSpec:
-    """Label volume in the phantom frame: box [0, n*spacing), labels[z, y, x]."""
+    """Label volume in the phantom frame: box [0, n*spacing), labels[z, y, x]. Labels are
+    uint16, as the reference's label image (geometry/volumes.py:1237), or uint8."""
 
-    labels: np.ndarray  # uint8 (nz, ny, nx)
+    labels: np.ndarray  # uint8 or uint16 (nz, ny, nx)
     spacing: tuple  # mm (sx, sy, sz)
     world_material: int = -1  # -1: vacuum outside the box
     world_lo: tuple = (0.0, 0.0, 0.0)  # world box in the phantom frame
     world_hi: tuple = (0.0, 0.0, 0.0)
 
     def __post_init__(self):
-        self.labels = np.ascontiguousarray(self.labels, dtype=np.uint8)
+        lab = np.asarray(self.labels)
+        if lab.dtype != np.uint16 and lab.dtype != np.uint8:
+            if lab.size and (lab.min() < 0 or lab.max() > 65535):
+                raise ValueError("labels must lie in 0..65535")
+            lab = lab.astype(np.uint16 if lab.size and lab.max() > 255 else np.uint8)
+        self.labels = np.ascontiguousarray(lab)
         assert self.labels.ndim == 3
 
     @property
@@ -44,7 +50,8 @@ class PhantomSpec:
         d.spacing = _f3(self.spacing)
         d.world_lo = _f3(self.world_lo)
         d.world_hi = _f3(self.world_hi)
-        d.labels = self.labels.ctypes.data_as(C.POINTER(C.c_uint8))
+        d.label_bytes = self.labels.dtype.itemsize
+        d.labels = self.labels.ctypes.data_as(C.c_void_p)
         return d
 
 
@@ -251,6 +258,12 @@ def physics_desc(tables):
         return A.fptr(a)
 
     d.density = f(tables.density)
+    comp = getattr(tables, "composition", None)
+    if comp is not None:
+        c = np.ascontiguousarray(comp, dtype=np.int32)
+        keep.append(c)
+        d.n_compositions = int(tables.n_compositions)
+        d.composition = c.ctypes.data_as(C.POINTER(C.c_int32))
     d.mu_total = f(tables.mu_total)
     d.mu_majorant = f(tables.mu_majorant)
     d.p_compton = f(tables.p_compton)

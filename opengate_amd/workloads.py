@@ -45,6 +45,53 @@ def ct256(engine, seed=2):
     return tables, ph, src, tal, {"desc": CT256_DESC, "hu": hu, "names": names}
 
 
+CT256S_DESC = ("ct256-schneider: the ct256 HU phantom mapped by HounsfieldUnit_to_material (Schneider 2000 tables, "
+               "density tolerance {tol} g/cm3: {n} materials of {c} compositions, {bits}-bit labels), air world, "
+               "120 kVp cone from 600 mm; dose on the 256x256x128 grid")
+
+
+def ct256_schneider(engine, tol=0.05, seed=2, share_compositions=True):
+    """ct256 with the reference's own HU conversion: the Schneider materials and densities
+    tables split into sub-intervals of density width <= tol g/cm3
+    (HounsfieldUnit_to_material, geometry/materials.py:149-230); the label image follows the
+    ImageVolume label rule (default material G4_AIR = label 0). Returns (tables, phantom,
+    source, tally, info); labels are computed on the GPU and kept as uint16 when needed."""
+    from types import SimpleNamespace
+    from pathlib import Path as _P
+
+    from .geometry.materials import HounsfieldUnit_to_material
+    from .units import g4_units
+
+    db = MaterialDatabase()
+    data = _P(__file__).resolve().parent / "data"
+    vm, created = HounsfieldUnit_to_material(SimpleNamespace(volume_manager=SimpleNamespace(material_database=db)),
+                                             tol * g4_units.g_cm3, str(data / "Schneider2000MaterialsTable.txt"),
+                                             str(data / "Schneider2000DensitiesTable.txt"))
+    hu = P.ct256_hu(seed)
+    names = ["G4_AIR"] + [r[2] for r in vm]
+    lut = {n: i for i, n in enumerate(names)}
+    rows = sorted(vm, key=lambda r: r[0])
+    bins = np.array([r[0] for r in rows] + [rows[-1][1]], dtype=np.float64)
+    labels_sorted = np.array([0] + [lut[r[2]] for r in rows] + [0], dtype=np.uint16)
+    lab = engine.hu_to_labels(hu, bins, labels_sorted)
+    mats = [db.get(n) for n in names]
+    tables = xs.build_tables(mats, 0.1201, share_compositions=share_compositions)
+    L = np.array([256.0, 256.0, 128.0])
+    half_w = 1500.0
+    ph = PhantomSpec(lab if len(names) > 256 else lab.astype(np.uint8), (1.0, 1.0, 1.0), 0,
+                     tuple(-half_w + L / 2), tuple(half_w + L / 2))
+    e, wts = P.kramers_spectrum(db=db)
+    ev, ew, cdf = spectrum_histogram(e, wts)
+    src = SourceSpec(pos_trans=(128.0, 128.0, 64.0 + 600.0), theta=(0.0, np.deg2rad(15.0)),
+                     ene_type="spectrum_histogram", energy_value=ev, energy_width=ew, energy_cdf=cdf)
+    tal = TallySpec(size=(256, 256, 128), spacing=(1.0, 1.0, 1.0))
+    n_rows = tables.n_compositions or tables.n_materials
+    bits = 4 if int(lab.max()) < 16 else (8 if int(lab.max()) < 256 else 16)
+    desc = CT256S_DESC.format(tol=tol, n=len(names), c=n_rows, bits=bits)
+    return tables, ph, src, tal, {"desc": desc, "hu": hu, "names": names, "n_materials": len(names),
+                                  "n_rows": n_rows, "label_bits": bits}
+
+
 def water64(seed=1):
     db = MaterialDatabase()
     lab, spheres = P.water64_labels(seed)

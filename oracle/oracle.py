@@ -292,3 +292,74 @@ def beer_lambert(p0, p1, labels_zyx, spacing, mu_per_label):
     idx, seg = siddon_path_lengths(p0, p1, tuple(reversed(labels_zyx.shape)), spacing)
     lab = labels_zyx[idx[:, 2], idx[:, 1], idx[:, 0]]
     return float(np.exp(-np.sum(np.asarray(mu_per_label)[lab] * seg)))
+
+
+# ------------------------------------------------------------------ source run loop
+def source_events_reference(run_timing_intervals, start_time, end_time, activity, half_life, n_per_run, uniform):
+    """Events of one source per run, following the reference's loop event by event
+    (GateSourceManager::PrepareRunToStart / PrepareNextSource, GateSourceManager.cpp:313-359;
+    GateVSource::PrepareNextTime, CalcNextTime, UpdateActivity, GateVSource.cpp:46-89).
+    Times in seconds, activity in 1/s (already multiplied by the spectrum yield), half_life
+    in s (<= 0: none), n_per_run: list (number_of_primaries) or None (activity);
+    uniform(): a uniform draw in (0, 1). Pure-Python loop: small activities only."""
+    counts = []
+    lam = np.log(2.0) / half_life if half_life > 0 else 0.0
+    for run, (a, b) in enumerate(run_timing_intervals):
+        max_n = 0 if n_per_run is None else int(n_per_run[run])
+        t = a  # fCurrentSimulationTime at the start of the run
+        generated = 0
+        while True:
+            act = activity * np.exp(-lam * (t - start_time)) if lam > 0 else activity
+            if not (max_n <= 0 or max_n > generated):
+                break
+            if t < start_time:
+                nt = start_time
+            elif t >= end_time:
+                break
+            else:
+                nt = t if max_n > 0 else t - np.log(uniform()) / act
+                if nt >= end_time:
+                    break
+            if not (a <= nt < b):  # PrepareNextSource keeps times inside the run interval
+                break
+            t = nt
+            generated += 1
+        counts.append(generated)
+    return counts
+
+
+# ------------------------------------------------------------------ uncertainty goal
+def dose_mean_uncertainty_reference(edep, edep2, n_events, top_count, thresh):
+    """GateDoseActor::ComputeMeanUncertainty and GetMeanOfHighestNValues
+    (GateDoseActor.cpp:281-330, 401-428), voxel by voxel as the C++ iterates (a min-heap of
+    the top values, then the thresholded mean). Pure Python: small images only."""
+    import heapq
+
+    flat = [float(v) for v in np.asarray(edep, dtype=np.float64).ravel()]
+    flat2 = [float(v) for v in np.asarray(edep2, dtype=np.float64).ravel()]
+    heap = []
+    for v in flat:
+        if len(heap) < top_count:
+            heapq.heappush(heap, v)
+        elif v > heap[0]:
+            heapq.heapreplace(heap, v)
+    mean_max = sum(heap) / len(heap) if heap else 0.0
+    n = float(n_events)
+    if n < 2.0:
+        n = 2.0
+    mean_unc, n_vox = 0.0, 0
+    for v, v2 in zip(flat, flat2):
+        if v > mean_max * thresh:
+            v /= n
+            n_vox += 1
+            sq = v2 / n
+            u = (1.0 / (n - 1.0)) * (sq - v ** 2)
+            if u < 0:
+                u = 0.0
+            u = u ** 0.5 / v
+            if u > 1:
+                u = 1.0
+            mean_unc += u
+    if n_vox > 0 and mean_unc > 0:
+        return mean_unc / n_vox
+    return 1.0

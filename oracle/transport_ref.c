@@ -215,8 +215,31 @@ static float qgrid(const float *ys, int n, float x) {
 typedef struct {
   const gh_physics_desc *d;
   int nb1;
-  float inv_rho[256];
+  /* per material: 1/density (dose), the factor of its table row (density with shared
+   * compositions, 1 with per-material tables) and that row (composition or material) */
+  float *inv_rho, *rho;
+  int *row;
 } phys_t;
+
+static void phys_init(phys_t *P, const gh_physics_desc *d) {
+  const int M = d->n_materials;
+  P->d = d;
+  P->nb1 = d->n_buckets + 1;
+  P->inv_rho = (float *)malloc((size_t)M * sizeof(float));
+  P->rho = (float *)malloc((size_t)M * sizeof(float));
+  P->row = (int *)malloc((size_t)M * sizeof(int));
+  for (int i = 0; i < M; ++i) {
+    P->inv_rho[i] = d->density[i] > 0.0f ? 1.0f / d->density[i] : 0.0f;
+    P->rho[i] = d->n_compositions > 0 ? d->density[i] : 1.0f;
+    P->row[i] = d->n_compositions > 0 ? d->composition[i] : i;
+  }
+}
+
+static void phys_free(phys_t *P) {
+  free(P->inv_rho);
+  free(P->rho);
+  free(P->row);
+}
 
 typedef struct {
   int bucket;
@@ -234,6 +257,11 @@ static void energy_index(const phys_t *P, float E, eidx_t *x) {
 
 static float node_interp(const float *row, const eidx_t *x) {
   return fmaf(x->frac, row[x->bucket + 1] - row[x->bucket], row[x->bucket]);
+}
+
+/* attenuation coefficient of material m at the energy of x: its row times its factor */
+static float material_mu(const phys_t *P, int m, const eidx_t *x) {
+  return P->rho[m] * node_interp(P->d->mu_total + (size_t)P->row[m] * P->nb1, x);
 }
 
 /* Kahn's method for Klein-Nishina, then acceptance with S(q)/Z */
@@ -382,6 +410,10 @@ typedef struct {
   float inv_s[3], L[3], tinv[3], d_ct, d_phi;
   int aligned;
 } sim_t;
+
+static int label_at(const sim_t *S, int v) {
+  return S->ph->label_bytes == 2 ? (int)((const uint16_t *)S->ph->labels)[v] : (int)((const uint8_t *)S->ph->labels)[v];
+}
 
 static int voxel_of(const sim_t *S, const float p[3]) {
   int ix = (int)floorf(p[0] * S->inv_s[0]);
@@ -597,7 +629,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
   int birth = !in_box;
   if (ph.E < d->e_cut || (tle && ph.E <= tle_emin)) {
     if (in_box) {
-      int lab = S->ph->labels[bvox];
+      int lab = label_at(S, bvox);
       float x = ph.E * ph.w;
       score(t, &sl, tally_index(S, ph.p), x, x * P->inv_rho[lab], st);
       if (det >= 0 && lab >= det) ev += x;
@@ -635,10 +667,10 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         if (vox < 0) { outcome = 2; break; }
         ph.p[0] = q[0]; ph.p[1] = q[1]; ph.p[2] = q[2];
         st->c[K_STEPS]++;
-        lab = S->ph->labels[vox];
-        float mu = node_interp(d->mu_total + (size_t)lab * P->nb1, &ex);
+        lab = label_at(S, vox);
+        float mu = material_mu(P, lab, &ex);
         if (tle && ph.E <= tle_emax) {
-          const float x = (ph.E * ph.w) * (node_interp(d->mu_en + (size_t)lab * P->nb1, &ex) * ex.inv_maj);
+          const float x = (ph.E * ph.w) * ((P->rho[lab] * node_interp(d->mu_en + (size_t)P->row[lab] * P->nb1, &ex)) * ex.inv_maj);
           score(t, &sl, tally_index(S, ph.p), x, x * P->inv_rho[lab], st);
         }
         if ((float)((w[2 * k + 1] >> 8) | 1u) * 0x1p-24f * ex.maj < mu) { outcome = 1; break; }
@@ -646,8 +678,8 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
       if (outcome == 0) continue;
       if (outcome == 1) {
         float r = uni(&rs);
-        float pc = node_interp(d->p_compton + (size_t)lab * P->nb1, &ex);
-        float pr = node_interp(d->p_rayleigh + (size_t)lab * P->nb1, &ex);
+        float pc = node_interp(d->p_compton + (size_t)P->row[lab] * P->nb1, &ex);
+        float pr = node_interp(d->p_rayleigh + (size_t)P->row[lab] * P->nb1, &ex);
         int tv = tally_index(S, ph.p);
         float irho = P->inv_rho[lab];
         int is_det = det >= 0 && lab >= det;
@@ -655,7 +687,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         if (r < pc) {
           st->c[K_COMPT]++;
           float qq, c;
-          compton_sample(&rs, P, lab, ph.E, &qq, &c, &st->c[K_REJ]);
+          compton_sample(&rs, P, P->row[lab], ph.E, &qq, &c, &st->c[K_REJ]);
           float e1 = ph.E / qq;
           float x = (ph.E - e1) * ph.w;
           if (!tle_on) {
@@ -676,7 +708,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
           energy_index(P, ph.E, &ex);
         } else if (r < pr) {
           st->c[K_RAYL]++;
-          float c = rayleigh_sample(&rs, P, lab, ph.E);
+          float c = rayleigh_sample(&rs, P, P->row[lab], ph.E);
           float cp, sp;
           ref_azimuth(&rs, &cp, &sp);
           scatter_dir(ph.d, c, cp, sp);
@@ -733,7 +765,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         if (birth) { rs.pos = 4; birth = 0; }
         continue;
       }
-      float muw = node_interp(d->mu_total + (size_t)wm * P->nb1, &ex);
+      float muw = material_mu(P, wm, &ex);
       float tw = world_distance(S, ph.p, ph.d);
       float dist = (muw > 0.0f) ? (-ref_log(uni(&rs)) / muw) : INFINITY;
       if (birth) { rs.pos = 4; birth = 0; }
@@ -750,11 +782,11 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
       for (int i = 0; i < 3; ++i) ph.p[i] = fmaf(dist, ph.d[i], ph.p[i]);
       st->c[K_WORLD]++;
       float r = uni(&rs);
-      float pc = node_interp(d->p_compton + (size_t)wm * P->nb1, &ex);
-      float pr = node_interp(d->p_rayleigh + (size_t)wm * P->nb1, &ex);
+      float pc = node_interp(d->p_compton + (size_t)P->row[wm] * P->nb1, &ex);
+      float pr = node_interp(d->p_rayleigh + (size_t)P->row[wm] * P->nb1, &ex);
       if (r < pc) {
         float qq, c;
-        compton_sample(&rs, P, wm, ph.E, &qq, &c, &st->c[K_REJ]);
+        compton_sample(&rs, P, P->row[wm], ph.E, &qq, &c, &st->c[K_REJ]);
         float e1 = ph.E / qq;
         st->e[EN_WORLD] += (double)((ph.E - e1) * ph.w);
         float cp, sp;
@@ -768,7 +800,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         }
         energy_index(P, ph.E, &ex);
       } else if (r < pr) {
-        float c = rayleigh_sample(&rs, P, wm, ph.E);
+        float c = rayleigh_sample(&rs, P, P->row[wm], ph.E);
         float cp, sp;
         ref_azimuth(&rs, &cp, &sp);
         scatter_dir(ph.d, c, cp, sp);
@@ -813,9 +845,7 @@ int ref_run(const gh_physics_desc *phys, const gh_phantom_desc *ph, const gh_sou
   if (!phys || !ph || !src || !tal || !edep || n_threads < 1) return 1;
   if ((rec && rec[0]) || event_edep) n_threads = 1;
   phys_t P;
-  P.d = phys;
-  P.nb1 = phys->n_buckets + 1;
-  for (int i = 0; i < phys->n_materials && i < 256; ++i) P.inv_rho[i] = phys->density[i] > 0.0f ? 1.0f / phys->density[i] : 0.0f;
+  phys_init(&P, phys);
   sim_t S;
   S.ph = ph;
   S.src = src;
@@ -881,6 +911,7 @@ int ref_run(const gh_physics_desc *phys, const gh_phantom_desc *ph, const gh_sou
   }
   free(W);
   free(T);
+  phys_free(&P);
   return 0;
 }
 
@@ -894,7 +925,8 @@ void ref_hu_to_labels(const int16_t *hu, int64_t n, const double *bins, int n_bi
 }
 
 /* Sampler hooks for the distribution tests: n draws of the Compton (q = E/E', cos)
- * and Rayleigh (cos) samplers for material m at energy E, stream (seed, history i). */
+ * and Rayleigh (cos) samplers of table row m (a material, or a composition when the
+ * materials share compositions) at energy E, stream (seed, history i). */
 void ref_sample_compton(const gh_physics_desc *phys, int m, float E, uint64_t n, uint64_t seed, float *q_out,
                         float *c_out) {
   phys_t P;
@@ -919,14 +951,15 @@ void ref_sample_rayleigh(const gh_physics_desc *phys, int m, float E, uint64_t n
   }
 }
 
-/* interpolated table value at E (mu_total row of material m), as the transport sees it */
+/* attenuation coefficient of material m at E, as the transport sees it */
 float ref_mu_at(const gh_physics_desc *phys, int m, float E) {
   phys_t P;
-  P.d = phys;
-  P.nb1 = phys->n_buckets + 1;
+  phys_init(&P, phys);
   eidx_t x;
   energy_index(&P, E, &x);
-  return node_interp(phys->mu_total + (size_t)m * P.nb1, &x);
+  const float mu = material_mu(&P, m, &x);
+  phys_free(&P);
+  return mu;
 }
 
 /* ------------------------------------------------------------------ learned exit-plane source

@@ -153,3 +153,36 @@ def test_tle_dose_actor_user_info():
         tle.tle_energy_bounds()
     with pytest.raises(Exception):
         tle.database = "simulated"
+
+
+def test_uncertainty_goal_rule_matches_the_reference_loop():
+    """dose_actor_mean_uncertainty (vectorised) against the oracle's voxel loop of
+    GateDoseActor::ComputeMeanUncertainty: per-voxel clamp at 1, negative variances at 0,
+    n raised to 2, no qualifying voxel -> 1; and the truncated next-check estimate."""
+    from oracle import oracle as O
+    from opengate_amd.actors.dose import dose_actor_mean_uncertainty, next_uncertainty_check
+
+    g = np.random.default_rng(4)
+    for trial in range(40):
+        n = int(g.integers(0, 60))
+        e = g.exponential(1.0, size=(5, 6, 7)) * (g.random((5, 6, 7)) < 0.7)
+        # squared sums from 1 (a single deposit) to n (spread) times the mean, and some that
+        # make the variance negative
+        e2 = e * e * g.uniform(0.0, 1.5, size=e.shape) / max(n, 1) * g.integers(1, 4)
+        top = int(g.integers(1, 300))
+        th = float(g.uniform(0.0, 1.0))
+        a = dose_actor_mean_uncertainty(e, e2, n, top, th)
+        b = O.dose_mean_uncertainty_reference(e, e2, n, top, th)
+        assert a == pytest.approx(b, rel=1e-12, abs=1e-15), trial
+    assert dose_actor_mean_uncertainty(np.zeros(8), np.zeros(8), 100, 10, 0.7) == 1.0
+    # Bernoulli voxels: k of n events deposit 1 (S = S2 = k, p = k / n):
+    # unc = sqrt((p - p^2) / (n - 1)) / p = sqrt((1 - p) / (p (n - 1)))
+    n = 400
+    k = np.array([100.0, 120.0, 150.0, 400.0])
+    p = k / n
+    exp = np.mean(np.sqrt((1 - p) / (p * (n - 1))))
+    assert dose_actor_mean_uncertainty(k, k, n, 10, 0.1) == pytest.approx(exp, rel=1e-12)
+    # one deposit in n events: unc = 1 (the clamp holds it there)
+    assert dose_actor_mean_uncertainty(np.ones(3), np.ones(3), n, 10, 0.5) == pytest.approx(1.0)
+    assert next_uncertainty_check(0.1, 0.05, 1000, 1.05) == 4200
+    assert next_uncertainty_check(0.0999, 0.05, 1001, 1.0) == int((0.0999 / 0.05) * (0.0999 / 0.05) * 1001)

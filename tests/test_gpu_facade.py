@@ -110,14 +110,59 @@ def test_ct256_labels_and_transport_parity(engine):
 
 
 def test_uncertainty_goal_stops_early(tmp_path):
+    """The run stops at the first check whose mean uncertainty (the reference's rule,
+    restated by the oracle from the output images) is at or below the goal; every earlier
+    check was above it and set the next one at (u / goal)^2 n overshoot (truncated)."""
+    from opengate_amd.actors.dose import next_uncertainty_check
+
     sim, dose, stats = build_sim(tmp_path)
     sim.sources["tc99m"].n = 2_000_000
     dose.uncertainty_goal = 0.05
     dose.uncertainty_first_check_after_n_events = 20_000
     print("[gpu case] uncertainty goal", flush=True)
     sim.run()
-    assert stats.counts["events"] < 2_000_000
-    assert sim.engine.goal_uncertainty <= 0.05
+    n = stats.counts["events"]
+    assert n < 2_000_000
+    checks = sim.engine.goal_checks
+    assert checks[0][0] == 20_000 and checks[-1][0] == n
+    assert checks[-1][1] <= 0.05 and all(u > 0.05 for _, u in checks[:-1])
+    for (n0, u0), (n1, _) in zip(checks[:-1], checks[1:]):
+        assert n1 == max(n0 + 1, next_uncertainty_check(u0, 0.05, n0, dose.uncertainty_overshoot_factor_N_events))
+    t = sim.engine.engine.get_tally(0)
+    u = O.dose_mean_uncertainty_reference(t["edep"], t["edep2"], n, dose.uncertainty_top_voxels_count,
+                                          dose.uncertainty_voxel_edep_threshold)
+    assert u == pytest.approx(checks[-1][1], rel=1e-9)
+
+
+def test_runs_split_by_run_timing_intervals(tmp_path):
+    """Two run intervals with number_of_primaries per run: the histories of both runs
+    continue one stream, so the result equals one run of their sum; an activity source
+    gives Poisson counts per run, and the statistics actor counts the runs."""
+    u = gate.g4_units
+    out = []
+    for split in (True, False):
+        sim, dose, stats = build_sim(tmp_path / ("split" if split else "one"))
+        if split:
+            sim.run_timing_intervals = [[0, 1 * u.s], [1 * u.s, 3 * u.s]]
+            sim.sources["tc99m"].number_of_primaries = [15_000, 25_000]
+            sim.sources["tc99m"].n = 0
+        print(f"[gpu case] run split {split}", flush=True)
+        sim.run()
+        out.append((stats.counts["events"], stats.counts["runs"], dose.edep.image.array.copy()))
+    assert out[0][0] == out[1][0] == 40_000
+    assert out[0][1] == 2 and out[1][1] == 1
+    assert np.max(np.abs(out[0][2] - out[1][2])) <= 1e-9 * out[1][2].max()
+    sim, dose, stats = build_sim(tmp_path / "act")
+    sim.run_timing_intervals = [[0, 1 * u.s], [1 * u.s, 2 * u.s]]
+    src = sim.sources["tc99m"]
+    src.n = 0
+    src.activity = 30_000 * u.Bq
+    print("[gpu case] activity per run", flush=True)
+    sim.run()
+    counts = sim.engine.events_per_source["tc99m"]
+    assert len(counts) == 2 and stats.counts["events"] == sum(counts)
+    for c in counts:
+        assert abs(c - 30_000) < 5 * np.sqrt(30_000)
 
 
 def test_tle_dose_actor_script_matches_oracle(tmp_path):
