@@ -222,6 +222,11 @@ int gh_get_event_edep(gh_ctx *ctx, int32_t sim_id, float *out, uint64_t n);
  * (numpy.digitize with ascending bins, right=False). */
 int gh_hu_to_labels(gh_ctx *ctx, const int16_t *hu, int64_t n_voxels, const double *bins,
                     int32_t n_bins, const uint16_t *labels_sorted, uint16_t *out);
+/* The same rule for images of any other type (create_label_image takes any dtype,
+ * volumes.py:1233-1239): values as FP64, which holds integers and float32 exactly. Up to
+ * 4096 bins in either form. */
+int gh_hu_to_labels_f64(gh_ctx *ctx, const double *hu, int64_t n_voxels, const double *bins,
+                        int32_t n_bins, const uint16_t *labels_sorted, uint16_t *out);
 
 /* ---- dense-layer GEMM for the learned models (no reference counterpart: the reference
  * runs its GAN source and ARF networks through torch, gansources.py:512, arfactors.py:388).

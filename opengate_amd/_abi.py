@@ -141,6 +141,8 @@ SIGNATURES = {
     "gh_get_event_edep": (C.c_int, [C.c_void_p, C.c_int32, FP, C.c_uint64]),
     "gh_hu_to_labels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int16), C.c_int64, DP, C.c_int32,
                                   C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]),
+    "gh_hu_to_labels_f64": (C.c_int, [C.c_void_p, DP, C.c_int64, DP, C.c_int32, C.POINTER(C.c_uint16),
+                                      C.POINTER(C.c_uint16)]),
     "gh_device_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
     "gh_gemm_bf16": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_int32,
                                C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,

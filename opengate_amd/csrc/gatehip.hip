@@ -984,17 +984,23 @@ __global__ void phsp_check_kernel(const float *__restrict__ E, const float *__re
 }
 
 // ---------------------------------------------------------------------- HU -> labels
-__global__ void hu_to_labels_kernel(const int16_t *__restrict__ hu, int64_t n, const double *__restrict__ bins,
-                                    int n_bins, const uint16_t *__restrict__ lut, uint16_t *__restrict__ out) {
-  __shared__ double sb[256];
-  __shared__ uint16_t sl[257];
+// label = lut[#{k : bins[k] <= hu}] (numpy.digitize, right=False), bins and lut in LDS
+// (dynamic: n_bins doubles, then n_bins + 1 uint16); HU as int16 or as FP64 (any image
+// type whose values FP64 holds exactly: integers, float32, float64)
+constexpr int HU_MAX_BINS = 4096;
+template <typename T>
+__global__ void hu_to_labels_kernel(const T *__restrict__ hu, int64_t n, const double *__restrict__ bins, int n_bins,
+                                    const uint16_t *__restrict__ lut, uint16_t *__restrict__ out) {
+  extern __shared__ double hu_lds[];
+  double *sb = hu_lds;
+  uint16_t *sl = (uint16_t *)(hu_lds + n_bins);
   for (int i = threadIdx.x; i < n_bins; i += blockDim.x) sb[i] = bins[i];
   for (int i = threadIdx.x; i <= n_bins; i += blockDim.x) sl[i] = lut[i];
   __syncthreads();
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const double x = (double)hu[i];
-    // number of bins <= x (digitize, right=False)
+    // number of bins <= x (digitize, right=False); NaN compares false everywhere
     int lo = 0, hi = n_bins;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
@@ -1894,17 +1900,19 @@ int gh_alias_table(const double *w, int64_t n, float *prob, int32_t *alias) {
   return GH_OK;
 }
 
-int gh_hu_to_labels(gh_ctx *ctx, const int16_t *hu, int64_t n_voxels, const double *bins, int32_t n_bins,
-                    const uint16_t *labels_sorted, uint16_t *out) {
+extern "C++" {
+template <typename T>
+static int hu_to_labels_impl(gh_ctx *ctx, const T *hu, int64_t n_voxels, const double *bins, int32_t n_bins,
+                             const uint16_t *labels_sorted, uint16_t *out, const char *fname) {
   if (!ctx) return GH_EINVAL;
-  if (!hu || !bins || !labels_sorted || !out || n_voxels < 0) FAIL(GH_EINVAL, "gh_hu_to_labels: null argument");
-  if (n_bins < 0 || n_bins > 256) FAIL(GH_ERANGE, "gh_hu_to_labels: at most 256 bins");
+  if (!hu || !bins || !labels_sorted || !out || n_voxels < 0) FAIL(GH_EINVAL, std::string(fname) + ": null argument");
+  if (n_bins < 0 || n_bins > HU_MAX_BINS) FAIL(GH_ERANGE, std::string(fname) + ": at most 4096 bins");
   for (int i = 1; i < n_bins; ++i)
-    if (!(bins[i - 1] <= bins[i])) FAIL(GH_EINVAL, "gh_hu_to_labels: bins must be ascending");
+    if (!(bins[i - 1] <= bins[i])) FAIL(GH_EINVAL, std::string(fname) + ": bins must be ascending");
   if (n_voxels == 0) return GH_OK;
   HIPCHK(hipSetDevice(ctx->device));
   std::vector<void *> tmp;
-  int16_t *d_hu;
+  T *d_hu;
   double *d_bins;
   uint16_t *d_lut, *d_out;
   int rc;
@@ -1916,14 +1924,26 @@ int gh_hu_to_labels(gh_ctx *ctx, const int16_t *hu, int64_t n_voxels, const doub
   const int block = 256;
   const int64_t want = (n_voxels + block - 1) / block;
   const int grid = (int)std::min<int64_t>(want, (int64_t)std::max(ctx->n_cu, 1) * 16);
-  hipLaunchKernelGGL(hu_to_labels_kernel, dim3(grid), dim3(block), 0, ctx->stream, d_hu, n_voxels, d_bins,
+  const size_t shmem = (size_t)n_bins * sizeof(double) + ((size_t)n_bins + 1) * sizeof(uint16_t);
+  hipLaunchKernelGGL(hu_to_labels_kernel<T>, dim3(grid), dim3(block), shmem, ctx->stream, d_hu, n_voxels, d_bins,
                      n_bins, d_lut, d_out);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n_voxels * sizeof(uint16_t), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   free_all(tmp);
-  if (e != hipSuccess) FAIL(GH_EHIP, std::string("gh_hu_to_labels: ") + hipGetErrorString(e));
+  if (e != hipSuccess) FAIL(GH_EHIP, std::string(fname) + ": " + hipGetErrorString(e));
   return GH_OK;
+}
+}  // extern "C++"
+
+int gh_hu_to_labels(gh_ctx *ctx, const int16_t *hu, int64_t n_voxels, const double *bins, int32_t n_bins,
+                    const uint16_t *labels_sorted, uint16_t *out) {
+  return hu_to_labels_impl(ctx, hu, n_voxels, bins, n_bins, labels_sorted, out, "gh_hu_to_labels");
+}
+
+int gh_hu_to_labels_f64(gh_ctx *ctx, const double *hu, int64_t n_voxels, const double *bins, int32_t n_bins,
+                        const uint16_t *labels_sorted, uint16_t *out) {
+  return hu_to_labels_impl(ctx, hu, n_voxels, bins, n_bins, labels_sorted, out, "gh_hu_to_labels_f64");
 }
 
 }  // extern "C"

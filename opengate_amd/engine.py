@@ -294,12 +294,17 @@ class SimulationEngine:
             raise ValueError("no source with primaries (set source.number_of_primaries or source.activity)")
         self.events_per_source = {s.name: counts for s, counts in per_run}
         e_max = max(spec.max_energy() for _, spec, _ in runs)
-        e_cut = max(float(sim.physics_manager.photon_tracking_cut), 2.0 ** -10)
+        checked = list(mats)
+        if spect.spect_actors(sim):
+            from .learn.arf import HEAD_MATERIALS
+
+            checked += [sim.volume_manager.material_database.get(n) for n in HEAD_MATERIALS]
+        e_cut = max(sim.physics_manager.resolve_photon_cut(checked, e_max), 2.0 ** -10)
         tables = xs.build_tables(mats, e_max * (1 + 1e-6), e_cut=e_cut)
         self.spect = None
         n_total = sum(n for _, _, n in runs)
         if spect.spect_actors(sim):
-            self.spect = spect.SpectProjections(self, seed, e_max)
+            self.spect = spect.SpectProjections(self, seed, e_max, e_cut)
         tal = self.tally_spec(vol, ph, n_total)
         self.engine.set_physics(tables)
         sid = self.engine.add_simulation(ph, runs[0][1], tal)

@@ -17,23 +17,41 @@ from ..physics.elements import symbol_of, z_of
 
 
 def read_voxel_materials(filename, def_mat="G4_AIR"):
-    """Text table 'hu_min hu_max material' -> [[lo, hi, name], ...]; gaps are filled with
-    def_mat (materials.py:14-58)."""
-    rows = []
+    """Text table of (hu_min, hu_max, material) triples -> [[lo, hi, name], ...], as the
+    reference reads it (materials.py:14-58): the words of the file form triples in order
+    (a word starting with '#' ends its line), the triples are sorted, and a gap before an
+    interval adds a def_mat row [previous, lo] followed by the interval's row starting at
+    `previous` as well (so a gap makes two rows overlap, which the label table then
+    rejects); a previous bound of 0 counts as no bound. Overlapping intervals and an
+    interval with lo > hi are errors."""
+    triples, state, start, stop = [], 0, None, None
     with open(filename) as f:
         for line in f:
-            w = line.split()
-            if not w or w[0].startswith("#"):
-                continue
-            rows.append([float(w[0]), float(w[1]), w[2]])
-    pix = []
-    previous = None
-    for lo, hi, name in rows:
-        if previous is not None and previous != lo:
-            pix.append([previous, lo, def_mat])
-        pix.append([lo, hi, name])
+            for word in line.split():
+                if word[0] == "#":
+                    break
+                if state == 0:
+                    start, state = float(word), 1
+                elif state == 1:
+                    stop, state = float(word), 2
+                else:
+                    triples.append([start, stop, word])
+                    state = 0
+    triples.sort()
+    rows = []
+    previous = triples[0][0]
+    for lo, hi, name in triples:
+        if previous and previous > lo:
+            raise ValueError(f"Error while reading {filename}\nIntervals are not disjoint: {previous} {[lo, hi, name]}")
+        if lo > hi:
+            raise ValueError(f"Error while reading {filename}\nWrong interval {[lo, hi, name]}")
+        if not previous or previous == lo:
+            rows.append([previous, hi, name])
+        else:
+            rows.append([previous, lo, def_mat])
+            rows.append([previous, hi, name])
         previous = hi
-    return pix
+    return rows
 
 
 def HU_read_materials_table(file_mat):
@@ -156,9 +174,15 @@ def HounsfieldUnit_to_material(simulation, density_tolerance, file_mat, file_den
     return voxel_materials, created
 
 
-def create_density_array(hu, voxel_materials, material_database):
-    """rho[g/cm3] per voxel from [hu0, hu1) intervals (materials.py:294-326)."""
+def create_density_img(img_volume, material_database):
+    """Density image [g/cm3] of an ImageVolume: each voxel gets the density of the material
+    whose [hu0, hu1) interval holds its value, 0 outside every interval; same size and
+    geometry as the image (materials.py:294-326)."""
+    from ..image import Image
+
+    img = img_volume.itk_image
+    hu = np.asarray(img.array)
     rho = np.zeros(hu.shape, dtype=np.float32)
-    for lo, hi, name in voxel_materials:
-        rho[(hu >= lo) & (hu < hi)] = material_database.get(name).density
-    return rho
+    for hu0, hu1, name in img_volume.voxel_materials:
+        rho[(hu >= hu0) & (hu < hu1)] = material_database.get(name).density
+    return Image(rho, img.spacing, img.origin, img.direction)

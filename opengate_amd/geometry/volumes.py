@@ -34,6 +34,21 @@ class VolumeBase(GateObject):
     def type_name(self):
         return self.volume_type
 
+    def _physics_manager(self):
+        if self.volume_manager is None:
+            raise RuntimeError(f"volume '{self.name}' is not in a simulation")
+        return self.volume_manager.simulation.physics_manager
+
+    # volumes.py:526-548 of the reference: per-volume physics settings create a region
+    def set_production_cut(self, particle_name, value):
+        self._physics_manager().set_production_cut(self.name, particle_name, value)
+
+    def set_min_ekine(self, min_ekine):
+        self._physics_manager().set_min_ekine(self.name, min_ekine)
+
+    def set_max_step_size(self, max_step_size):
+        self._physics_manager().set_max_step_size(self.name, max_step_size)
+
 
 class BoxVolume(VolumeBase):
     user_info_defaults = {"size": ([10.0, 10.0, 10.0], {"doc": "full size [mm]"})}
@@ -125,17 +140,22 @@ class ImageVolume(VolumeBase):
             labels_sorted.append(l)
         return np.array(bins_sorted, dtype=np.float64), np.array(labels_sorted, dtype=np.uint16)
 
+    def create_density_image(self):
+        """Density [g/cm3] per voxel of the image (ImageVolume.create_density_image,
+        volumes.py:1294-1295)."""
+        from .materials import create_density_img
+
+        return create_density_img(self, self.volume_manager.material_database)
+
     def create_label_image(self, engine):
         """uint16 label image on the GPU (gh_hu_to_labels)."""
         self.material_to_label_lut = self.create_material_to_label_lut()
         bins, labels_sorted = self.label_bins(self.material_to_label_lut)
         img = self.itk_image
         arr = img.array
-        if arr.dtype != np.int16:
-            if np.issubdtype(arr.dtype, np.integer) and arr.min() >= -32768 and arr.max() <= 32767:
-                arr = arr.astype(np.int16)
-            else:
-                raise TypeError(f"ImageVolume '{self.name}': the GPU label kernel takes int16 images, got {arr.dtype}")
+        if arr.dtype != np.int16 and np.issubdtype(arr.dtype, np.integer) and arr.size and \
+                arr.min() >= -32768 and arr.max() <= 32767:
+            arr = arr.astype(np.int16)  # exact: the int16 kernel moves 2 bytes per voxel
         finite = np.clip(bins, -1e300, 1e300)
         lab = engine.hu_to_labels(arr, finite, labels_sorted)
         self.label_image = Image(lab, img.spacing, -(img.size * img.spacing) / 2.0 + img.spacing / 2.0)

@@ -28,6 +28,8 @@ SIZES = [3, 128, 128, 128, 4]
 # Tc-99m triple energy window (MeV): lower scatter, photopeak 140.5 keV +-10 %, upper scatter
 WINDOWS = ((0.114, 0.126), (0.126, 0.154), (0.154, 0.166))
 E_RANGE = (0.010, 0.160)
+# materials of the head model, by label: gap, collimator, crystal
+HEAD_MATERIALS = ("G4_AIR", "G4_Pb", "G4_SODIUM_IODIDE")
 
 
 @dataclass(frozen=True)
@@ -137,16 +139,17 @@ class HeadModel:
         lab[nzc + nzg:] = col[None]
         return lab
 
-    def simulation(self, capacity_events=0, e_max=None):
+    def simulation(self, capacity_events=0, e_max=None, e_cut=None):
         """(tables, phantom, tally) of the head model; the tally scores the per-event NaI energy.
-        The physics tables cover E_RANGE and e_max (MeV) if larger."""
+        The physics tables cover E_RANGE and e_max (MeV) if larger; photons below e_cut (MeV,
+        None = the lowest table energy) deposit where they are."""
         from ..physics import xs
         from ..physics.materials import MaterialDatabase
         from ..transport import PhantomSpec, TallySpec
 
         db = MaterialDatabase()
         top = max(E_RANGE[1], float(e_max or 0.0))
-        tables = xs.build_tables([db.get("G4_AIR"), db.get("G4_Pb"), db.get("G4_SODIUM_IODIDE")], top * 1.001)
+        tables = xs.build_tables([db.get(n) for n in HEAD_MATERIALS], top * 1.001, e_cut=e_cut)
         ph = PhantomSpec(self.labels(), (self.voxel,) * 3, world_material=-1)
         tal = TallySpec(size=(1, 1, 1), spacing=(self.patch, self.patch, self.face_z), aligned=False, edep2=False,
                         detector_label_min=2)
@@ -361,12 +364,12 @@ class ArfModel:
 class HeadSimulator:
     """Plain transport through the head model: labels photons with their energy window."""
 
-    def __init__(self, model=None, device=0, e_max=None):
+    def __init__(self, model=None, device=0, e_max=None, e_cut=None):
         from ..transport import GpuEngine
 
         self.model = model or HeadModel()
         self.engine = GpuEngine(device)
-        self.tables, self.phantom, self.tally = self.model.simulation(e_max=e_max)
+        self.tables, self.phantom, self.tally = self.model.simulation(e_max=e_max, e_cut=e_cut)
         self.engine.set_physics(self.tables)
         self.sid = None
 

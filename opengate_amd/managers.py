@@ -28,17 +28,102 @@ from .units import g4_units
 VACUUM_MATERIALS = ("G4_Galactic", "Vacuum", "G4_VACUUM")
 
 
+# Physics-list names the reference builds (opengate/physics.py:241-339, :341-353): the
+# reference lists alone or with an EM extension, the special builders, and the single
+# EM constructors. Any other name is an error there (create_physics_list, :405-417).
+PHYSICS_LIST_BASES = ("FTFP_BERT", "FTFP_BERT_ATL", "FTFP_BERT_HP", "FTFP_BERT_TRV", "FTFQGSP_BERT", "FTFP_INCLXX",
+                      "FTFP_INCLXX_HP", "FTF_BIC", "LBE", "NuBeam", "QBBC", "QGSP_BERT", "QGSP_BERT_HP", "QGSP_BIC",
+                      "QGSP_BIC_HP", "QGSP_BIC_AllHP", "QGSP_BIC_HPT", "QGSP_FTFP_BERT", "QGSP_INCLXX",
+                      "QGSP_INCLXX_HP", "QGS_BIC", "Shielding", "ShieldingLEND")
+EM_EXTENSIONS = {"_EM0": "G4EmStandardPhysics", "_EMV": "G4EmStandardPhysics_option1",
+                 "_EMX": "G4EmStandardPhysics_option2", "_EMY": "G4EmStandardPhysics_option3",
+                 "_EMZ": "G4EmStandardPhysics_option4", "_LIV": "G4EmLivermorePhysics",
+                 "_PEN": "G4EmPenelopePhysics", "__GS": "G4EmStandardPhysicsGS", "__LE": "G4EmLowEPPhysics"}
+SPECIAL_BUILDERS = ("Shielding_HP", "ShieldingM", "ShieldingM_HP", "ShieldingLIQMD", "ShieldingLIQMD_HP",
+                    "FTFP_BERT_HPT", "FTFP_INCLXX_HPT", "QGSP_BERT_HPT", "QGSP_BIC_AllHPT", "QGSP_INCLXX_HPT",
+                    "Shielding_HPT", "ShieldingLIQMD_HPT", "ShieldingM_HPT")
+EM_CONSTRUCTORS = ("G4EmStandardPhysics", "G4EmStandardPhysics_option1", "G4EmStandardPhysics_option2",
+                   "G4EmStandardPhysics_option3", "G4EmStandardPhysics_option4", "G4EmStandardPhysicsGS",
+                   "G4EmLowEPPhysics", "G4EmLivermorePhysics", "G4EmLivermorePolarizedPhysics", "G4EmPenelopePhysics",
+                   "G4OpticalPhysics")
+# EM constructors that switch atomic de-excitation (fluorescence) on by default
+FLUO_BY_DEFAULT = ("G4EmStandardPhysics_option3", "G4EmStandardPhysics_option4", "G4EmLivermorePhysics",
+                   "G4EmLivermorePolarizedPhysics", "G4EmPenelopePhysics", "G4EmLowEPPhysics")
+# Heaviest element whose K x-rays (<= 3.7 keV, Ca) are absorbed within about 0.1 mm of
+# tissue or bone: with fluorescence on, a material holding only such elements gets the
+# same dose on a mm grid as the local deposit of the binding energy done here.
+FLUO_LOCAL_Z_MAX = 20
+# particle names user limits may name (GATE names, then Geant4 names; physics.py:18-36)
+USER_LIMIT_PARTICLES = ("all", "all_charged", "gamma", "electron", "positron", "proton", "e-", "e+", "neutron",
+                        "alpha", "deuteron", "triton", "He3", "GenericIon", "mu-", "mu+", "pi-", "pi+", "opticalphoton")
+
+
+def available_physics_lists():
+    """Names accepted for physics_list_name (the reference's set, physics.py:288-299)."""
+    names = list(EM_CONSTRUCTORS)
+    for base in PHYSICS_LIST_BASES:
+        names.append(base)
+        names.extend(base + sfx for sfx in EM_EXTENSIONS)
+    return names + list(SPECIAL_BUILDERS)
+
+
+def em_constructor_of(physics_list_name):
+    """The Geant4 EM constructor a physics list name selects; ValueError for a name the
+    reference cannot build."""
+    name = physics_list_name
+    if name in EM_CONSTRUCTORS:
+        return name
+    if name in SPECIAL_BUILDERS:
+        return "G4EmStandardPhysics"
+    for sfx in sorted(EM_EXTENSIONS, key=len, reverse=True):
+        if name.endswith(sfx) and name[: -len(sfx)] in PHYSICS_LIST_BASES:
+            return EM_EXTENSIONS[sfx]
+    if name in PHYSICS_LIST_BASES:
+        # LBE registers the Livermore EM constructor, every other reference list the standard one
+        return "G4EmLivermorePhysics" if name == "LBE" else "G4EmStandardPhysics"
+    raise ValueError(f"Cannot find the physic list: {name}\n"
+                     f"Known lists: {', '.join(available_physics_lists())}\n"
+                     f"Default is {PhysicsManager.user_info_defaults['physics_list_name'][0]}")
+
+
+def _hook_physics_list_name(self, name):
+    # managers.py:677-685 of the reference
+    if str(name).startswith("G4EmDNAPhysics"):
+        raise ValueError(f"Global DNA EM physics lists are not supported in GATE. Received '{name}'.")
+    return name
+
+
+def _hook_user_limits_particles(self, names):
+    # managers.py:688-692 of the reference: one name becomes a list
+    return list(names) if isinstance(names, (list, set, tuple)) else [names]
+
+
 class PhysicsManager(GateObject):
-    runtime_attributes = ("simulation", "min_ekine", "production_cuts")
+    """Physics options of the photon path (opengate/managers.py:695-1130, physics.py).
+
+    The photon model is the same for every physics list: total attenuation from the NIST
+    tables, Compton scattering with the Klein-Nishina cross section times the incoherent
+    scattering function S(q) (no Doppler broadening), Rayleigh scattering with the atomic
+    form factor, photoabsorption with the whole photon energy deposited at the interaction
+    point, electrons deposited where they are set in motion. The options raise where they
+    would make the reference compute something this path does not: an unknown list,
+    optical or polarised photon physics, fluorescence (by the list's default or by
+    em_parameters.fluo) with materials heavier than calcium, PIXE, photon kill energies
+    that differ between volumes, and an electron kill energy below the highest photon
+    energy (the reference transports the electrons above it)."""
+    runtime_attributes = ("simulation", "min_ekine", "production_cuts", "region_volumes")
     user_info_defaults = {
-        "physics_list_name": ("QGSP_BERT_EMV", {"doc": "any EM list: gamma processes are Compton, "
-                                                       "photoelectric and Rayleigh in every case"}),
+        "physics_list_name": ("QGSP_BERT_EMV", {"doc": "Geant4 physics list name; the photon model does not "
+                                                       "depend on it (see check_supported)",
+                                                "setter_hook": _hook_physics_list_name}),
         "global_production_cuts": (Box(all=0.7 * g4_units.mm, gamma=None, electron=None, positron=None,
                                        proton=None), {}),
         "em_parameters": (Box(fluo=None, auger=None, auger_cascade=None, pixe=None, deexcitation_ignore_cut=None),
                           {}),
-        "user_limits_particles": (Box(all=False, all_charged=True, gamma=False, electron=False, positron=False,
-                                      proton=False), {}),
+        "special_physics_constructors": (Box(G4DecayPhysics=False, G4RadioactiveDecayPhysics=False,
+                                             G4OpticalPhysics=False), {}),
+        "user_limits_particles": (["all_charged"], {"doc": "particles to which user limits apply",
+                                                    "setter_hook": _hook_user_limits_particles}),
         "energy_range_min": (None, {}),
         "energy_range_max": (None, {}),
         "apply_cuts": (True, {}),
@@ -52,22 +137,112 @@ class PhysicsManager(GateObject):
         self.simulation = simulation
         self.min_ekine = {}
         self.production_cuts = {}
+        # volumes holding a region of their own (any per-volume setting creates one)
+        self.region_volumes = set()
+
+    def _region(self, volume_name):
+        name = volume_name.name if hasattr(volume_name, "name") else str(volume_name)
+        if name != "world":
+            self.region_volumes.add(name)
+        return name
 
     def set_production_cut(self, volume_name, particle_name, value):
-        self.production_cuts[(volume_name, particle_name)] = value
+        name = self._region(volume_name)
+        if name == "world":
+            self.global_production_cuts[particle_name] = value
+        else:
+            self.production_cuts[(name, particle_name)] = value
 
     def set_min_ekine(self, volume_name, min_ekine):
-        self.min_ekine[volume_name] = min_ekine
+        self.min_ekine[self._region(volume_name)] = min_ekine
 
     def set_max_step_size(self, volume_name, max_step_size):
-        pass  # photon transport with Woodcock tracking has no step limit to apply
+        # a step limit does not change photon transport (photons have no continuous losses)
+        self._region(volume_name)
+
+    def set_user_limits_particles(self, particle_names):
+        raise AttributeError("set_user_limits_particles has been removed: set "
+                             "sim.physics_manager.user_limits_particles = [...] instead")
+
+    @staticmethod
+    def available_physics_lists():
+        return available_physics_lists()
+
+    def dump_available_physics_lists(self):
+        return "\n".join(available_physics_lists())
+
+    def em_constructor(self):
+        return em_constructor_of(self.physics_list_name)
+
+    def _limited(self, *names):
+        ulp = self.user_limits_particles
+        return "all" in ulp or any(n in ulp for n in names)
+
+    def fluorescence_on(self):
+        fluo = self.em_parameters.get("fluo")
+        return bool(fluo) if fluo is not None else self.em_constructor() in FLUO_BY_DEFAULT
 
     def check_supported(self):
-        em = self.em_parameters
-        for k in ("fluo", "auger", "pixe"):
-            if em.get(k):
-                raise NotImplementedError(f"em_parameters.{k} = True: atomic de-excitation is not simulated; "
-                                          "the photon path deposits binding energy locally")
+        """Checks that need no geometry: list name, optical and polarised physics, PIXE,
+        user-limit particle names."""
+        em = self.em_constructor()
+        if em == "G4OpticalPhysics" or self.special_physics_constructors.get("G4OpticalPhysics"):
+            raise NotImplementedError("optical photon physics is not part of the photon dose path")
+        if em == "G4EmLivermorePolarizedPhysics":
+            raise NotImplementedError("G4EmLivermorePolarizedPhysics: polarised Compton and Rayleigh scattering "
+                                      "change the azimuth of every scatter after the first; photons here are "
+                                      "unpolarised")
+        if self.em_parameters.get("pixe"):
+            raise NotImplementedError("em_parameters.pixe = True: x-rays emitted by electrons are not simulated "
+                                      "(electrons deposit where they are set in motion)")
+        unknown = [p for p in self.user_limits_particles if p not in USER_LIMIT_PARTICLES]
+        if unknown:
+            raise ValueError(f"UserLimits were requested for the following unknown particles: {unknown}")
+
+    def resolve_photon_cut(self, materials, e_max):
+        """Checks that need the materials and the highest photon energy e_max [MeV], and
+        the photon kill energy of the run [MeV].
+
+        Photon user limits (physics.py:102-130: G4UserSpecialCuts for the particles named
+        in user_limits_particles, or for all with 'all'; 'all_charged' adds only a step
+        limiter) kill a photon below min_ekine with its energy deposited where it is,
+        which is what the photon tracking cut does here, once for every volume."""
+        self.check_supported()
+        if self.fluorescence_on():
+            heavy = sorted({(m.name, int(z)) for m in materials for z in m.elements if z > FLUO_LOCAL_Z_MAX})
+            if heavy:
+                src = ("em_parameters.fluo = True" if self.em_parameters.get("fluo")
+                       else f"{self.physics_list_name} ({self.em_constructor()} switches fluorescence on)")
+                raise NotImplementedError(
+                    f"{src}: fluorescence x-rays of {', '.join(f'Z={z} in {n}' for n, z in heavy[:4])} "
+                    "travel away from the absorption point, and this path deposits the binding energy "
+                    "there; set sim.physics_manager.em_parameters.fluo = False")
+        volumes = ["world"] + sorted(self.region_volumes)
+
+        def kill_energy(v):
+            e = self.min_ekine.get(v)
+            return 0.0 if e is None else float(e)
+
+        cut = float(self.photon_tracking_cut)
+        if self._limited("gamma"):
+            per_volume = {v: kill_energy(v) for v in volumes}
+            if len(set(per_volume.values())) > 1:
+                raise NotImplementedError(f"photon min_ekine differs between volumes ({per_volume}); this path "
+                                          "applies one photon kill energy everywhere")
+            cut = max(cut, per_volume["world"])
+        if self._limited("electron", "e-"):
+            low = {v: kill_energy(v) for v in volumes if kill_energy(v) < e_max}
+            if low:
+                raise NotImplementedError(
+                    f"electron min_ekine {low} [MeV] is below the highest photon energy {e_max:.6g} MeV: the "
+                    "reference would transport electrons above it, this path deposits every electron where it "
+                    "is set in motion; set the electron min_ekine of every volume to at least that energy")
+        else:
+            note = ("electrons are deposited where they are set in motion; the reference transports them unless "
+                    "user_limits_particles names 'electron' with min_ekine >= the highest photon energy")
+            if self.simulation is not None and note not in self.simulation.warnings:
+                self.simulation.warnings.append(note)
+        return cut
 
 
 class VolumeManager:

@@ -101,7 +101,7 @@ def projection_inputs(sim, names):
 class SpectProjections:
     """Projection consumers of one phantom run."""
 
-    def __init__(self, engine, seed, e_max):
+    def __init__(self, engine, seed, e_max, e_cut=None):
         sim = engine.sim
         self.sim = sim
         self.actors = spect_actors(sim)
@@ -143,7 +143,8 @@ class SpectProjections:
                 if self.head_sim is None:
                     from .learn.arf import HeadSimulator
 
-                    self.head_sim = HeadSimulator(self.head.head_model(), device=sim.device, e_max=e_max)
+                    self.head_sim = HeadSimulator(self.head.head_model(), device=sim.device, e_max=e_max,
+                                                  e_cut=e_cut)
                 from .learn.arf import E_RANGE
 
                 heads = self.head.heads(center, a.size, a.spacing, E_RANGE)
@@ -265,7 +266,11 @@ def run_arf_training(sim, seed):
     if not plans:
         raise ValueError("no source with primaries (set source.number_of_primaries or source.activity)")
     e_range = (e_lo if e_lo < e_hi else 0.0, e_hi)
-    hs = HeadSimulator(head.head_model(), device=sim.device, e_max=e_hi)
+    from .learn.arf import HEAD_MATERIALS
+
+    db = sim.volume_manager.material_database
+    e_cut = max(sim.physics_manager.resolve_photon_cut([db.get(n) for n in HEAD_MATERIALS], e_hi), 2.0 ** -10)
+    hs = HeadSimulator(head.head_model(), device=sim.device, e_max=e_hi, e_cut=e_cut)
     det_seed = derived_seed(seed, 3)
     rows = {"E": [], "Theta": [], "Phi": [], "window": []}
     first = 0

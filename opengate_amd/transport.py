@@ -430,13 +430,24 @@ class GpuEngine:
         return out
 
     def hu_to_labels(self, hu, bins, labels_sorted):
-        hu = np.ascontiguousarray(hu, dtype=np.int16)
+        """label = labels_sorted[digitize(hu, bins)] on the GPU: int16 images as they are,
+        any other type as FP64 (exact for integers and float32)."""
+        hu = np.asarray(hu)
         bins = np.ascontiguousarray(bins, dtype=np.float64)
         lut = np.ascontiguousarray(labels_sorted, dtype=np.uint16)
         if len(lut) != len(bins) + 1:
             raise GhError("labels_sorted must have len(bins)+1 entries")
         out = np.zeros(hu.shape, dtype=np.uint16)
-        self._check(self.lib.gh_hu_to_labels(self.ctx, hu.ctypes.data_as(C.POINTER(C.c_int16)), hu.size,
-                                             A.dptr(bins), len(bins), lut.ctypes.data_as(C.POINTER(C.c_uint16)),
-                                             out.ctypes.data_as(C.POINTER(C.c_uint16))), "gh_hu_to_labels")
+        outp = out.ctypes.data_as(C.POINTER(C.c_uint16))
+        lutp = lut.ctypes.data_as(C.POINTER(C.c_uint16))
+        if hu.dtype == np.int16:
+            hu = np.ascontiguousarray(hu)
+            self._check(self.lib.gh_hu_to_labels(self.ctx, hu.ctypes.data_as(C.POINTER(C.c_int16)), hu.size,
+                                                 A.dptr(bins), len(bins), lutp, outp), "gh_hu_to_labels")
+        else:
+            if hu.dtype.kind not in "iuf" or (hu.dtype.kind in "iu" and hu.dtype.itemsize > 4):
+                raise TypeError(f"HU image of type {hu.dtype}: integers up to 32 bits or floats")
+            h = np.ascontiguousarray(hu, dtype=np.float64)
+            self._check(self.lib.gh_hu_to_labels_f64(self.ctx, A.dptr(h), h.size, A.dptr(bins), len(bins), lutp,
+                                                     outp), "gh_hu_to_labels_f64")
         return out

@@ -128,9 +128,8 @@ def test_uncertainty_goal_stops_early(tmp_path):
     assert checks[-1][1] <= 0.05 and all(u > 0.05 for _, u in checks[:-1])
     for (n0, u0), (n1, _) in zip(checks[:-1], checks[1:]):
         assert n1 == max(n0 + 1, next_uncertainty_check(u0, 0.05, n0, dose.uncertainty_overshoot_factor_N_events))
-    t = sim.engine.engine.get_tally(0)
-    u = O.dose_mean_uncertainty_reference(t["edep"], t["edep2"], n, dose.uncertainty_top_voxels_count,
-                                          dose.uncertainty_voxel_edep_threshold)
+    u = O.dose_mean_uncertainty_reference(dose.edep.image.array, dose.edep_squared.image.array, n,
+                                          dose.uncertainty_top_voxels_count, dose.uncertainty_voxel_edep_threshold)
     assert u == pytest.approx(checks[-1][1], rel=1e-9)
 
 

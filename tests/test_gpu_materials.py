@@ -74,8 +74,8 @@ def test_287_materials_uint16_labels(engine, share):
 def test_power_of_two_densities_shared_equals_per_material(engine):
     mats = water_family()
     ph = PhantomSpec(layered_phantom(len(mats)), (2.5, 2.5, 2.5))
-    src = SourceSpec(pos_trans=(25.0, 22.5, 20.0), energy=0.1405)
-    tal = TallySpec(size=(20, 18, 16), spacing=(2.5, 2.5, 2.5))
+    src = SourceSpec(pos_trans=(20.0, 22.5, 25.0), energy=0.1405)
+    tal = TallySpec(size=(16, 18, 20), spacing=(2.5, 2.5, 2.5))
     out = []
     for share in (True, False):
         T = xs.build_tables(mats, 0.15, share_compositions=share)

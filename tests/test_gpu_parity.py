@@ -435,3 +435,25 @@ def test_hu_to_labels_bit_exact(engine):
     bins, ls = O.label_bins(lut, vm)
     out = engine.hu_to_labels(hu, bins, ls)
     np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.gpu
+def test_hu_to_labels_float_images_and_many_bins(engine):
+    """Float HU images (any dtype, create_label_image, volumes.py:1233-1239) and tables of
+    more than 256 bins, against numpy.digitize itself: values on every bin edge, one ulp
+    either side, NaN and infinities."""
+    g = np.random.default_rng(5)
+    bins = np.unique(np.round(g.uniform(-1100, 3100, size=600), 3)).astype(np.float64)
+    lut = g.integers(0, 700, size=len(bins) + 1).astype(np.uint16)
+    lut[0] = lut[-1] = 0
+    for dt in (np.float32, np.float64, np.int32):
+        edges = bins.astype(dt)
+        vals = np.concatenate([edges, np.nextafter(edges, -np.inf) if dt != np.int32 else edges - 1,
+                               np.nextafter(edges, np.inf) if dt != np.int32 else edges + 1,
+                               g.uniform(-1200, 3200, size=5000).astype(dt)])
+        if dt != np.int32:
+            vals = np.concatenate([vals, np.array([np.nan, np.inf, -np.inf], dtype=dt)])
+        print(f"[gpu case] hu_to_labels {np.dtype(dt).name} {len(bins)} bins", flush=True)
+        out = engine.hu_to_labels(vals.reshape(1, 1, -1), bins, lut)
+        exp = lut[np.digitize(vals.astype(np.float64), bins)]
+        np.testing.assert_array_equal(out.ravel(), exp)

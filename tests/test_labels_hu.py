@@ -96,8 +96,94 @@ def test_schneider_conversion():
 
 
 def test_read_voxel_materials(tmp_path):
-    f = tmp_path / "vm.txt"
-    f.write_text("# lo hi name\n-1000 -500 G4_AIR\n-400 100 G4_WATER\n100 2000 G4_BONE_CORTICAL_ICRP\n")
-    vm = M.read_voxel_materials(f)
-    assert vm == [[-1000, -500, "G4_AIR"], [-500, -400, "G4_AIR"], [-400, 100, "G4_WATER"],
+    """Rows worked out by hand from the reference's reading rule (materials.py:14-58)."""
+    from opengate_amd.geometry.volumes import ImageVolume
+
+    def rows(text, **kw):
+        f = tmp_path / "vm.txt"
+        f.write_text(text)
+        return M.read_voxel_materials(f, **kw)
+
+    # contiguous intervals, unsorted in the file, comments, a triple split across lines
+    assert rows("# lo hi name\n100 2000 BONE\n-1000 -500 AIR # air\n-500\n100 WATER\n") == [
+        [-1000, -500, "AIR"], [-500, 100, "WATER"], [100, 2000, "BONE"]]
+    # a gap: def_mat fills [-500, -400], then WATER's row starts at -500 as well
+    vm = rows("-1000 -500 G4_AIR\n-400 100 G4_WATER\n100 2000 G4_BONE_CORTICAL_ICRP\n")
+    assert vm == [[-1000, -500, "G4_AIR"], [-500, -400, "G4_AIR"], [-500, 100, "G4_WATER"],
                   [100, 2000, "G4_BONE_CORTICAL_ICRP"]]
+    # ... which the label table rejects as overlapping (volumes.py:1144-1168)
+    v = ImageVolume("ct")
+    v.voxel_materials = vm
+    with pytest.raises(ValueError, match="Overlapping"):
+        v.create_material_to_label_lut()
+    # a bound of 0 counts as no bound: the gap [0, 10] is not filled and B starts at 0
+    assert rows("-100 0 A\n10 20 B\n", def_mat="D") == [[-100, 0, "A"], [0, 20, "B"]]
+    with pytest.raises(ValueError, match="not disjoint"):
+        rows("0 10 A\n5 20 B\n-5 3 C\n")
+    with pytest.raises(ValueError, match="Wrong interval"):
+        rows("-10 -20 A\n")
+
+
+
+
+def test_schneider_rows_worked_out_by_hand():
+    """Rows of HounsfieldUnit_to_material (materials.py:149-230) derived by hand from the
+    committed tables. Lung, HU [-950, -120): both ends lie between the density nodes
+    (-1000, 0.00121) and (-98, 0.93), so d(h) = 0.00121 + (h + 1000) 0.92879 / 902, the
+    density span is d(-120) - d(-950) = 0.92879 * 830 / 902 = 0.854652 g/cm3, at a tolerance
+    of 0.1 n = 8.54652 -> 9 sub-intervals of 830 / 8.54652 = 97.1156 HU (the last one
+    clipped at -120), and sub-interval j has the midpoint density d(-950) + (j + 1/2) 0.1 =
+    0.052695 + 0.1 (j + 1/2). Marrow_Bone01, HU [120, 200): between the nodes (101, 1.0762)
+    and (1600, 1.9642) the span is 0.888 * 80 / 1499 = 0.047392 < 0.05: one material, at
+    the midpoint density 1.0762 + 0.888 * 59 / 1499 = 1.111151. The Lung weights (H 10.3,
+    C 10.5, N 3.1, O 74.9, Na 0.2, P 0.2, S 0.3, Cl 0.3, K 0.2) sum to 100."""
+    from types import SimpleNamespace
+
+    from opengate_amd.physics.materials import MaterialDatabase
+    from opengate_amd.units import g4_units
+
+    data = __import__("opengate_amd").__path__[0] + "/data/"
+    for tol in (0.1, 0.05):
+        db = MaterialDatabase()
+        sim = SimpleNamespace(volume_manager=SimpleNamespace(material_database=db))
+        vm, created = M.HounsfieldUnit_to_material(sim, tol * g4_units.g_cm3, data + "Schneider2000MaterialsTable.txt",
+                                                   data + "Schneider2000DensitiesTable.txt")
+        lung = [r for r in vm if r[2].startswith("Lung_")]
+        marrow = [r for r in vm if r[2].startswith("Marrow_Bone01_")]
+        assert len(marrow) == 1 and marrow[0][:2] == [120, 200]
+        assert db.get(marrow[0][2]).density == pytest.approx(1.111151, rel=2e-6)  # g/cm3
+        if tol == 0.1:
+            assert len(lung) == 9
+            step = 830 / (0.92879 * 830 / 902 / 0.1)
+            assert step == pytest.approx(97.1156, abs=1e-4)
+            for j, (lo, hi, name) in enumerate(lung):
+                assert lo == pytest.approx(-950 + j * step, abs=1e-9)
+                assert hi == pytest.approx(min(-950 + (j + 1) * step, -120), abs=1e-9)
+                if j < 8:
+                    assert db.get(name).density == pytest.approx(0.052695 + 0.1 * (j + 0.5),
+                                                                rel=1e-5)
+            m = db.get(lung[0][2])
+            frac = dict(zip(m.elements, m.fractions))
+            assert frac[8] == pytest.approx(0.749) and frac[1] == pytest.approx(0.103) and frac[19] == pytest.approx(0.002)
+            assert len(frac) == 9
+        else:
+            # 0.854652 / 0.05 = 17.09 -> 18 sub-intervals
+            assert len(lung) == 18
+
+
+def test_density_image_from_intervals():
+    """create_density_img (materials.py:294-326): density of the material whose [hu0, hu1)
+    interval holds the voxel, 0 in gaps and outside; geometry copied from the image."""
+    import opengate_amd as gate
+
+    sim = gate.Simulation()
+    ct = sim.add_volume("Image", "ct")
+    hu = np.array([[[-1001, -1000, -501], [-500, -400, 99]], [[100, 1999, 2000], [3000, -300, 0]]], dtype=np.int16)
+    ct.image = gate.Image(hu, (1.0, 2.0, 3.0), origin=(1.0, 2.0, 3.0))
+    ct.voxel_materials = [[-1000, -500, "G4_AIR"], [-400, 100, "G4_WATER"], [100, 2000, "G4_BONE_CORTICAL_ICRP"]]
+    rho = ct.create_density_image()
+    db = sim.volume_manager.material_database
+    a, w, b = (db.get(n).density for n in ("G4_AIR", "G4_WATER", "G4_BONE_CORTICAL_ICRP"))
+    exp = np.array([[[0, a, a], [0, w, w]], [[b, b, 0], [0, w, w]]], dtype=np.float32)
+    np.testing.assert_array_equal(rho.array, exp)
+    np.testing.assert_array_equal(rho.spacing, (1.0, 2.0, 3.0))

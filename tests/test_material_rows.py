@@ -59,8 +59,8 @@ def test_power_of_two_densities_give_identical_mu_and_runs():
         np.testing.assert_array_equal(Ts.material_mu(m), Tp.mu_total[m])
     np.testing.assert_array_equal(Ts.mu_majorant, Tp.mu_majorant)
     ph = PhantomSpec(layered_phantom(len(mats)), (2.5, 2.5, 2.5))
-    src = SourceSpec(pos_trans=(25.0, 22.5, 20.0), energy=0.1405)
-    tal = TallySpec(size=(20, 18, 16), spacing=(2.5, 2.5, 2.5), dose=True)
+    src = SourceSpec(pos_trans=(20.0, 22.5, 25.0), energy=0.1405)
+    tal = TallySpec(size=(16, 18, 20), spacing=(2.5, 2.5, 2.5), dose=True)
     rs, cs = O.run(Ts, ph, src, tal, 20000, 5, threads=4)
     rp, cp = O.run(Tp, ph, src, tal, 20000, 5, threads=4)
     for k in ("histories", "steps", "deposits", "compton", "rayleigh", "photo", "escaped", "kn_rejections"):
@@ -82,8 +82,8 @@ def test_uint16_labels_run_like_uint8_labels():
     p8 = PhantomSpec(lab.astype(np.uint8), (2.5, 2.5, 2.5))
     p16 = PhantomSpec(lab.astype(np.uint16), (2.5, 2.5, 2.5))
     assert p8.desc().label_bytes == 1 and p16.desc().label_bytes == 2
-    src = SourceSpec(pos_trans=(25.0, 22.5, 20.0), energy=0.1405)
-    tal = TallySpec(size=(20, 18, 16), spacing=(2.5, 2.5, 2.5))
+    src = SourceSpec(pos_trans=(20.0, 22.5, 25.0), energy=0.1405)
+    tal = TallySpec(size=(16, 18, 20), spacing=(2.5, 2.5, 2.5))
     r8, c8 = O.run(T, p8, src, tal, 5000, 9)
     r16, c16 = O.run(T, p16, src, tal, 5000, 9)
     assert c8 == c16
