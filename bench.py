@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--block", type=int, default=0,
                     help="ct256: workgroup size (0: automatic staging, all tables in LDS for one simulation)")
     ap.add_argument("--workload", default="headline",
-                    choices=("headline", "ct256", "batch64", "exitplane", "spect16", "mlp"))
+                    choices=("headline", "ct256", "ct256_schneider", "batch64", "exitplane", "spect16", "mlp"))
     ap.add_argument("--config1-runs", type=int, default=20, help="headline: runs of 1e6 histories on config 1")
     ap.add_argument("--arf-train-photons", type=float, default=4e7,
                     help="spect16: photons through the head model that label the ARF training set")
@@ -140,7 +140,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "histories shard across GPUs by RNG key range, see DESIGN.md", file=sys.stderr)
+              "histories would shard across GPUs by ranges of history ids (the RNG counter)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "headline":
@@ -155,6 +155,14 @@ def main():
         return bench_spect(args)
     if args.workload == "mlp":
         return bench_mlp(args)
+    if args.workload == "ct256_schneider":
+        leg = leg_ct256_schneider(args, args.steps)
+        print(json.dumps({"metric": METRIC, "value": leg["photons_per_s"], "unit": "photons/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": 1,
+                          "ms_per_step": 1e3 * leg["histories_per_step"] / leg["photons_per_s"],
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+                          "data": "synthetic (seed-generated phantom and spectrum)", "config": leg}), flush=True)
+        return
     leg = leg_ct256(args, args.steps, cpu=not args.no_cpu_baseline)
     line = {
         "metric": METRIC, "value": leg["photons_per_s"], "unit": "photons/s", "n_gpus": 1, "steps": args.steps,

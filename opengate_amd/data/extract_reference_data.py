@@ -12,6 +12,7 @@ Sources (reference tree):
   opengate/data/Schneider2000MaterialsTable.txt             HU -> composition
   opengate/data/Schneider2000DensitiesTable.txt             HU -> density
   opengate/data/rad_gamma_spectrum.json                     gamma lines
+  opengate/contrib/GateMaterials.db                         GATE material database
 
 Usage: python opengate_amd/data/extract_reference_data.py [/root/reference]
 """
@@ -67,6 +68,7 @@ def extract_elements(ref):
 def copy_tables(ref):
     for name in ("Schneider2000MaterialsTable.txt", "Schneider2000DensitiesTable.txt"):
         shutil.copyfile(ref / "opengate/data" / name, HERE / name)
+    shutil.copyfile(ref / "opengate/contrib/GateMaterials.db", HERE / "GateMaterials.db")
     d = json.loads((ref / "opengate/data/rad_gamma_spectrum.json").read_text())
     with open(HERE / "rad_gamma_lines.json", "w") as f:
         json.dump(d, f, indent=1)

@@ -275,8 +275,11 @@ class VolumeManager:
         return v
 
     def add_material_database(self, filename):
-        raise NotImplementedError("Gate .db material files are not read; use "
-                                  "material_database.add_material_weights")
+        """Reads a GATE material database file (managers.py VolumeManager.add_material_database,
+        materials.py:778-835)."""
+        if str(filename) in self.material_database.filenames:
+            raise ValueError(f"material database {filename} was already read")
+        self.material_database.read_from_file(filename)
 
     def global_transform(self, name):
         """(R, t) of a volume in the world frame, composing the mother chain."""

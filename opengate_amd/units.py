@@ -23,6 +23,10 @@ def _make():
     kg = joule * 1e18 / 1e6  # joule * s^2 / m^2 with s = 1e9, m = 1e3
     u.update(kg=kg, kilogram=kg, g=kg * 1e-3, gram=kg * 1e-3, mg=kg * 1e-6)
     u.update(g_cm3=u["g"] / u["cm3"], mg_cm3=u["mg"] / u["cm3"], kg_m3=kg / u["m3"])
+    u.update(mole=1.0, g_mole=u["g"])
+    # unit strings of material database files
+    u.update({"g/cm3": u["g_cm3"], "mg/cm3": u["mg_cm3"], "kg/m3": u["kg_m3"], "g/mole": u["g"], "mg/mole": u["mg"],
+              "kg/mole": kg})
     u.update(Gy=joule / kg, gray=joule / kg, mGy=1e-3 * joule / kg, uGy=1e-6 * joule / kg)
     u.update(Bq=1.0 / u["s"], becquerel=1.0 / u["s"], kBq=1e3 / u["s"], MBq=1e6 / u["s"], GBq=1e9 / u["s"])
     u.update(mCi=3.7e7 / u["s"], Ci=3.7e10 / u["s"], uCi=3.7e4 / u["s"])
