@@ -82,7 +82,8 @@ typedef struct gh_physics_desc {
   const float *density;      /* [M] g/cm3 */
   const int32_t *composition;/* [M] composition of each material (n_compositions > 0), else NULL */
   const float *mu_total;     /* [T][NB+1] */
-  const float *mu_majorant;  /* [NB+1]   1/mm, >= the mu of every material at every node */
+  const float *mu_majorant;  /* [NB+1]   1/mm, >= the mu of every material that occurs in a phantom
+                                (checked by gh_add_simulation), at every node */
   const float *p_compton;    /* [T][NB+1] sigma_C / mu */
   const float *p_rayleigh;   /* [T][NB+1] (sigma_C + sigma_R) / mu */
   const float *mu_en;        /* [T][NB+1] (optional) */

@@ -513,6 +513,7 @@ transport_kernel(Launch L) {
   T.mu = P.mu; T.maj = P.maj; T.pcpr = P.pcpr; T.rx = P.rx; T.rcdf = P.rcdf; T.sinc = P.sinc;
   T.rguide = P.rguide;
   const MatRow *mat_tab = P.mat;
+  const bool shared_rows = P.shared_rows != 0;  // uniform over the launch
   if (STAGE >= 1) {
     // staged by the threads of the launch (a launch may use fewer than BLOCK)
     const int nt = (int)blockDim.x;
@@ -773,11 +774,20 @@ transport_kernel(Launch L) {
               st.steps++;
               const int vox = ix + hot.nx * (iy + hot.ny * iz);
               const int lab = hot.label(vox);
-              const MatRow mr = mat_tab[lab];
-              const float mu = mr.rho * lerp_node(mu_tab + mr.row * NB1, ec);
+              // per-material rows: row = label, factor 1 (no second dependent LDS read);
+              // shared composition rows: the material's (density, row) pair first
+              int row = lab;
+              float rho = 1.0f;
+              if (shared_rows) {
+                const MatRow mr = mat_tab[lab];
+                row = mr.row;
+                rho = mr.rho;
+              }
+              const float mu_row = lerp_node(mu_tab + row * NB1, ec);
+              const float mu = shared_rows ? rho * mu_row : mu_row;
               if (TLE && ph.E <= tle_emax) {
                 const SimDev &s0 = sim(0);
-                const float x = (ph.E * ph.wt) * ((mr.rho * lerp_node(P.muen + mr.row * NB1, ec)) * ec.inv_maj);
+                const float x = (ph.E * ph.wt) * ((rho * lerp_node(P.muen + row * NB1, ec)) * ec.inv_maj);
                 deposit(s0, sq, tally_voxel(s0, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
               }
               if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
@@ -858,7 +868,7 @@ transport_kernel(Launch L) {
       float x_dep = 0.0f;   // energy given to the medium at this interaction
       const bool in_world = cvox < 0;
       if (mode == M_COLLIDE && !kill) clab = in_world ? s.world_mat : clab;
-      const int crow = mat_tab[clab].row;  // table row of the interaction's material
+      const int crow = shared_rows ? mat_tab[clab].row : clab;  // table row of the interaction's material
       if (mode == M_COLLIDE && !kill) {
         const float r = u01(rng);
         float pc, pr;
@@ -1055,6 +1065,7 @@ struct gh_ctx {
   // physics
   bool have_phys = false;
   Phys P{};
+  std::vector<uint8_t> under_majorant;  // per material: mu <= majorant at every node
   std::vector<void *> phys_owned;
   size_t lds1_bytes = 0, lds2_bytes = 0;
   // simulations
@@ -1319,6 +1330,7 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
   const size_t R = ph->n_compositions > 0 ? (size_t)ph->n_compositions : M;  // table rows
   // material m: attenuation rho * row; rho = density (shared compositions) or 1
   std::vector<MatRow> mat(M);
+  std::vector<uint8_t> under;
   for (size_t m = 0; m < M; ++m) {
     if (!(ph->density[m] >= 0.0f) || !std::isfinite(ph->density[m]))
       FAIL(GH_EINVAL, "gh_set_physics: densities must be finite and >= 0");
@@ -1336,12 +1348,16 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
         FAIL(GH_EINVAL, "gh_set_physics: majorant must be positive and finite");
       for (size_t r = 0; r < R; ++r)
         if (!(ph->mu_total[r * NB1 + j] >= 0.0f)) FAIL(GH_EINVAL, "gh_set_physics: mu_total must be >= 0");
-      // every material's mu, as the kernel forms it (FP32 product), within the majorant
-      for (size_t m = 0; m < M; ++m) {
-        const float mu = mat[m].rho * ph->mu_total[(size_t)mat[m].row * NB1 + j];
-        if (!(mu <= ph->mu_majorant[j])) FAIL(GH_EINVAL, "gh_set_physics: a material's mu exceeds the majorant");
-      }
     }
+    // which materials lie within the majorant (mu as the kernel forms it, an FP32
+    // product, at every node): a phantom may only hold those (checked per simulation)
+    under.assign(M, 1);
+    for (size_t m = 0; m < M; ++m)
+      for (size_t j = 0; j < NB1; ++j)
+        if (!(mat[m].rho * ph->mu_total[(size_t)mat[m].row * NB1 + j] <= ph->mu_majorant[j])) {
+          under[m] = 0;
+          break;
+        }
     if (ph->ray_x[0] != 0.0f) FAIL(GH_EINVAL, "gh_set_physics: ray_x[0] must be 0");
     for (size_t i = 1; i < NX; ++i) {
       uint32_t b;
@@ -1402,12 +1418,14 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
   if ((rc = dev_upload(ctx, inv_rho.data(), M, &t, ctx->phys_owned))) return rc; P.inv_rho = t;
   P.n_mat = (int)M;
   P.n_rows = (int)R;
+  P.shared_rows = ph->n_compositions > 0 ? 1 : 0;
   P.nb = ph->n_buckets;
   P.nx = (int)NX;
   P.e_cut = ph->e_cut;
   ctx->P = P;
   ctx->lds1_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 1).total * sizeof(float);
   ctx->lds2_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 2).total * sizeof(float);
+  ctx->under_majorant.swap(under);
   ctx->have_phys = true;
   return GH_OK;
 }
@@ -1438,8 +1456,19 @@ int gh_add_simulation(gh_ctx *ctx, const gh_phantom_desc *ph, const gh_source_de
     return ph->label_bytes == 2 ? ((const uint16_t *)ph->labels)[i] : ((const uint8_t *)ph->labels)[i];
   };
   uint32_t max_label = 0;
-  for (int64_t i = 0; i < nvox; ++i) max_label = std::max(max_label, label_in(i));
+  std::vector<uint8_t> present((size_t)ctx->P.n_mat + 1, 0);
+  for (int64_t i = 0; i < nvox; ++i) {
+    const uint32_t l = label_in(i);
+    max_label = std::max(max_label, l);
+    present[std::min<size_t>(l, (size_t)ctx->P.n_mat)] = 1;
+  }
   if (max_label >= (uint32_t)ctx->P.n_mat) FAIL(GH_EINVAL, "gh_add_simulation: label out of range of the material table");
+  for (int m = 0; m < ctx->P.n_mat; ++m)
+    if (present[m] && !ctx->under_majorant[m]) {
+      ctx->err = "gh_add_simulation: material " + std::to_string(m) +
+                 " occurs in the phantom but its mu exceeds the majorant of gh_set_physics";
+      return GH_EINVAL;
+    }
   {
     const int rc_src = validate_source(ctx, src);
     if (rc_src) return rc_src;

@@ -291,6 +291,7 @@ struct Phys {
   const float *sinc;     // [R][NX]
   const float *inv_rho;  // [M]
   int n_mat, n_rows, nb, nx;
+  int shared_rows;       // 1: materials share composition rows (mat[m] = {density, row})
   float e_cut;
 };
 

@@ -300,7 +300,9 @@ class SimulationEngine:
 
             checked += [sim.volume_manager.material_database.get(n) for n in HEAD_MATERIALS]
         e_cut = max(sim.physics_manager.resolve_photon_cut(checked, e_max), 2.0 ** -10)
-        tables = xs.build_tables(mats, e_max * (1 + 1e-6), e_cut=e_cut)
+        # the Woodcock majorant covers the materials that occur in the phantom
+        present = np.flatnonzero(np.bincount(ph.labels.ravel(), minlength=len(mats)))
+        tables = xs.build_tables(mats, e_max * (1 + 1e-6), e_cut=e_cut, majorant_materials=present)
         self.spect = None
         n_total = sum(n for _, _, n in runs)
         if spect.spect_actors(sim):

@@ -335,6 +335,7 @@ class PhysicsTables:
     e_cut: float
     e_max: float
     composition: np.ndarray = None  # [M] int32, or None (a row per material)
+    majorant_materials: list = None  # materials the majorant covers (None: all)
 
     @property
     def n_materials(self):
@@ -360,14 +361,18 @@ def _composition_key(mat):
     return tuple((int(z), float(w)) for z, w in zip(mat.elements, mat.fractions))
 
 
-def build_tables(materials, e_max, e_cut=None, majorant_margin=2.0**-16, share_compositions=True):
+def build_tables(materials, e_max, e_cut=None, majorant_margin=2.0**-16, share_compositions=True,
+                 majorant_materials=None):
     """Build the FP32 transport tables for `materials` (list of Material) up to e_max MeV.
 
     All arithmetic is FP64; the arrays are rounded to FP32 once at the end. When several
     materials share an elemental composition (Schneider HU materials at different
     densities) and share_compositions is set, the tables hold one row per composition for
     unit density; otherwise one row per material. The majorant node values are the
-    per-node maximum of every material's mu (as the transport forms it in FP32), raised by
+    per-node maximum of the mu (as the transport forms it in FP32) of every material, or of
+    the materials with indices `majorant_materials` (those that occur in the phantoms: a
+    CT table may hold dense materials, dental amalgam for one, that no voxel uses, and
+    a majorant raised by them multiplies the Woodcock steps), raised by
     `majorant_margin` (relative) so that FP32 interpolation never exceeds it.
     """
     nb = n_buckets_for(e_max)
@@ -404,7 +409,12 @@ def build_tables(materials, e_max, e_cut=None, majorant_margin=2.0**-16, share_c
         per_mat = np.stack([density[m] * mu32[comp[m]] for m in range(len(materials))])  # FP32 products
     else:
         per_mat = mu32
-    maj = per_mat.astype(np.float64).max(axis=0) * (1.0 + majorant_margin)
+    sel = np.arange(len(materials)) if majorant_materials is None else np.asarray(sorted(set(majorant_materials)),
+                                                                                     dtype=np.int64)
+    if sel.size == 0:
+        raise ValueError("majorant_materials is empty")
+    maj = per_mat[sel].astype(np.float64).max(axis=0) * (1.0 + majorant_margin)
+    maj = np.maximum(maj, np.finfo(np.float32).tiny)  # vacuum-only phantoms: a positive majorant
     xn = rayleigh_x_nodes(float(e_nodes[-1]))
     cdf = np.zeros((r, len(xn)))
     sfun = np.zeros((r, len(xn)))
@@ -436,5 +446,6 @@ def build_tables(materials, e_max, e_cut=None, majorant_margin=2.0**-16, share_c
         incoh_s=sfun.astype(f32),
         e_cut=float(e_cut),
         e_max=float(e_nodes[-1]),
+        majorant_materials=None if majorant_materials is None else [int(i) for i in sel],
         composition=np.array(comp, dtype=np.int32) if shared else None,
     )

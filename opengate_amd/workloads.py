@@ -50,11 +50,13 @@ CT256S_DESC = ("ct256-schneider: the ct256 HU phantom mapped by HounsfieldUnit_t
                "120 kVp cone from 600 mm; dose on the 256x256x128 grid")
 
 
-def ct256_schneider(engine, tol=0.05, seed=2, share_compositions=True):
+def ct256_schneider(engine, tol=0.05, seed=2, share_compositions=True, majorant_over_phantom=True):
     """ct256 with the reference's own HU conversion: the Schneider materials and densities
     tables split into sub-intervals of density width <= tol g/cm3
     (HounsfieldUnit_to_material, geometry/materials.py:149-230); the label image follows the
-    ImageVolume label rule (default material G4_AIR = label 0). Returns (tables, phantom,
+    ImageVolume label rule (default material G4_AIR = label 0). The Woodcock majorant covers
+    the materials of this phantom (or every material of the table when
+    majorant_over_phantom is False, for other phantoms). Returns (tables, phantom,
     source, tally, info); labels are computed on the GPU and kept as uint16 when needed."""
     from types import SimpleNamespace
     from pathlib import Path as _P
@@ -75,7 +77,9 @@ def ct256_schneider(engine, tol=0.05, seed=2, share_compositions=True):
     labels_sorted = np.array([0] + [lut[r[2]] for r in rows] + [0], dtype=np.uint16)
     lab = engine.hu_to_labels(hu, bins, labels_sorted)
     mats = [db.get(n) for n in names]
-    tables = xs.build_tables(mats, 0.1201, share_compositions=share_compositions)
+    present = np.flatnonzero(np.bincount(np.asarray(lab).ravel(), minlength=len(names)))
+    tables = xs.build_tables(mats, 0.1201, share_compositions=share_compositions,
+                             majorant_materials=present if majorant_over_phantom else None)
     L = np.array([256.0, 256.0, 128.0])
     half_w = 1500.0
     ph = PhantomSpec(lab if len(names) > 256 else lab.astype(np.uint8), (1.0, 1.0, 1.0), 0,

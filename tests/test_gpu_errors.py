@@ -37,6 +37,19 @@ def test_rejections_leave_the_context_usable(engine):
                          energy_cdf=np.array([0.6, 0.5], np.float32))
     with pytest.raises(GhError):
         engine.add_simulation(ph, bad_cdf, tal)
+    # a majorant over water only: a phantom holding bone is refused, water alone runs
+    from helpers import DB
+    from opengate_amd.physics import xs
+
+    Tw = xs.build_tables([DB.get("G4_WATER"), DB.get("G4_BONE_CORTICAL_ICRP")], 0.1405, majorant_materials=[0])
+    engine.set_physics(Tw)
+    with pytest.raises(GhError, match="material 1 occurs in the phantom but its mu exceeds the majorant"):
+        engine.add_simulation(ph, src, tal)
+    water = PhantomSpec(np.zeros((8, 8, 8), np.uint8), (2.0, 2.0, 2.0))
+    engine.add_simulation(water, src, aligned_tally(water))
+    assert engine.run([1000], 1)["histories"] == 1000
+    engine.clear()
+    engine.set_physics(T)
     # TLE tallies need single-simulation launches
     engine.add_simulation(ph, src, aligned_tally(ph, tle=True))
     engine.add_simulation(ph, src, tal)

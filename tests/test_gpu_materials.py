@@ -44,7 +44,7 @@ def schneider_ramp(tol, share):
         def hu_to_labels(self, hu, bins, labels_sorted):
             return labels_sorted[np.digitize(hu, bins)]
 
-    T, _, _, _, info = workloads.ct256_schneider(_Host(), tol, share_compositions=share)
+    T, _, _, _, info = workloads.ct256_schneider(_Host(), tol, share_compositions=share, majorant_over_phantom=False)
     n = T.n_materials
     g = np.random.default_rng(11)
     lab = np.repeat(g.integers(1, n, size=(36, 1, 1)), 40, axis=1).repeat(48, axis=2)

@@ -106,9 +106,16 @@ def test_schneider_materials_share_rows_and_fit_the_majorant():
     T, ph, src, tal, info = workloads.ct256_schneider(_HostLabels(), 0.01)
     assert info["n_materials"] == 287 and T.n_compositions < 40
     mu = np.stack([T.material_mu(m) for m in range(T.n_materials)]).astype(np.float64)
-    assert (mu <= T.mu_majorant[None, :]).all()
+    # the majorant covers the materials that occur in the phantom, and only those: the
+    # table's dental amalgam (HU >= 2300) is not in it and lies far above
+    present = np.flatnonzero(np.bincount(ph.labels.ravel(), minlength=T.n_materials))
+    assert T.majorant_materials == list(present)
+    assert (mu[present] <= T.mu_majorant[None, :]).all()
+    amalgam = [m for m, n in enumerate(info["names"]) if n.startswith("AmalgamTooth")]
+    assert amalgam and not set(amalgam) & set(present)
+    assert (mu[amalgam].max(0) > 10 * T.mu_majorant).any()
     # the majorant is the per-node maximum raised by 2^-16 (no looser)
-    assert np.all(T.mu_majorant <= mu.max(0) * (1 + 2.0 ** -15))
+    assert np.all(T.mu_majorant <= mu[present].max(0) * (1 + 2.0 ** -15))
     # the sub-materials of one Schneider row share its composition row
     names = info["names"]
     rows = {}

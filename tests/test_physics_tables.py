@@ -88,3 +88,20 @@ def test_fp32_tables():
     assert T.ray_x[0] == 0 and np.all(np.diff(T.ray_x) > 0)
     kk = e_nodes[-1] * xs.K_PER_MEV
     assert T.ray_x[-1] >= 4 * kk * kk
+
+
+def test_majorant_over_the_materials_that_occur():
+    """build_tables(majorant_materials=...): the majorant is the per-node maximum over the
+    named materials only (x (1 + 2^-16)); all of them lie under it and a denser material
+    left out does not."""
+    mats = [DB.get("G4_WATER"), DB.get("G4_BONE_CORTICAL_ICRP"), DB.get("G4_Pb")]
+    full = xs.build_tables(mats, 0.15)
+    sub = xs.build_tables(mats, 0.15, majorant_materials=[0, 1])
+    np.testing.assert_array_equal(full.mu_total, sub.mu_total)
+    exp = np.maximum(sub.mu_total[0], sub.mu_total[1]).astype(np.float64) * (1 + 2.0**-16)
+    np.testing.assert_array_equal(sub.mu_majorant, exp.astype(np.float32))
+    assert np.all(sub.mu_total[:2] <= sub.mu_majorant)
+    assert np.all(sub.mu_total[2] > sub.mu_majorant)
+    assert np.all(full.mu_total[2] <= full.mu_majorant) and sub.majorant_materials == [0, 1]
+    with pytest.raises(ValueError):
+        xs.build_tables(mats, 0.15, majorant_materials=[])
