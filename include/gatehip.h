@@ -273,6 +273,25 @@ int gh_mlp_forward_bf16(const void *X, int64_t ldx, int64_t M, int32_t n_layers,
                         int32_t n_out, const void *const *wt, const float *const *bias, float *Z, int64_t ldz,
                         void *stream);
 
+/* Training: one Adam step (Kingma & Ba) over all parameter tensors of a network in one
+ * launch. No counterpart in the reference's source: its networks come trained from the
+ * external gaga / garf packages (loaded in sources/gansources.py and actors/arfactors.py);
+ * here the learned exit-plane source and detector response are trained in place. Each tensor is rows x cols FP32,
+ * row-major, with its gradient and both moments; entries outside the valid_rows x valid_cols
+ * corner are set to zero (GEMM padding); param_bf16, if not NULL, receives bf16(param).
+ * decay: L2 weight decay added to the gradient. step = 1, 2, ... (bias correction). */
+typedef struct gh_adam_tensor {
+  float *param;
+  const float *grad;
+  float *m, *v;
+  void *param_bf16;
+  int32_t rows, cols, valid_rows, valid_cols;
+  float decay;
+  int32_t reserved;
+} gh_adam_tensor;
+int gh_adam_step(const gh_adam_tensor *tensors, int32_t n_tensors, float lr, float beta1, float beta2, float eps,
+                 int32_t step, void *stream);
+
 /* Learned exit-plane source (replaces the generator call of GANSourceDefaultGenerator.generator,
  * sources/gansources.py:644-684, and its copy/move of the keys, :686-757).
  * gh_gaussian_bf16: out[r*ld + c] = N(0,1) noise in bf16 for rows first_row .. first_row+n_rows-1

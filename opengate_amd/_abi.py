@@ -86,6 +86,12 @@ class Launch(C.Structure):
                 ("slow_min", C.c_int32), ("refill_min", C.c_int32), ("scatter_tries", C.c_int32)]
 
 
+class AdamTensor(C.Structure):
+    _fields_ = [("param", C.c_void_p), ("grad", C.c_void_p), ("m", C.c_void_p), ("v", C.c_void_p),
+                ("param_bf16", C.c_void_p), ("rows", C.c_int32), ("cols", C.c_int32), ("valid_rows", C.c_int32),
+                ("valid_cols", C.c_int32), ("decay", C.c_float), ("reserved", C.c_int32)]
+
+
 class Counters(C.Structure):
     _fields_ = [
         ("histories", C.c_uint64), ("steps", C.c_uint64), ("deposits", C.c_uint64), ("compton", C.c_uint64),
@@ -157,6 +163,8 @@ SIGNATURES = {
     "gh_alias_table": (C.c_int, [DP, C.c_int64, FP, C.POINTER(C.c_int32)]),
     "gh_mlp_forward_bf16": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_void_p), C.POINTER(FP), FP, C.c_int64, C.c_void_p]),
+    "gh_adam_step": (C.c_int, [C.POINTER(AdamTensor), C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
+                               C.c_int32, C.c_void_p]),
     "gh_spect_pairs": (C.c_int, [FP * 8, C.c_int64, C.POINTER(SpectDesc), C.c_void_p, C.c_int64, C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.POINTER(FP), C.c_void_p, C.c_void_p, C.c_void_p]),
     "gh_energy_windows": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(Blur), FP,

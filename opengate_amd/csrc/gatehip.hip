@@ -460,10 +460,12 @@ __device__ __forceinline__ float u01w(uint32_t w) { return (float)((w >> 8) | 1u
 struct LdsLayout {
   int mu, maj, mat, sinc, rcdf, rx, total;
 };
-__host__ __device__ inline LdsLayout lds_layout(int n_rows, int n_mat, int nb1, int nx, int stage) {
+// expand: the staged mu table has one row per material (node products density x row) instead
+// of one per table row, so that a Woodcock step reads it without the (density, row) lookup
+__host__ __device__ inline LdsLayout lds_layout(int n_rows, int n_mat, int nb1, int nx, int stage, bool expand) {
   LdsLayout o;
   o.mu = 0;
-  o.maj = n_rows * nb1;
+  o.maj = (expand ? n_mat : n_rows) * nb1;
   o.mat = (o.maj + nb1 + 1) & ~1;  // MatRow pairs, 8-byte aligned
   const int end = o.mat + 2 * n_mat;
   o.sinc = end;
@@ -514,12 +516,25 @@ transport_kernel(Launch L) {
   T.rguide = P.rguide;
   const MatRow *mat_tab = P.mat;
   const bool shared_rows = P.shared_rows != 0;  // uniform over the launch
+  // mu by material index (rows per material, or shared rows expanded into LDS): no
+  // (density, row) lookup in front of the table read
+  const bool mu_by_mat = !shared_rows || (STAGE >= 1 && P.mu_expanded != 0);
   if (STAGE >= 1) {
     // staged by the threads of the launch (a launch may use fewer than BLOCK)
     const int nt = (int)blockDim.x;
-    const LdsLayout o = lds_layout(P.n_rows, P.n_mat, NB1, P.nx, STAGE);
-    const int n_mu = P.n_rows * NB1;
-    for (int i = threadIdx.x; i < n_mu; i += nt) lds[o.mu + i] = P.mu[i];
+    const bool expand = shared_rows && P.mu_expanded != 0;
+    const LdsLayout o = lds_layout(P.n_rows, P.n_mat, NB1, P.nx, STAGE, expand);
+    if (expand) {
+      const int n_mu = P.n_mat * NB1;
+      for (int i = threadIdx.x; i < n_mu; i += nt) {
+        const int m = i / NB1, j = i - m * NB1;
+        const MatRow mr = P.mat[m];
+        lds[o.mu + i] = mr.rho * P.mu[mr.row * NB1 + j];
+      }
+    } else {
+      const int n_mu = P.n_rows * NB1;
+      for (int i = threadIdx.x; i < n_mu; i += nt) lds[o.mu + i] = P.mu[i];
+    }
     for (int i = threadIdx.x; i < NB1; i += nt) lds[o.maj + i] = P.maj[i];
     for (int i = threadIdx.x; i < P.n_mat; i += nt) ((MatRow *)(lds + o.mat))[i] = P.mat[i];
     T.mu = lds + o.mu;
@@ -608,8 +623,13 @@ transport_kernel(Launch L) {
       st.add_e(E_ESC, (double)(p.E * p.wt));
       return M_DEAD;
     }
-    const MatRow mw = mat_tab[s.world_mat];
-    const float muw = mw.rho * lerp_node(mu_tab + mw.row * NB1, e);
+    float muw;
+    if (mu_by_mat) {
+      muw = lerp_node(mu_tab + s.world_mat * NB1, e);
+    } else {
+      const MatRow mw = mat_tab[s.world_mat];
+      muw = lerp_node_scaled(mu_tab + mw.row * NB1, mw.rho, e);
+    }
     const float tw = world_exit(s, p.x, p.y, p.z, p.u, p.v, p.w, iv());
     const float d = (muw > 0.0f) ? (-glog(u01(g)) / muw) : INFINITY;
     if (hit && d >= tin && tin < tw) {
@@ -774,20 +794,19 @@ transport_kernel(Launch L) {
               st.steps++;
               const int vox = ix + hot.nx * (iy + hot.ny * iz);
               const int lab = hot.label(vox);
-              // per-material rows: row = label, factor 1 (no second dependent LDS read);
-              // shared composition rows: the material's (density, row) pair first
-              int row = lab;
-              float rho = 1.0f;
-              if (shared_rows) {
+              // table rows per material (or expanded in LDS): one read; shared composition
+              // rows: the material's (density, row) pair first
+              float mu;
+              if (mu_by_mat) {
+                mu = lerp_node(mu_tab + lab * NB1, ec);
+              } else {
                 const MatRow mr = mat_tab[lab];
-                row = mr.row;
-                rho = mr.rho;
+                mu = lerp_node_scaled(mu_tab + mr.row * NB1, mr.rho, ec);
               }
-              const float mu_row = lerp_node(mu_tab + row * NB1, ec);
-              const float mu = shared_rows ? rho * mu_row : mu_row;
               if (TLE && ph.E <= tle_emax) {
                 const SimDev &s0 = sim(0);
-                const float x = (ph.E * ph.wt) * ((rho * lerp_node(P.muen + row * NB1, ec)) * ec.inv_maj);
+                const MatRow mr = shared_rows ? mat_tab[lab] : MatRow{1.0f, lab};
+                const float x = (ph.E * ph.wt) * (lerp_node_scaled(P.muen + mr.row * NB1, mr.rho, ec) * ec.inv_maj);
                 deposit(s0, sq, tally_voxel(s0, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
               }
               if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
@@ -1067,7 +1086,8 @@ struct gh_ctx {
   Phys P{};
   std::vector<uint8_t> under_majorant;  // per material: mu <= majorant at every node
   std::vector<void *> phys_owned;
-  size_t lds1_bytes = 0, lds2_bytes = 0;
+  size_t lds1_bytes = 0, lds2_bytes = 0;    // staged tables (mu by table row)
+  size_t lds1e_bytes = 0, lds2e_bytes = 0;  // the same with mu expanded per material
   // simulations
   std::vector<SimHost> sims;
   // run scratch
@@ -1423,8 +1443,10 @@ int gh_set_physics(gh_ctx *ctx, const gh_physics_desc *ph) {
   P.nx = (int)NX;
   P.e_cut = ph->e_cut;
   ctx->P = P;
-  ctx->lds1_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 1).total * sizeof(float);
-  ctx->lds2_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 2).total * sizeof(float);
+  ctx->lds1_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 1, false).total * sizeof(float);
+  ctx->lds2_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 2, false).total * sizeof(float);
+  ctx->lds1e_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 1, true).total * sizeof(float);
+  ctx->lds2e_bytes = (size_t)lds_layout((int)R, (int)M, (int)NB1, (int)NX, 2, true).total * sizeof(float);
   ctx->under_majorant.swap(under);
   ctx->have_phys = true;
   return GH_OK;
@@ -1710,7 +1732,16 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   if (big) lp.block = 1024;
   if (lp.block % 64 || lp.block > 1024 || (!big && lp.block > 256))
     FAIL(GH_EINVAL, "gh_run: block must be a multiple of 64, <= 256, or 1024");
-  const size_t shmem = stage == 2 ? ctx->lds2_bytes : (stage == 1 ? ctx->lds1_bytes : 0);
+  size_t shmem = stage == 2 ? ctx->lds2_bytes : (stage == 1 ? ctx->lds1_bytes : 0);
+  // shared composition rows: stage mu per material (node products) when that still fits
+  L.P.mu_expanded = 0;
+  if (ctx->P.shared_rows && stage >= 1) {
+    const size_t e = stage == 2 ? ctx->lds2e_bytes : ctx->lds1e_bytes;
+    if (e <= (big ? budget_big : budget_small)) {
+      L.P.mu_expanded = 1;
+      shmem = e;
+    }
+  }
   const bool batched = ns > 1;
   bool tle = false;
   for (int s = 0; s < ns; ++s) tle = tle || ctx->sims[s].dev.tle;

@@ -254,6 +254,14 @@ GH_DEV float lerp_node(const float *row, const ECache &c) {
   return fmaf(c.frac, v1 - v0, v0);
 }
 
+// value of material (rho, row) at E: interpolation between its node values, each the FP32
+// product rho * row[node] (with rho = 1, the row itself); a table of those products gives
+// the same values through lerp_node
+GH_DEV float lerp_node_scaled(const float *row, float rho, const ECache &c) {
+  const float v0 = rho * row[c.b], v1 = rho * row[c.b + 1];
+  return fmaf(c.frac, v1 - v0, v0);
+}
+
 // value of a table on the q^2 grid at x >= 0: the interval comes from the bits of x
 // (no search), the position inside it from the low mantissa bits; clamped at the top
 GH_DEV float qgrid_value(const float *ys, int n, float x) {
@@ -292,6 +300,7 @@ struct Phys {
   const float *inv_rho;  // [M]
   int n_mat, n_rows, nb, nx;
   int shared_rows;       // 1: materials share composition rows (mat[m] = {density, row})
+  int mu_expanded;       // launch: the staged mu table holds a row of node products per material
   float e_cut;
 };
 

@@ -26,6 +26,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <string>
@@ -627,4 +628,75 @@ extern "C" int gh_mlp_forward_bf16(const void *X, int64_t ldx, int64_t M, int32_
   a.x_vec = ((uintptr_t)X % 16 == 0 && ldx % 8 == 0) ? 1 : 0;
   if (hidden == 256) return launch_mlp<256>((const __bf16 *)X, ldx, M, a, Z, ldz, (hipStream_t)stream);
   return launch_mlp<128>((const __bf16 *)X, ldx, M, a, Z, ldz, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ optimiser
+// One Adam step over every parameter tensor of a network in one launch (the training loop
+// otherwise issues about ten elementwise kernels per tensor). Entry e of the flattened
+// launch belongs to tensor i with offset[i] <= e < offset[i+1]; inside the tensor's
+// valid_rows x valid_cols corner it takes the Adam update (with L2 weight decay added to the
+// gradient when decay != 0), outside it the parameter and both moments stay
+// zero (the padding of the bf16 GEMM operands). The bf16 copy, when given, is the parameter
+// rounded to nearest even, ready for the next forward pass.
+namespace {
+constexpr int ADAM_MAX = 32;
+struct AdamArgs {
+  gh_adam_tensor t[ADAM_MAX];
+  int64_t offset[ADAM_MAX + 1];
+  int n;
+  float lr, b1, b2, eps, c1, c2;  // c1 = 1 / (1 - b1^t), c2 = 1 / (1 - b2^t)
+};
+
+__global__ void __launch_bounds__(256) adam_kernel(AdamArgs a) {
+  const int64_t total = a.offset[a.n];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int i = 0;
+    while (e >= a.offset[i + 1]) ++i;  // at most ADAM_MAX tensors, offsets in kernel arguments
+    const gh_adam_tensor &t = a.t[i];
+    const int64_t j = e - a.offset[i];
+    const int64_t r = j / t.cols, c = j - r * t.cols;
+    float p = 0.0f, m = 0.0f, v = 0.0f;
+    if (r < t.valid_rows && c < t.valid_cols) {
+      p = t.param[j];
+      const float g = t.decay != 0.0f ? fmaf(t.decay, p, t.grad[j]) : t.grad[j];
+      m = fmaf(a.b1, t.m[j], (1.0f - a.b1) * g);
+      v = fmaf(a.b2, t.v[j], (1.0f - a.b2) * (g * g));
+      const float mh = m * a.c1, vh = v * a.c2;
+      p -= a.lr * mh / (sqrtf(vh) + a.eps);
+    }
+    t.param[j] = p;
+    t.m[j] = m;
+    t.v[j] = v;
+    if (t.param_bf16) ((__bf16 *)t.param_bf16)[j] = f2bf(p);
+  }
+}
+}  // namespace
+
+extern "C" int gh_adam_step(const gh_adam_tensor *tensors, int32_t n_tensors, float lr, float beta1, float beta2,
+                            float eps, int32_t step, void *stream) {
+  if (!tensors || n_tensors <= 0 || n_tensors > ADAM_MAX || step <= 0 || !(lr >= 0.0f) || !(beta1 >= 0.0f) ||
+      !(beta1 < 1.0f) || !(beta2 >= 0.0f) || !(beta2 < 1.0f) || !(eps > 0.0f))
+    return GH_EINVAL;
+  AdamArgs a{};
+  a.n = n_tensors;
+  a.offset[0] = 0;
+  for (int i = 0; i < n_tensors; ++i) {
+    const gh_adam_tensor &t = tensors[i];
+    if (!t.param || !t.grad || !t.m || !t.v || t.rows <= 0 || t.cols <= 0 || t.valid_rows < 0 ||
+        t.valid_cols < 0 || t.valid_rows > t.rows || t.valid_cols > t.cols)
+      return GH_EINVAL;
+    a.t[i] = t;
+    a.offset[i + 1] = a.offset[i] + (int64_t)t.rows * t.cols;
+  }
+  a.lr = lr;
+  a.b1 = beta1;
+  a.b2 = beta2;
+  a.eps = eps;
+  a.c1 = (float)(1.0 / (1.0 - std::pow((double)beta1, (double)step)));
+  a.c2 = (float)(1.0 / (1.0 - std::pow((double)beta2, (double)step)));
+  const int64_t total = a.offset[n_tensors];
+  const unsigned grid = (unsigned)std::min<int64_t>((total + 255) / 256, 2048);
+  hipLaunchKernelGGL(adam_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream, a);
+  return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
 }

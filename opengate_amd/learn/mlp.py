@@ -52,7 +52,12 @@ class MLP:
 
     # ------------------------------------------------------------------ helpers
     def _refresh(self):
-        self.W16 = [w.to(_torch().bfloat16).contiguous() for w in self.W]
+        """bf16 copies of the weights (kept in place: the optimiser kernel writes them)."""
+        torch = _torch()
+        if getattr(self, "W16", None) is None or [t.shape for t in self.W16] != [w.shape for w in self.W]:
+            self.W16 = [torch.empty(w.shape, device=self.dev, dtype=torch.bfloat16) for w in self.W]
+        for t, w in zip(self.W16, self.W):
+            t.copy_(w)
         self._fused = None
 
     def _stream(self):
@@ -190,26 +195,27 @@ class MLP:
         return dW, db
 
     def step(self, dW, db):
-        """Adam update of the FP32 master weights (padding kept at zero)."""
-        torch = _torch()
+        """Adam update of the FP32 master weights and biases in one kernel (gh_adam_step),
+        which also keeps the padding at zero and writes the bf16 weight copies."""
         self.t += 1
-        b1, b2 = self.betas
         params = self.W + self.b
-        grads = dW + db
-        for i, (p, g) in enumerate(zip(params, grads)):
-            if self.wd and i < len(self.W):
-                g = g + self.wd * p
-            self.m[i].mul_(b1).add_(g, alpha=1 - b1)
-            self.v[i].mul_(b2).addcmul_(g, g, value=1 - b2)
-            mh = self.m[i] / (1 - b1**self.t)
-            vh = self.v[i] / (1 - b2**self.t)
-            p.sub_(self.lr * mh / (vh.sqrt() + self.eps))
-        for w, k, n in zip(self.W, self.sizes[:-1], self.sizes[1:]):
-            w[k:, :] = 0
-            w[:, n:] = 0
-        for bb, n in zip(self.b, self.sizes[1:]):
-            bb[n:] = 0
-        self._refresh()
+        grads = list(dW) + list(db)
+        valid = [(k, n) for k, n in zip(self.sizes[:-1], self.sizes[1:])] + [(1, n) for n in self.sizes[1:]]
+        tab = (A.AdamTensor * len(params))()
+        for i, (p, g, (vr, vc)) in enumerate(zip(params, grads, valid)):
+            if g.shape != p.shape or not g.is_contiguous() or g.dtype != p.dtype:
+                raise ValueError("gradient shape / layout differs from its parameter")
+            rows, cols = (p.shape[0], p.shape[1]) if p.dim() == 2 else (1, p.shape[0])
+            is_w = i < len(self.W)
+            tab[i] = A.AdamTensor(p.data_ptr(), g.data_ptr(), self.m[i].data_ptr(), self.v[i].data_ptr(),
+                                  self.W16[i].data_ptr() if is_w else None, rows, cols, vr, vc,
+                                  float(self.wd) if is_w else 0.0, 0)
+        b1, b2 = self.betas
+        rc = self.lib.gh_adam_step(tab, len(params), float(self.lr), float(b1), float(b2), float(self.eps), self.t,
+                                   self._stream())
+        if rc != 0:
+            raise RuntimeError(f"gh_adam_step failed ({rc})")
+        self._fused = None
 
     def state_dict(self):
         return {"sizes": self.sizes, "W": [w.cpu() for w in self.W], "b": [b.cpu() for b in self.b]}

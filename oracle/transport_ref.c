@@ -259,9 +259,18 @@ static float node_interp(const float *row, const eidx_t *x) {
   return fmaf(x->frac, row[x->bucket + 1] - row[x->bucket], row[x->bucket]);
 }
 
-/* attenuation coefficient of material m at the energy of x: its row times its factor */
+/* value of material m at the energy of x from table `tab`: interpolation between the
+ * material's node values, each the FP32 product of its factor and its row's node (factor 1
+ * with one row per material: the row itself) */
+static float material_node_interp(const phys_t *P, const float *tab, int m, const eidx_t *x) {
+  const float *row = tab + (size_t)P->row[m] * P->nb1;
+  const float v0 = P->rho[m] * row[x->bucket], v1 = P->rho[m] * row[x->bucket + 1];
+  return fmaf(x->frac, v1 - v0, v0);
+}
+
+/* attenuation coefficient of material m at the energy of x */
 static float material_mu(const phys_t *P, int m, const eidx_t *x) {
-  return P->rho[m] * node_interp(P->d->mu_total + (size_t)P->row[m] * P->nb1, x);
+  return material_node_interp(P, P->d->mu_total, m, x);
 }
 
 /* Kahn's method for Klein-Nishina, then acceptance with S(q)/Z */
@@ -670,7 +679,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
         lab = label_at(S, vox);
         float mu = material_mu(P, lab, &ex);
         if (tle && ph.E <= tle_emax) {
-          const float x = (ph.E * ph.w) * ((P->rho[lab] * node_interp(d->mu_en + (size_t)P->row[lab] * P->nb1, &ex)) * ex.inv_maj);
+          const float x = (ph.E * ph.w) * (material_node_interp(P, d->mu_en, lab, &ex) * ex.inv_maj);
           score(t, &sl, tally_index(S, ph.p), x, x * P->inv_rho[lab], st);
         }
         if ((float)((w[2 * k + 1] >> 8) | 1u) * 0x1p-24f * ex.maj < mu) { outcome = 1; break; }

@@ -104,6 +104,65 @@ def test_mlp_trains_a_regression():
     assert losses[-1] < 0.1 * losses[0]
 
 
+def test_fused_adam_step_against_fp64():
+    """gh_adam_step (one launch over all tensors) against Adam in FP64 (Kingma & Ba, with
+    bias correction and L2 decay on the weights) over three steps: valid corners within
+    FP32 rounding, padding exactly zero, the bf16 copy the round-to-nearest-even of the
+    parameter."""
+    import ctypes as C
+
+    torch = _torch()
+    from opengate_amd import _abi as A
+
+    lib = A.load_library()
+    print("[gpu case] fused adam", flush=True)
+    g = torch.Generator().manual_seed(3)
+    shapes = [((40, 24), (33, 20), 0.01), ((1, 24), (1, 20), 0.0), ((24, 8), (24, 5), 0.01)]
+    P, G, M_, V_, P16 = [], [], [], [], []
+    for (r, c), (vr, vc), _ in shapes:
+        p = torch.zeros(r, c, dtype=torch.float64)
+        p[:vr, :vc] = torch.randn(vr, vc, generator=g, dtype=torch.float64)
+        P.append(p)
+        M_.append(torch.zeros(r, c, dtype=torch.float64))
+        V_.append(torch.zeros(r, c, dtype=torch.float64))
+    dev = [t.float().cuda().contiguous() for t in P]
+    dm = [torch.zeros_like(t) for t in dev]
+    dv = [torch.zeros_like(t) for t in dev]
+    d16 = [torch.empty(t.shape, dtype=torch.bfloat16, device="cuda") for t in dev]
+    lr, b1, b2, eps = 1e-2, 0.9, 0.999, 1e-8
+    for step in range(1, 4):
+        grads = []
+        for ((r, c), (vr, vc), wd), p in zip(shapes, P):
+            gr = torch.randn(r, c, generator=g, dtype=torch.float64)
+            gr[vr:, :] = 7.0  # garbage in the padding must not leak
+            gr[:, vc:] = 7.0
+            grads.append(gr)
+        tab = (A.AdamTensor * 3)()
+        dg = [x.float().cuda().contiguous() for x in grads]
+        for i, ((r, c), (vr, vc), wd) in enumerate(shapes):
+            tab[i] = A.AdamTensor(dev[i].data_ptr(), dg[i].data_ptr(), dm[i].data_ptr(), dv[i].data_ptr(),
+                                  d16[i].data_ptr() if i != 1 else None, r, c, vr, vc, wd, 0)
+        assert lib.gh_adam_step(tab, 3, lr, b1, b2, eps, step, None) == 0
+        torch.cuda.synchronize()
+        for i, ((r, c), (vr, vc), wd) in enumerate(shapes):
+            gg = grads[i][:vr, :vc] + wd * P[i][:vr, :vc]
+            M_[i][:vr, :vc] = b1 * M_[i][:vr, :vc] + (1 - b1) * gg
+            V_[i][:vr, :vc] = b2 * V_[i][:vr, :vc] + (1 - b2) * gg * gg
+            mh = M_[i][:vr, :vc] / (1 - b1**step)
+            vh = V_[i][:vr, :vc] / (1 - b2**step)
+            P[i][:vr, :vc] -= lr * mh / (vh.sqrt() + eps)
+    for i, ((r, c), (vr, vc), wd) in enumerate(shapes):
+        got = dev[i].double().cpu()
+        np.testing.assert_allclose(got[:vr, :vc].numpy(), P[i][:vr, :vc].numpy(), rtol=2e-5, atol=2e-6)
+        assert torch.count_nonzero(got[vr:, :]) == 0 and torch.count_nonzero(got[:, vc:]) == 0
+        assert torch.count_nonzero(dm[i].cpu()[vr:, :]) == 0 and torch.count_nonzero(dv[i].cpu()[:, vc:]) == 0
+        if i != 1:
+            assert torch.equal(d16[i].cpu(), dev[i].cpu().to(torch.bfloat16))
+    bad = (A.AdamTensor * 1)(A.AdamTensor(None, None, None, None, None, 1, 1, 1, 1, 0.0, 0))
+    assert lib.gh_adam_step(bad, 1, lr, b1, b2, eps, 1, None) != 0
+    assert lib.gh_adam_step(tab, 3, lr, 1.0, b2, eps, 1, None) != 0
+
+
 @pytest.mark.parametrize("sizes,M", [([8, 256, 256, 256, 256, 256, 6], 1000), ([3, 128, 128, 128, 4], 4099),
                                      ([8, 256, 256, 6], 64), ([5, 128, 128, 16], 130)])
 def test_fused_forward_equals_layered(sizes, M):

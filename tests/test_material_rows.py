@@ -75,6 +75,42 @@ def test_power_of_two_densities_give_identical_mu_and_runs():
     assert rel.max() <= 2.0 ** -22
 
 
+def test_shared_rows_interpolate_node_products():
+    """With shared composition rows, material m's mu at E interpolates its node values
+    density x row (FP32 products): at a node it is that product, between nodes the FP32
+    interpolation of the two products; a per-material table of those products gives the
+    same runs (the kernel's LDS expansion of shared rows relies on it)."""
+    m3 = water_family() + [scaled("G4_WATER", 1.3, "W13")]
+    Ts = xs.build_tables(m3, 0.15)
+    nodes = xs.bucket_node_energies(Ts.n_buckets)
+    prod = Ts.material_mu(5)
+    for j in (0, 40, 200, Ts.n_buckets - 1):
+        assert O.mu_at(Ts, 5, nodes[j]) == prod[j]
+    j = 250
+    e_mid = np.float32((np.float64(nodes[j]) + np.float64(nodes[j + 1])) / 2)
+    v = O.mu_at(Ts, 5, e_mid)
+    assert min(prod[j], prod[j + 1]) <= v <= max(prod[j], prod[j + 1])
+    assert v == pytest.approx(0.5 * (float(prod[j]) + float(prod[j + 1])), rel=2e-7)
+    # per-material tables built from the products run identically
+    import copy
+
+    Te = copy.deepcopy(Ts)
+    Te.mu_total = np.stack([Ts.material_mu(m) for m in range(len(m3))]).astype(np.float32)
+    Te.composition = None
+    Te.density = Ts.density.copy()
+    rows = [Ts.row_of(m) for m in range(len(m3))]
+    for name in ("p_compton", "p_rayleigh", "mu_en", "ray_cdf", "incoh_s"):
+        setattr(Te, name, getattr(Ts, name)[rows].copy())
+    Te.mu_en = np.stack([np.float32(Ts.density[m]) * Ts.mu_en[rows[m]] for m in range(len(m3))]).astype(np.float32)
+    ph = PhantomSpec(layered_phantom(len(m3)), (2.5, 2.5, 2.5))
+    src = SourceSpec(pos_trans=(20.0, 22.5, 25.0), energy=0.1405)
+    tal = TallySpec(size=(16, 18, 20), spacing=(2.5, 2.5, 2.5), dose=True)
+    rs, cs = O.run(Ts, ph, src, tal, 20000, 5, threads=4)
+    re, ce = O.run(Te, ph, src, tal, 20000, 5, threads=4)
+    assert cs == ce
+    np.testing.assert_array_equal(rs["edep"], re["edep"])
+
+
 def test_uint16_labels_run_like_uint8_labels():
     mats = water_family()
     T = xs.build_tables(mats, 0.15)
