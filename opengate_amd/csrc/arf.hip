@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(256) energy_windows_kernel(const float *__rest
         const float u0 = (float)((x0 >> 8) | 1u) * 0x1p-24f, u1 = (float)((x1 >> 8) | 1u) * 0x1p-24f;
         float sn, csn;
         gsincos(6.28318530717958647692f * u1, sn, csn);
-        const float g = sqrtf(-2.0f * glog(u0)) * csn;
+        const float g = sqrtf(-2.0f * glog(u0, c_logtab)) * csn;
         eb = fmaf(blur_sigma(B, e), g, e);
       }
       for (int k = 0; k < W.n; ++k)

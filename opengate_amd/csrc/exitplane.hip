@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(256) gaussian_bf16_kernel(uint32_t k0, uint32_
       philox10(x0, x1, x2, x3, k0, k1);
       const float u0 = (float)((x0 >> 8) | 1u) * 0x1p-24f, u1 = (float)((x1 >> 8) | 1u) * 0x1p-24f;
       const float u2 = (float)((x2 >> 8) | 1u) * 0x1p-24f, u3 = (float)((x3 >> 8) | 1u) * 0x1p-24f;
-      const float ra = sqrtf(-2.0f * glog(u0)), rb = sqrtf(-2.0f * glog(u2));
+      const float ra = sqrtf(-2.0f * glog(u0, c_logtab)), rb = sqrtf(-2.0f * glog(u2, c_logtab));
       float sa, ca, sb, cb;
       gsincos(6.28318530717958647692f * u1, sa, ca);
       gsincos(6.28318530717958647692f * u3, sb, cb);

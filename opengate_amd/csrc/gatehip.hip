@@ -561,6 +561,9 @@ transport_kernel(Launch L) {
   __shared__ double s_en[N_ENERGIES * BLOCK];
   // birth queues: wave w owns columns [64 w, 64 w + 64)
   __shared__ uint32_t s_q[QWORDS * BLOCK];
+  // the logarithm's table (a per-lane index: LDS rather than constant memory)
+  __shared__ float2 s_logtab[64];
+  if (threadIdx.x < 64) s_logtab[threadIdx.x] = c_logtab[threadIdx.x];
   if (threadIdx.x < N_COUNTERS) s_cnt[threadIdx.x] = 0ull;
   for (int k = 0; k < N_ENERGIES; ++k) s_en[k * BLOCK + threadIdx.x] = 0.0;
   __syncthreads();
@@ -631,7 +634,7 @@ transport_kernel(Launch L) {
       muw = lerp_node_scaled(mu_tab + mw.row * NB1, mw.rho, e);
     }
     const float tw = world_exit(s, p.x, p.y, p.z, p.u, p.v, p.w, iv());
-    const float d = (muw > 0.0f) ? (-glog(u01(g)) / muw) : INFINITY;
+    const float d = (muw > 0.0f) ? (-glog(u01(g), s_logtab) / muw) : INFINITY;
     if (hit && d >= tin && tin < tw) {
       p.x = fmaf(tin, p.u, p.x); p.y = fmaf(tin, p.v, p.y); p.z = fmaf(tin, p.w, p.z);
       return M_STEP;
@@ -784,7 +787,7 @@ transport_kernel(Launch L) {
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
           if (mode == M_STEP) {
-            const float d = -glog(u01w(k == 0 ? w0 : w2)) * ec.inv_maj;
+            const float d = -glog(u01w(k == 0 ? w0 : w2), s_logtab) * ec.inv_maj;
             const float nx = fmaf(d, ph.u, ph.x), ny = fmaf(d, ph.v, ph.y), nz = fmaf(d, ph.w, ph.z);
             const int ix = (int)floorf(nx * hot.isx), iy = (int)floorf(ny * hot.isy),
                       iz = (int)floorf(nz * hot.isz);
@@ -1709,7 +1712,7 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   // 256-thread workgroup keeps to 40 KiB so that four of them share a CU.
   auto static_lds = [](size_t block) {
     return (size_t)N_ENERGIES * block * sizeof(double) + (size_t)QWORDS * block * sizeof(uint32_t) +
-           N_COUNTERS * sizeof(unsigned long long);
+           N_COUNTERS * sizeof(unsigned long long) + 64 * sizeof(float2);
   };
   const size_t budget_big = 160 * 1024 - static_lds(1024) - 512;
   const size_t budget_small = 40 * 1024 - static_lds(256) - 256;

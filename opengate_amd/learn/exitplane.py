@@ -29,6 +29,7 @@ through the network by the hand-written backward GEMMs.
 """
 import ctypes as C
 import math
+import time
 
 import numpy as np
 
@@ -164,11 +165,14 @@ class ExitPlaneGenerator:
         """Train on recorded exit-plane photons (8 arrays, host or device). Returns the
         loss history (one value per step, on the host)."""
         torch = _torch()
+        t0 = time.perf_counter()
         F = exit_features(phsp, self.focal, self.plane_z, self.dev)
         self.qmap = QuantileMap.fit(F, n_knots=n_knots)
         G = QuantileMap.gaussianize(F)
         del F
         N = G.shape[0]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         gen = torch.Generator(device=self.dev).manual_seed(int(seed))
         eye = torch.eye(N_FEATURES, device=self.dev)
         losses = torch.empty(steps, device=self.dev)
@@ -189,6 +193,13 @@ class ExitPlaneGenerator:
             if log_every and (it % log_every == 0 or it == steps - 1):
                 print(f"[exitplane fit] step {it} loss {float(loss):.5f}", flush=True)
         self.history = losses.cpu().numpy()
+        t2 = time.perf_counter()
+        # GEMM work of one step: forward, weight gradients, and input gradients below layer 0
+        kn = [k * n for k, n in zip(self.sizes[:-1], self.sizes[1:])]
+        flops = 2.0 * batch * (2 * sum(kn) + sum(kn[1:]))
+        self.fit_stats = {"steps": int(steps), "batch": int(batch), "setup_s": t1 - t0, "loop_s": t2 - t1,
+                          "step_ms": 1e3 * (t2 - t1) / max(1, steps), "gemm_flop_per_step": flops,
+                          "gemm_tflops_over_loop": flops * steps / max(t2 - t1, 1e-9) / 1e12}
         return self.history
 
     # ------------------------------------------------------------------ generation
