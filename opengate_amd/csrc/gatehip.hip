@@ -909,6 +909,10 @@ transport_kernel(Launch L) {
           fin = true;
         }
       }
+      // polar cosine of an accepted Compton or Rayleigh scatter; one azimuth-and-rotation
+      // block serves both (a wave with lanes of each kind runs it once)
+      bool scat = false;
+      float c_scat = 1.0f;
       if (mode == M_COMPTON && !fin) {
         float q = 1.0f, c = 1.0f;
         bool done = false;
@@ -921,14 +925,12 @@ transport_kernel(Launch L) {
           }
         }
         if (done) {
-          c = fmaxf(c, -1.0f);
           const float e1 = ph.E / q;
           // electrons of photons scored by the kerma estimator give nothing locally
           x_dep = (TLE && !in_world && ph.E <= tle_emax) ? 0.0f : (ph.E - e1) * ph.wt;
-          float cp, sp;
-          azimuth(rng, cp, sp);
-          rotate(ph.u, ph.v, ph.w, c, cp, sp);
           ph.E = e1;
+          c_scat = fmaxf(c, -1.0f);
+          scat = true;
           fin = true;
           comp = true;
         }
@@ -940,11 +942,15 @@ transport_kernel(Launch L) {
           if (rayleigh_try(rng, T, P.nx, crow, ph.E, c) || ++ntry == MAX_REJECT) done = true;
         }
         if (done) {
-          float cp, sp;
-          azimuth(rng, cp, sp);
-          rotate(ph.u, ph.v, ph.w, c, cp, sp);
+          c_scat = c;
+          scat = true;
           fin = true;
         }
+      }
+      if (scat) {
+        float cp, sp;
+        azimuth(rng, cp, sp);
+        rotate(ph.u, ph.v, ph.w, c_scat, cp, sp);
       }
       if (fin) {
         if (!kill) mode = in_world ? M_WORLD : M_STEP;
