@@ -555,6 +555,7 @@ def leg_exitplane(args):
                    "target_learned_over_plain": 3.0,
                    "dose_criterion": criterion,
                    "training_photons": off, "fit_steps": args.ep_fit_steps, "fit_s": t_fit,
+                   "training": dict(gen.fit_stats, frac_of_bf16_peak=gen.fit_stats["gemm_tflops_over_loop"] / 2500.0),
                    "training_record_s": t_train_rec,
                    "learned_over_plain_with_training": plain_s / (learned_s + t_fit + t_train_rec),
                    "generator_photons_per_s": gen_rate,

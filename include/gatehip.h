@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GH_ABI_VERSION 5
+#define GH_ABI_VERSION 6
 
 enum {
   GH_OK = 0,
@@ -50,11 +50,19 @@ enum {
 /* position types of gh_source_desc.pos_type (GateGenericSource.cpp:312-384) */
 enum { GH_POS_POINT = 0, GH_POS_BOX = 1, GH_POS_SPHERE = 2, GH_POS_ELLIPSOID = 3,
        GH_POS_DISC = 4, GH_POS_CYLINDER = 5, GH_POS_PHSP = 6,
-       GH_POS_VOXELS = 7 /* activity image (VoxelSource), Walker alias table over its voxels */ };
-/* direction types (GateGenericSource.cpp:386-480) */
-enum { GH_DIR_ISO = 0, GH_DIR_MOMENTUM = 1, GH_DIR_FOCUSED = 2 };
+       GH_POS_VOXELS = 7, /* activity image (VoxelSource), Walker alias table over its voxels */
+       GH_POS_SURFACE_SPHERE = 8 /* uniform on a sphere's surface (G4SPSPosDistribution "Surface") */ };
+/* direction types (GateGenericSource.cpp:386-480; G4SPSAngDistribution). ISO, COS and
+ * HISTOGRAM give momentum -(sin t cos p, sin t sin p, cos t); on a GH_POS_SURFACE_SPHERE,
+ * ISO and COS take it in the tangent frame at the point (surface_axis below). */
+enum { GH_DIR_ISO = 0, GH_DIR_MOMENTUM = 1, GH_DIR_FOCUSED = 2,
+       GH_DIR_COS = 3,       /* cosine law: sin^2 t uniform between sin^2 of the theta range */
+       GH_DIR_BEAM2D = 4,    /* (a, b) = dir_sigma N(0,1)^2, t = |(a, b)|, p = the angle of (a, b) */
+       GH_DIR_HISTOGRAM = 5  /* t and p piecewise uniform ("user" theta and phi histograms) */ };
 /* energy types (GateGenericSource.cpp:492-664, GateSPSEneDistribution.cpp:16-159) */
-enum { GH_ENE_MONO = 0, GH_ENE_LINES = 1, GH_ENE_HISTOGRAM = 2, GH_ENE_RANGE = 3 };
+enum { GH_ENE_MONO = 0, GH_ENE_LINES = 1, GH_ENE_HISTOGRAM = 2, GH_ENE_RANGE = 3,
+       GH_ENE_GAUSS = 4 /* energy_mono + energy_sigma N(0,1), negative values 0 (G4SPSEneDistribution
+                           "Gauss"); tails beyond energy_mono + 8 energy_sigma are cut there */ };
 /* tally flags */
 enum { GH_TALLY_EDEP2 = 1, GH_TALLY_DOSE = 2, GH_TALLY_DOSE2 = 4, GH_TALLY_COUNTS = 8,
        GH_TALLY_ALIGNED = 16,
@@ -110,7 +118,7 @@ typedef struct gh_phantom_desc {
 
 typedef struct gh_source_desc {
   int32_t pos_type, dir_type, ene_type, n_energy;
-  float pos_param[4];        /* box: size x,y,z; sphere: r; ellipsoid: a,b,c; disc: r; cylinder: r, half z */
+  float pos_param[4];        /* box: size x,y,z; sphere, surface_sphere: r; ellipsoid: a,b,c; disc: r; cylinder: r, half z */
   float pos_mat[12];         /* row-major 3x3 then translation: source frame -> phantom frame */
   float dir_mat[9];          /* row-major 3x3: source direction frame -> phantom frame */
   float cos_theta_min, cos_theta_max, phi_min, phi_max; /* iso: ranges in radians */
@@ -136,6 +144,22 @@ typedef struct gh_source_desc {
   int32_t vox_size[3];       /* activity image size (x, y, z) */
   int32_t reserved2;
   float vox_spacing[3];      /* voxel size [mm]; source frame: image corner at the origin */
+  float pos_sigma[2];        /* GH_POS_DISC: Gaussian spread added to x and y (GateGenericSource.cpp:362-366) */
+  float energy_sigma;        /* GH_ENE_GAUSS: standard deviation */
+  float weight_sigma;        /* >= 0: the history weight is weight + weight_sigma N(0,1)
+                                (GateGenericSource.cpp:247-259); < 0: the weight itself */
+  float dir_sigma[2];        /* GH_DIR_BEAM2D: angular sigma in x and y [rad], each in [0, 6] */
+  float surface_axis[3];     /* GH_POS_SURFACE_SPHERE with GH_DIR_ISO / GH_DIR_COS: axis A of the
+                                tangent frame at the local outward normal n: x' = unit(A x n),
+                                y' = unit(x' x n), z' = n (G4SPSPosDistribution takes A = the
+                                source rotation's third column); nonzero, normalised on upload */
+  int32_t n_theta, n_phi;    /* GH_DIR_HISTOGRAM: bins of the theta and phi tables */
+  const float *theta_value;  /* bin lower edges [rad], within [0, pi] */
+  const float *theta_width;  /* bin widths (0: a point mass) */
+  const float *theta_cdf;    /* normalised cumulative weights, last = 1 */
+  const float *phi_value;    /* the same for phi, within [0, 2 pi] */
+  const float *phi_width;
+  const float *phi_cdf;
 } gh_source_desc;
 
 typedef struct gh_tally_desc {

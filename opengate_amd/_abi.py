@@ -13,9 +13,10 @@ LIB_PATH = PKG / "libgatehip.so"
 
 GH_OK, GH_EINVAL, GH_EHIP, GH_ESTATE, GH_ERANGE = 0, 1, 2, 3, 4
 GEMM_NOALLOC_WS_BYTES = 16 << 20  # GH_GEMM_NOALLOC_WS_BYTES
-POS = {"point": 0, "box": 1, "sphere": 2, "ellipsoid": 3, "disc": 4, "cylinder": 5, "phsp": 6, "voxels": 7}
-DIR = {"iso": 0, "momentum": 1, "focused": 2}
-ENE = {"mono": 0, "spectrum_discrete": 1, "spectrum_histogram": 2, "range": 3}
+POS = {"point": 0, "box": 1, "sphere": 2, "ellipsoid": 3, "disc": 4, "cylinder": 5, "phsp": 6, "voxels": 7,
+       "surface_sphere": 8}
+DIR = {"iso": 0, "momentum": 1, "focused": 2, "cos": 3, "beam2d": 4, "histogram": 5}
+ENE = {"mono": 0, "spectrum_discrete": 1, "spectrum_histogram": 2, "range": 3, "gauss": 4}
 TALLY_EDEP2, TALLY_DOSE, TALLY_DOSE2, TALLY_COUNTS, TALLY_ALIGNED, TALLY_TLE = 1, 2, 4, 8, 16, 32
 
 FP = C.POINTER(C.c_float)
@@ -52,7 +53,10 @@ class SourceDesc(C.Structure):
         ("phsp", FP * 8),
         ("vox_n", C.c_int64), ("vox_prob", FP), ("vox_alias", C.POINTER(C.c_int32)),
         ("vox_index", C.POINTER(C.c_int32)), ("vox_size", C.c_int32 * 3), ("reserved2", C.c_int32),
-        ("vox_spacing", C.c_float * 3),
+        ("vox_spacing", C.c_float * 3), ("pos_sigma", C.c_float * 2), ("energy_sigma", C.c_float),
+        ("weight_sigma", C.c_float), ("dir_sigma", C.c_float * 2), ("surface_axis", C.c_float * 3),
+        ("n_theta", C.c_int32), ("n_phi", C.c_int32), ("theta_value", FP), ("theta_width", FP),
+        ("theta_cdf", FP), ("phi_value", FP), ("phi_width", FP), ("phi_cdf", FP),
     ]
 
 
@@ -201,7 +205,7 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.gh_abi_version() != 5:
+    if lib.gh_abi_version() != 6:
         raise RuntimeError("libgatehip.so ABI version mismatch")
     if path is None:
         _LIB = lib

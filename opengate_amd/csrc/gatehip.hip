@@ -50,6 +50,12 @@ struct SimDev {
   int phi_full, phsp_iso;
   float mom[3], focus[3];
   float e_mono, e_max, weight;
+  float pos_sigma[2], e_sigma, w_sigma;
+  float s2_min, s2_d;      // cosine law: sin^2 theta_min, sin^2 theta_max - sin^2 theta_min
+  float axis[3];           // surface sphere: unit axis of the tangent frame
+  float dir_sigma[2];      // beam2d
+  int n_theta, n_phi;      // histogram direction tables
+  const float *th_val, *th_width, *th_cdf, *ph_val, *ph_width, *ph_cdf;
   const float *e_val, *e_width, *e_cdf;
   const int32_t *e_guide;  // [SPEC_GUIDE+1] guide of e_cdf
   int64_t phsp_n, phsp_start;
@@ -310,8 +316,49 @@ __device__ __forceinline__ int tally_voxel(const SimDev &s, float x, float y, fl
 }
 
 // ---------------------------------------------------------------------- source
+// two independent N(0,1) values (Box-Muller): radius sqrt(-2 log u), angle 2 pi u'
+__device__ __forceinline__ void normal_pair(Rng &g, float &n1, float &n2) {
+  const float r = sqrtf(-2.0f * glog(u01(g), c_logtab));
+  float cp, sp;
+  azimuth(g, cp, sp);
+  n1 = r * cp;
+  n2 = r * sp;
+}
+
+// smallest k < n - 1 with cdf[k] > u, else n - 1 (binary search)
+__device__ __forceinline__ int cdf_index(const float *cdf, int n, float u) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void unit_or_zero(float &x, float &y, float &z) {
+  const float n2 = fmaf(z, z, fmaf(y, y, x * x));
+  if (n2 > 0.0f) {
+    const float inv = 1.0f / sqrtf(n2);
+    x = x * inv; y = y * inv; z = z * inv;
+  }
+}
+
+// (u, v, w) <- u x' + v y' + w n in the tangent frame at a surface point with outward unit
+// normal n: x' = unit(A x n), y' = unit(x' x n) (G4SPSPosDistribution, sphere surface)
+__device__ void tangent_frame(const float *A, float nx, float ny, float nz, float &u, float &v, float &w) {
+  float ax = fmaf(A[1], nz, -(A[2] * ny)), ay = fmaf(A[2], nx, -(A[0] * nz)), az = fmaf(A[0], ny, -(A[1] * nx));
+  unit_or_zero(ax, ay, az);
+  float bx = fmaf(ay, nz, -(az * ny)), by = fmaf(az, nx, -(ax * nz)), bz = fmaf(ax, ny, -(ay * nx));
+  unit_or_zero(bx, by, bz);
+  const float d0 = u, d1 = v, d2 = w;
+  u = fmaf(d0, ax, fmaf(d1, bx, d2 * nx));
+  v = fmaf(d0, ay, fmaf(d1, by, d2 * ny));
+  w = fmaf(d0, az, fmaf(d1, bz, d2 * nz));
+}
+
 __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
   float x = 0.0f, y = 0.0f, z = 0.0f, u = 0.0f, v = 0.0f, w = 1.0f, E = s.e_mono, wt = s.weight;
+  float nx = 0.0f, ny = 0.0f, nz = 1.0f;  // surface sphere: outward unit normal at the point
   if (s.pos_type == GH_POS_PHSP) {
     const int64_t row = (s.phsp_start + (int64_t)h) % s.phsp_n;
     x = s.phsp[0][row]; y = s.phsp[1][row]; z = s.phsp[2][row];
@@ -366,14 +413,39 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
         } while (r2 > 1.0f && ++it < MAX_REJECT);
         x = s.pos_param[0] * a; y = s.pos_param[0] * b;
         if (s.pos_type == GH_POS_CYLINDER) z = fmaf(2.0f, u01(g), -1.0f) * s.pos_param[1];
+        if (s.pos_type == GH_POS_DISC && (s.pos_sigma[0] > 0.0f || s.pos_sigma[1] > 0.0f)) {
+          // "Beam" circle: the uniform point plus a Gaussian spread in x and y
+          float n1, n2;
+          normal_pair(g, n1, n2);
+          x = fmaf(s.pos_sigma[0], n1, x);
+          y = fmaf(s.pos_sigma[1], n2, y);
+        }
+        break;
+      }
+      case GH_POS_SURFACE_SPHERE: {
+        const float ct = fmaf(2.0f, u01(g), -1.0f);
+        const float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+        float cp, sp;
+        azimuth(g, cp, sp);
+        nx = st * cp; ny = st * sp; nz = ct;
+        x = s.pos_param[0] * nx; y = s.pos_param[0] * ny; z = s.pos_param[0] * nz;
         break;
       }
       default: break;
     }
     // ---- direction (G4SPSAngDistribution iso convention: momentum = -(st cp, st sp, ct))
-    if (s.dir_type == GH_DIR_ISO) {
-      const float ct = fmaf(-u01(g), s.d_ct, s.ct_min);
-      const float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+    if (s.dir_type == GH_DIR_ISO || s.dir_type == GH_DIR_COS) {
+      // iso: cos theta uniform in [cos theta_max, cos theta_min]; cos: sin^2 theta uniform
+      // in [sin^2 theta_min, sin^2 theta_max] (GenerateIsotropicFlux / GenerateCosineLawFlux)
+      float ct, st;
+      if (s.dir_type == GH_DIR_ISO) {
+        ct = fmaf(-u01(g), s.d_ct, s.ct_min);
+        st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+      } else {
+        const float s2 = fmaf(u01(g), s.s2_d, s.s2_min);
+        st = sqrtf(s2);
+        ct = sqrtf(fmaxf(1.0f - s2, 0.0f));
+      }
       float cp, sp;
       if (s.phi_full) {
         azimuth(g, cp, sp);
@@ -381,6 +453,32 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
         const float phi = fmaf(u01(g), s.d_phi, s.phi_min);
         gsincos(phi, sp, cp);
       }
+      u = -(st * cp); v = -(st * sp); w = -ct;
+      if (s.pos_type == GH_POS_SURFACE_SPHERE) tangent_frame(s.axis, nx, ny, nz, u, v, w);
+    } else if (s.dir_type == GH_DIR_BEAM2D) {
+      // GenerateBeamFlux ("beam2d"): Gaussian angles (a, b), polar angle |(a, b)|, azimuth
+      // the angle of (a, b), 0 when both are 0
+      float n1, n2;
+      normal_pair(g, n1, n2);
+      const float a = s.dir_sigma[0] * n1, b = s.dir_sigma[1] * n2;
+      const float t = sqrtf(fmaf(b, b, a * a));
+      float cp = 1.0f, sp = 0.0f;
+      if (t > 0.0f) {
+        cp = a / t;
+        sp = b / t;
+      }
+      float sn, cs;
+      gsincos(t, sn, cs);
+      u = -(sn * cp); v = -(sn * sp); w = -cs;
+    } else if (s.dir_type == GH_DIR_HISTOGRAM) {
+      // GenerateUserDefFlux: theta, then phi, each piecewise uniform
+      int k = cdf_index(s.th_cdf, s.n_theta, u01(g));
+      const float t = fmaf(u01(g), s.th_width[k], s.th_val[k]);
+      k = cdf_index(s.ph_cdf, s.n_phi, u01(g));
+      const float ph = fmaf(u01(g), s.ph_width[k], s.ph_val[k]);
+      float st, ct, sp, cp;
+      gsincos(t, st, ct);
+      gsincos(ph, sp, cp);
       u = -(st * cp); v = -(st * sp); w = -ct;
     } else if (s.dir_type == GH_DIR_MOMENTUM) {
       u = s.mom[0]; v = s.mom[1]; w = s.mom[2];
@@ -394,6 +492,16 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
       E = fmaf(u01(g), s.e_width[k], s.e_val[k]);
     } else if (s.ene_type == GH_ENE_RANGE) {
       E = fmaf(u01(g), s.e_max - s.e_mono, s.e_mono);
+    } else if (s.ene_type == GH_ENE_GAUSS) {
+      float n1, n2;
+      normal_pair(g, n1, n2);
+      E = fminf(fmaxf(fmaf(s.e_sigma, n1, s.e_mono), 0.0f), s.e_max);
+    }
+    // Gaussian history weight (GateGenericSource.cpp:247-259)
+    if (s.w_sigma >= 0.0f) {
+      float n1, n2;
+      normal_pair(g, n1, n2);
+      wt = fmaf(s.w_sigma, n1, s.weight);
     }
   }
   // source frame -> phantom frame
@@ -1134,7 +1242,13 @@ void free_all(std::vector<void *> &v) {
 namespace {
 
 int validate_source(gh_ctx *ctx, const gh_source_desc *src) {
-  if (src->pos_type < GH_POS_POINT || src->pos_type > GH_POS_VOXELS) FAIL(GH_EINVAL, "invalid pos_type");
+  if (src->pos_type < GH_POS_POINT || src->pos_type > GH_POS_SURFACE_SPHERE) FAIL(GH_EINVAL, "invalid pos_type");
+  if (!(src->pos_sigma[0] >= 0.0f) || !(src->pos_sigma[1] >= 0.0f) || !std::isfinite(src->pos_sigma[0]) ||
+      !std::isfinite(src->pos_sigma[1]))
+    FAIL(GH_EINVAL, "pos_sigma must be finite and >= 0");
+  if ((src->pos_sigma[0] > 0.0f || src->pos_sigma[1] > 0.0f) && src->pos_type != GH_POS_DISC)
+    FAIL(GH_EINVAL, "pos_sigma applies to disc positions only");
+  if (!std::isfinite(src->weight_sigma)) FAIL(GH_EINVAL, "weight_sigma must be finite");
   if (src->pos_type != GH_POS_VOXELS && (src->vox_n != 0 || src->vox_prob || src->vox_alias || src->vox_index))
     FAIL(GH_EINVAL, "voxel-source fields are set but pos_type is not GH_POS_VOXELS");
   if (src->pos_type == GH_POS_VOXELS) {
@@ -1151,8 +1265,39 @@ int validate_source(gh_ctx *ctx, const gh_source_desc *src) {
       if (!(src->vox_prob[i] >= 0.0f && src->vox_prob[i] <= 1.0f)) FAIL(GH_EINVAL, "voxel source: probability outside [0, 1]");
     }
   }
-  if (src->dir_type < GH_DIR_ISO || src->dir_type > GH_DIR_FOCUSED) FAIL(GH_EINVAL, "invalid dir_type");
-  if (src->ene_type < GH_ENE_MONO || src->ene_type > GH_ENE_RANGE) FAIL(GH_EINVAL, "invalid ene_type");
+  if (src->dir_type < GH_DIR_ISO || src->dir_type > GH_DIR_HISTOGRAM) FAIL(GH_EINVAL, "invalid dir_type");
+  if (src->dir_type == GH_DIR_BEAM2D)
+    for (int k = 0; k < 2; ++k)
+      // |polar angle| <= 6 sqrt(2) x 5.8 (the largest Box-Muller normal) < 16 pi, the sine's range
+      if (!(src->dir_sigma[k] >= 0.0f && src->dir_sigma[k] <= 6.0f)) FAIL(GH_EINVAL, "dir_sigma must lie in [0, 6] rad");
+  if (src->pos_type == GH_POS_SURFACE_SPHERE && (src->dir_type == GH_DIR_ISO || src->dir_type == GH_DIR_COS)) {
+    const float *a = src->surface_axis;
+    if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2]) ||
+        (a[0] == 0.0f && a[1] == 0.0f && a[2] == 0.0f))
+      FAIL(GH_EINVAL, "surface_axis must be finite and nonzero");
+  }
+  if (src->dir_type == GH_DIR_HISTOGRAM) {
+    // piecewise-uniform angle tables: widths >= 0, cdf nondecreasing with last entry 1,
+    // angles within the sine's range
+    auto table_ok = [](int n, const float *val, const float *wid, const float *cdf) {
+      if (n <= 0 || n > (1 << 20) || !val || !wid || !cdf) return false;
+      float prev = 0.0f;
+      for (int i = 0; i < n; ++i) {
+        const float lo = val[i], hi = val[i] + wid[i];
+        if (!(wid[i] >= 0.0f) || !(std::fabs(lo) <= 50.0f) || !(std::fabs(hi) <= 50.0f)) return false;
+        if (!(cdf[i] >= prev) || !(cdf[i] <= 1.0f)) return false;
+        prev = cdf[i];
+      }
+      return cdf[n - 1] == 1.0f;
+    };
+    if (!table_ok(src->n_theta, src->theta_value, src->theta_width, src->theta_cdf))
+      FAIL(GH_EINVAL, "histogram direction: invalid theta table");
+    if (!table_ok(src->n_phi, src->phi_value, src->phi_width, src->phi_cdf))
+      FAIL(GH_EINVAL, "histogram direction: invalid phi table");
+  }
+  if (src->ene_type < GH_ENE_MONO || src->ene_type > GH_ENE_GAUSS) FAIL(GH_EINVAL, "invalid ene_type");
+  if (src->ene_type == GH_ENE_GAUSS && (!(src->energy_sigma >= 0.0f) || !std::isfinite(src->energy_sigma)))
+    FAIL(GH_EINVAL, "energy_sigma must be finite and >= 0");
   if ((src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) &&
       (src->n_energy <= 0 || !src->energy_value || !src->energy_cdf ||
        (src->ene_type == GH_ENE_HISTOGRAM && !src->energy_width)))
@@ -1195,6 +1340,9 @@ int validate_source(gh_ctx *ctx, const gh_source_desc *src) {
       out_of_grid = bad(src->energy_mono);
     } else if (src->ene_type == GH_ENE_RANGE) {
       out_of_grid = bad(src->energy_mono) || bad(src->energy_max) || src->energy_max < src->energy_mono;
+    } else if (src->ene_type == GH_ENE_GAUSS) {
+      // the sampler cuts the tail at mean + 8 sigma: that energy must be on the grid
+      out_of_grid = bad(src->energy_mono) || bad(src->energy_mono + 8.0f * src->energy_sigma);
     } else {
       for (int i = 0; i < src->n_energy && !out_of_grid; ++i) {
         const float top = src->ene_type == GH_ENE_HISTOGRAM ? src->energy_value[i] + src->energy_width[i]
@@ -1224,6 +1372,20 @@ int upload_source(gh_ctx *ctx, const gh_source_desc *src, SimDev &d, std::vector
   std::memcpy(d.pos_param, src->pos_param, sizeof(d.pos_param));
   std::memcpy(d.pos_mat, src->pos_mat, sizeof(d.pos_mat));
   std::memcpy(d.dir_mat, src->dir_mat, sizeof(d.dir_mat));
+  {
+    // cosine law and tangent axis: FP64, rounded once
+    const double c0 = src->cos_theta_min, c1 = src->cos_theta_max;
+    const double s0 = 1.0 - c0 * c0, s1 = 1.0 - c1 * c1;
+    d.s2_min = (float)s0;
+    d.s2_d = (float)(s1 - s0);
+    const double a0 = src->surface_axis[0], a1 = src->surface_axis[1], a2 = src->surface_axis[2];
+    const double an = std::sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    d.axis[0] = an > 0.0 ? (float)(a0 / an) : 0.0f;
+    d.axis[1] = an > 0.0 ? (float)(a1 / an) : 0.0f;
+    d.axis[2] = an > 0.0 ? (float)(a2 / an) : 1.0f;
+    d.dir_sigma[0] = src->dir_sigma[0];
+    d.dir_sigma[1] = src->dir_sigma[1];
+  }
   d.ct_min = src->cos_theta_min;
   d.d_ct = src->cos_theta_min - src->cos_theta_max;
   d.phi_min = src->phi_min;
@@ -1233,6 +1395,10 @@ int upload_source(gh_ctx *ctx, const gh_source_desc *src, SimDev &d, std::vector
   std::memcpy(d.mom, src->momentum, sizeof(d.mom));
   std::memcpy(d.focus, src->focus, sizeof(d.focus));
   d.e_mono = src->energy_mono; d.e_max = src->energy_max; d.weight = src->weight;
+  if (src->ene_type == GH_ENE_GAUSS) d.e_max = src->energy_mono + 8.0f * src->energy_sigma;
+  d.pos_sigma[0] = src->pos_sigma[0]; d.pos_sigma[1] = src->pos_sigma[1];
+  d.e_sigma = src->energy_sigma;
+  d.w_sigma = src->weight_sigma;
   float *t;
   if (src->ene_type == GH_ENE_LINES || src->ene_type == GH_ENE_HISTOGRAM) {
     if ((rc = dev_upload(ctx, src->energy_value, src->n_energy, &t, owned))) return rc;
@@ -1254,6 +1420,24 @@ int upload_source(gh_ctx *ctx, const gh_source_desc *src, SimDev &d, std::vector
       if ((rc = dev_upload(ctx, src->energy_width, src->n_energy, &t, owned))) return rc;
       d.e_width = t;
     }
+  }
+  d.n_theta = d.n_phi = 0;
+  d.th_val = d.th_width = d.th_cdf = d.ph_val = d.ph_width = d.ph_cdf = nullptr;
+  if (src->dir_type == GH_DIR_HISTOGRAM) {
+    d.n_theta = src->n_theta;
+    d.n_phi = src->n_phi;
+    if ((rc = dev_upload(ctx, src->theta_value, src->n_theta, &t, owned))) return rc;
+    d.th_val = t;
+    if ((rc = dev_upload(ctx, src->theta_width, src->n_theta, &t, owned))) return rc;
+    d.th_width = t;
+    if ((rc = dev_upload(ctx, src->theta_cdf, src->n_theta, &t, owned))) return rc;
+    d.th_cdf = t;
+    if ((rc = dev_upload(ctx, src->phi_value, src->n_phi, &t, owned))) return rc;
+    d.ph_val = t;
+    if ((rc = dev_upload(ctx, src->phi_width, src->n_phi, &t, owned))) return rc;
+    d.ph_width = t;
+    if ((rc = dev_upload(ctx, src->phi_cdf, src->n_phi, &t, owned))) return rc;
+    d.ph_cdf = t;
   }
   d.vox_n = 0; d.vox_prob = nullptr; d.vox_alias = nullptr; d.vox_index = nullptr;
   if (src->pos_type == GH_POS_VOXELS) {

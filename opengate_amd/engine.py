@@ -24,7 +24,8 @@ from .geometry.volumes import BoxVolume, ImageVolume
 from .physics import xs
 from .sources.gan import GANSource
 from .sources.generic import GenericSource, PhaseSpaceSource, VoxelSource
-from .transport import GpuEngine, PhantomSpec, SourceSpec, TallySpec, spectrum_histogram, spectrum_lines
+from .transport import (GpuEngine, PhantomSpec, SourceSpec, TallySpec, angle_histogram, energy_user_histogram,
+                        spectrum_histogram, spectrum_lines)
 from .units import g4_units
 
 VACUUM = ("G4_Galactic", "Vacuum", "G4_VACUUM")
@@ -153,6 +154,8 @@ class SimulationEngine:
             pp = (pos.radius, 0.0, 0.0, 0.0)
         elif kind == "disc":
             pp = (pos.radius, 0.0, 0.0, 0.0)
+        elif kind == "surface_sphere":
+            pp = (pos.radius, 0.0, 0.0, 0.0)
         elif kind == "cylinder":
             pp = (pos.radius, pos.dz if pos.dz is not None else 0.0, 0.0, 0.0)
         elif kind == "ellipsoid":
@@ -166,14 +169,40 @@ class SimulationEngine:
         dir_rot = self.R.T @ (Rg if rel else np.eye(3))
         spec = SourceSpec(pos_type=kind, pos_param=pp, pos_rot=self.R.T @ Rg,
                           pos_trans=tuple(self.R.T @ (Tg - self.t) + self.L / 2), dir_rot=dir_rot,
-                          weight=1.0 if src.weight is None or src.weight < 0 else float(src.weight))
+                          weight=float(src.weight) if src.weight is not None and src.weight > 0 else 1.0)
         if kind == "voxels":
             spec.activity = act
             spec.activity_spacing = tuple(sp)
-        if d.type == "iso":
-            spec.dir_type = "iso"
+        sx, sy = float(pos.get("sigma_x", 0.0) or 0.0), float(pos.get("sigma_y", 0.0) or 0.0)
+        if sx < 0 or sy < 0:
+            raise ValueError(f"source '{src.name}': position.sigma_x / sigma_y must be >= 0")
+        if (sx > 0 or sy > 0) and kind != "disc":
+            raise NotImplementedError(f"source '{src.name}': position.sigma_x / sigma_y apply to disc positions")
+        spec.pos_sigma = (sx, sy)
+        # a Gaussian weight only around a positive weight, as GateGenericSource::GeneratePrimaries
+        ws = src.weight_sigma
+        if ws is not None and ws >= 0 and src.weight is not None and src.weight > 0:
+            spec.weight_sigma = float(ws)
+        if d.type in ("iso", "cos"):
+            spec.dir_type = d.type
             spec.theta = (float(d.theta[0]), float(d.theta[1]))
             spec.phi = (float(d.phi[0]), float(d.phi[1]))
+            # on a sphere surface the angles are taken in the tangent frame built with the
+            # source rotation's third column (G4SPSPosDistribution::GeneratePointsOnSurface)
+            spec.surface_axis = tuple(Rg[:, 2])
+        elif d.type == "beam2d":
+            sig = np.asarray(d.sigma, dtype=np.float64).ravel()
+            if len(sig) != 2 or np.any(~np.isfinite(sig)) or np.any(sig < 0):
+                raise ValueError(f"source '{src.name}': direction.sigma must be two values >= 0")
+            if np.any(sig > 6.0):
+                raise NotImplementedError(f"source '{src.name}': direction.sigma above 6 rad")
+            spec.dir_type, spec.dir_sigma = "beam2d", (float(sig[0]), float(sig[1]))
+        elif d.type == "histogram":
+            # GateGenericSource.cpp:437-466; the angles outside [0, pi] / [0, 2 pi] are drawn
+            # again by Geant4, which clipping the bins reproduces
+            spec.dir_type = "histogram"
+            spec.theta_table = angle_histogram(d.histogram_theta_angles, d.histogram_theta_weights, 0.0, np.pi)
+            spec.phi_table = angle_histogram(d.histogram_phi_angles, d.histogram_phi_weights, 0.0, 2 * np.pi)
         elif d.type == "momentum":
             spec.dir_type = "momentum"
             spec.momentum = tuple(np.asarray(d.momentum, dtype=np.float64))
@@ -192,6 +221,15 @@ class SimulationEngine:
             spec.ene_type, spec.energy = "mono", float(e.mono)
         elif e.type == "range":
             spec.ene_type, spec.energy, spec.energy_max = "range", float(e.min_energy), float(e.max_energy)
+        elif e.type == "gauss":
+            if float(e.sigma_gauss) < 0:
+                raise ValueError(f"source '{src.name}': sigma_gauss must be >= 0")
+            spec.ene_type, spec.energy, spec.energy_sigma = "gauss", float(e.mono), float(e.sigma_gauss)
+        elif e.type == "histogram":
+            # G4SPSEneDistribution "User" points (GateGenericSource.cpp:530-542)
+            spec.ene_type = "spectrum_histogram"
+            spec.energy_value, spec.energy_width, spec.energy_cdf = energy_user_histogram(
+                e.histogram_energy, e.histogram_weight)
         elif e.type == "spectrum_discrete":
             spec.ene_type = "spectrum_discrete"
             spec.energy_value, _, spec.energy_cdf = spectrum_lines(e.spectrum_energies, e.spectrum_weights)

@@ -124,7 +124,8 @@ class GenericSource(SourceBase):
     user_info_defaults = {
         "particle": ("gamma", {"doc": "only 'gamma' on the photon path"}),
         "weight": (-1, {"doc": "history weight (-1: 1)"}),
-        "weight_sigma": (-1, {"doc": "not supported"}),
+        "weight_sigma": (-1, {"doc": "if >= 0 (and weight > 0), the history weight is Gaussian around weight "
+                                     "with this sigma (GateGenericSource.cpp:247-259)"}),
         "direction_relative_to_attached_volume": (False, {"doc": "rotate directions with the attached volume"}),
         "position": (_position_parameters(), {"doc": "position parameters"}),
         "direction": (_direction_parameters(), {"doc": "direction parameters"}),
@@ -136,7 +137,7 @@ class GenericSource(SourceBase):
     def total_yield(self):
         """Sum of the spectrum weights by which the reference multiplies the activity
         (GateGenericSource.cpp:544-574 spectrum_discrete: the weights; :583-632
-        spectrum_histogram: weight x bin width)."""
+        spectrum_histogram: weight x bin width; :530-542 histogram: the weights)."""
         e = self.energy
         if e.type == "spectrum_discrete":
             return float(np.sum(np.asarray(e.spectrum_weights, dtype=np.float64)))
@@ -144,6 +145,8 @@ class GenericSource(SourceBase):
             w = np.asarray(e.spectrum_weights, dtype=np.float64)
             edges = np.asarray(e.spectrum_energy_bin_edges, dtype=np.float64)
             return float(np.sum(w * np.diff(edges)))
+        if e.type == "histogram":
+            return float(np.sum(np.asarray(e.histogram_weight, dtype=np.float64)))
         return 1.0
 
 

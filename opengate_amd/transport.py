@@ -76,6 +76,57 @@ def spectrum_histogram(edges, weights):
     return e[:-1].astype(np.float32), np.diff(e).astype(np.float32), cdf
 
 
+def angle_histogram(edges, weights, lo, hi):
+    """Piecewise-uniform table (lower edges, widths, cdf) of a "user" angle histogram:
+    GateGenericSource.cpp:437-466 gives G4SPSAngDistribution the points (edges[0], 0),
+    (edges[i], weights[i-1]), whose inverse cumulative is linear between points
+    (G4PhysicsVector::GetEnergy), so bin i is uniform on (edges[i], edges[i+1]] with
+    probability weights[i]. G4SPSAngDistribution::GenerateUserDefFlux draws again while
+    the angle lies outside [lo, hi] ([0, pi] for theta, [0, 2 pi] for phi): the bins are
+    clipped to that range with their weights scaled by the part kept, which is the same law."""
+    e = np.asarray(edges, dtype=np.float64)
+    w = np.asarray(weights, dtype=np.float64)
+    if len(e) != len(w) + 1 or len(w) == 0:
+        raise ValueError("an angle histogram needs one more angle than weights")
+    if np.any(np.diff(e) <= 0):
+        raise ValueError("histogram angles must increase strictly")
+    if np.any(w < 0) or not np.all(np.isfinite(w)) or not np.all(np.isfinite(e)):
+        raise ValueError("histogram weights must be finite and >= 0")
+    a, b = np.maximum(e[:-1], lo), np.minimum(e[1:], hi)
+    keep = b > a
+    frac = np.where(keep, (b - a) / np.diff(e), 0.0)
+    wk = (w * frac)[keep]
+    if wk.sum() <= 0:
+        raise ValueError(f"the angle histogram has no weight inside [{lo}, {hi}]")
+    acc = np.cumsum(wk)
+    cdf = (acc / acc[-1]).astype(np.float32)
+    cdf[-1] = 1.0
+    return a[keep].astype(np.float32), (b - a)[keep].astype(np.float32), cdf
+
+
+def energy_user_histogram(energies, weights):
+    """Energy type "histogram" (GateGenericSource.cpp:530-542: G4SPSEneDistribution "User"
+    with the points (energies[i], weights[i])). G4SPSEneDistribution::GenerateUserEnergies
+    normalises the running sum of the weights and inverts it linearly between points: the
+    first weight is a point mass at energies[0], weight i > 0 is spread uniformly on
+    (energies[i-1], energies[i]]. Returned as a piecewise-uniform table (lower edges,
+    widths, cdf) whose first bin has width 0."""
+    e = np.asarray(energies, dtype=np.float64)
+    w = np.asarray(weights, dtype=np.float64)
+    if len(e) != len(w) or len(e) == 0:
+        raise ValueError("histogram_energy and histogram_weight must have the same length")
+    if np.any(np.diff(e) <= 0):
+        raise ValueError("histogram_energy must increase strictly")
+    if np.any(w < 0) or not np.all(np.isfinite(w)) or w.sum() <= 0:
+        raise ValueError("histogram_weight must be finite, >= 0 and not all 0")
+    lo = np.concatenate([[e[0]], e[:-1]])
+    width = np.concatenate([[0.0], np.diff(e)])
+    acc = np.cumsum(w)
+    cdf = (acc / acc[-1]).astype(np.float32)
+    cdf[-1] = 1.0
+    return lo.astype(np.float32), width.astype(np.float32), cdf
+
+
 def voxel_alias_table(activity):
     """(flat voxel indices with activity > 0, prob, alias) for an activity image (z, y, x):
     the Walker alias table of the library (gh_alias_table, a host-only routine)."""
@@ -118,6 +169,13 @@ class SourceSpec:
     phsp_iso: bool = False
     activity: Optional[np.ndarray] = None  # voxels: activity image (z, y, x), image corner at the source origin
     activity_spacing: tuple = (1.0, 1.0, 1.0)  # voxels: voxel size (x, y, z) mm
+    pos_sigma: tuple = (0.0, 0.0)  # disc: Gaussian spread added to x and y (mm)
+    energy_sigma: float = 0.0  # gauss: standard deviation (MeV)
+    weight_sigma: float = -1.0  # >= 0: Gaussian history weight around `weight`
+    dir_sigma: tuple = (0.0, 0.0)  # beam2d: angular sigma in x and y (rad)
+    surface_axis: tuple = (0.0, 0.0, 1.0)  # surface_sphere iso / cos: axis of the tangent frame
+    theta_table: Optional[tuple] = None  # histogram direction: (lower edges, widths, cdf) of theta
+    phi_table: Optional[tuple] = None  # ... and of phi
 
     def _phsp_on_device(self):
         return self.phsp is not None and hasattr(self.phsp[0], "data_ptr")
@@ -131,6 +189,9 @@ class SourceSpec:
             return float(self.energy)
         if self.ene_type == "range":
             return float(self.energy_max)
+        if self.ene_type == "gauss":
+            # the sampler cuts the Gaussian tail at 8 standard deviations
+            return float(self.energy) + 8.0 * float(self.energy_sigma)
         if self.ene_type == "spectrum_histogram":
             return float(self.energy_value[-1] + self.energy_width[-1])
         return float(np.max(self.energy_value))
@@ -156,6 +217,18 @@ class SourceSpec:
         d.energy_mono = float(self.energy)
         d.energy_max = float(self.energy_max)
         d.weight = float(self.weight)
+        d.pos_sigma = (C.c_float * 2)(float(self.pos_sigma[0]), float(self.pos_sigma[1]))
+        d.energy_sigma = float(self.energy_sigma)
+        d.weight_sigma = float(self.weight_sigma)
+        d.dir_sigma = (C.c_float * 2)(float(self.dir_sigma[0]), float(self.dir_sigma[1]))
+        d.surface_axis = _f3(self.surface_axis)
+        if self.dir_type == "histogram":
+            if self.theta_table is None or self.phi_table is None:
+                raise ValueError("a histogram direction needs theta_table and phi_table")
+            self._ang = [np.ascontiguousarray(a, dtype=np.float32) for a in (*self.theta_table, *self.phi_table)]
+            d.n_theta, d.n_phi = len(self._ang[0]), len(self._ang[3])
+            d.theta_value, d.theta_width, d.theta_cdf, d.phi_value, d.phi_width, d.phi_cdf = (
+                A.fptr(a) for a in self._ang)
         if self.ene_type in ("spectrum_discrete", "spectrum_histogram"):
             self._ev = np.ascontiguousarray(self.energy_value, dtype=np.float32)
             self._ec = np.ascontiguousarray(self.energy_cdf, dtype=np.float32)

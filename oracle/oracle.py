@@ -63,6 +63,8 @@ def lib():
                                          A.FP, A.FP]
         L.ref_sample_rayleigh.argtypes = [C.POINTER(A.PhysicsDesc), C.c_int, C.c_float, C.c_uint64, C.c_uint64,
                                           A.FP]
+        L.ref_sample_source.argtypes = [C.POINTER(A.SourceDesc), C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32,
+                                        C.POINTER(A.FP * 8)]
         L.ref_mu_at.restype = C.c_float
         L.ref_mu_at.argtypes = [C.POINTER(A.PhysicsDesc), C.c_int, C.c_float]
         L.ref_cell_index.restype = C.c_int
@@ -165,6 +167,18 @@ def sample_rayleigh(tables, m, E, n, seed=1):
     c = np.zeros(n, np.float32)
     lib().ref_sample_rayleigh(C.byref(pd), m, E, n, seed, A.fptr(c))
     return c
+
+
+def sample_source(source, n, seed=1, first=0, sim_id=0):
+    """Primaries of histories first .. first + n - 1 as emitted by the source, before any
+    transport: 8 float32 arrays (x, y, z, u, v, w, E, weight)."""
+    from opengate_amd import _abi as A
+
+    sd = source.desc()
+    out = [np.zeros(n, np.float32) for _ in range(8)]
+    ptrs = (A.FP * 8)(*[A.fptr(a) for a in out])
+    lib().ref_sample_source(C.byref(sd), n, first, seed, sim_id, C.byref(ptrs))
+    return out
 
 
 def mu_at(tables, m, E):

@@ -436,8 +436,25 @@ typedef struct {
   const gh_source_desc *src;
   const gh_tally_desc *tal;
   float inv_s[3], L[3], tinv[3], d_ct, d_phi;
+  float s2_min, s2_d;   /* cosine law: sin^2 of theta_min, sin^2 theta_max - sin^2 theta_min */
+  float axis[3];        /* surface sphere: unit axis of the tangent frame */
   int aligned;
 } sim_t;
+
+/* per-source constants of the direction samplers, computed in FP64 and rounded once */
+static void sim_source_init(sim_t *S, const gh_source_desc *src) {
+  S->d_ct = src->cos_theta_min - src->cos_theta_max;
+  S->d_phi = src->phi_max - src->phi_min;
+  const double c0 = src->cos_theta_min, c1 = src->cos_theta_max;
+  const double s0 = 1.0 - c0 * c0, s1 = 1.0 - c1 * c1;
+  S->s2_min = (float)s0;
+  S->s2_d = (float)(s1 - s0);
+  const double a0 = src->surface_axis[0], a1 = src->surface_axis[1], a2 = src->surface_axis[2];
+  const double an = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+  S->axis[0] = an > 0.0 ? (float)(a0 / an) : 0.0f;
+  S->axis[1] = an > 0.0 ? (float)(a1 / an) : 0.0f;
+  S->axis[2] = an > 0.0 ? (float)(a2 / an) : 1.0f;
+}
 
 static int label_at(const sim_t *S, int v) {
   return S->ph->label_bytes == 2 ? (int)((const uint16_t *)S->ph->labels)[v] : (int)((const uint8_t *)S->ph->labels)[v];
@@ -528,10 +545,41 @@ typedef struct {
   float p[3], d[3], E, w;
 } photon_t;
 
+/* d = d0 x' + d1 y' + d2 n in the tangent frame of a surface point with outward unit
+   normal n (G4SPSPosDistribution::GeneratePointsOnSurface, sphere): x' = unit(A x n),
+   y' = unit(x' x n); a zero vector stays zero */
+static void unit_or_zero(float v[3]) {
+  const float n2 = fmaf(v[2], v[2], fmaf(v[1], v[1], v[0] * v[0]));
+  if (n2 > 0.0f) {
+    const float inv = 1.0f / sqrtf(n2);
+    v[0] = v[0] * inv; v[1] = v[1] * inv; v[2] = v[2] * inv;
+  }
+}
+
+static void tangent_frame(const float A[3], const float n[3], float d[3]) {
+  float xp[3] = {fmaf(A[1], n[2], -(A[2] * n[1])), fmaf(A[2], n[0], -(A[0] * n[2])), fmaf(A[0], n[1], -(A[1] * n[0]))};
+  unit_or_zero(xp);
+  float yp[3] = {fmaf(xp[1], n[2], -(xp[2] * n[1])), fmaf(xp[2], n[0], -(xp[0] * n[2])),
+                 fmaf(xp[0], n[1], -(xp[1] * n[0]))};
+  unit_or_zero(yp);
+  const float d0 = d[0], d1 = d[1], d2 = d[2];
+  for (int i = 0; i < 3; ++i) d[i] = fmaf(d0, xp[i], fmaf(d1, yp[i], d2 * n[i]));
+}
+
+/* two independent N(0,1) values by Box-Muller: radius sqrt(-2 log u), angle 2 pi u' */
+static void normal_pair(stream_t *rs, float *n1, float *n2) {
+  const float r = sqrtf(-2.0f * ref_log(uni(rs)));
+  float cp, sp;
+  ref_azimuth(rs, &cp, &sp);
+  *n1 = r * cp;
+  *n2 = r * sp;
+}
+
 static void emit(const sim_t *S, stream_t *rs, uint64_t h, photon_t *ph) {
   const gh_source_desc *src = S->src;
   float lp[3] = {0.0f, 0.0f, 0.0f}, ld[3] = {0.0f, 0.0f, 1.0f};
   float E = src->energy_mono, w = src->weight;
+  float nrm[3] = {0.0f, 0.0f, 1.0f}; /* surface sphere: outward unit normal at the point */
   if (src->pos_type == GH_POS_PHSP) {
     int64_t row = (src->phsp_start + (int64_t)h) % src->phsp_n;
     for (int k = 0; k < 3; ++k) {
@@ -579,10 +627,38 @@ static void emit(const sim_t *S, stream_t *rs, uint64_t h, photon_t *ph) {
       lp[0] = src->pos_param[0] * a;
       lp[1] = src->pos_param[0] * b;
       if (src->pos_type == GH_POS_CYLINDER) lp[2] = fmaf(2.0f, uni(rs), -1.0f) * src->pos_param[1];
+      if (src->pos_type == GH_POS_DISC && (src->pos_sigma[0] > 0.0f || src->pos_sigma[1] > 0.0f)) {
+        /* G4SPSPosDistribution "Beam" circle: the uniform point plus a Gaussian spread */
+        float n1, n2;
+        normal_pair(rs, &n1, &n2);
+        lp[0] = fmaf(src->pos_sigma[0], n1, lp[0]);
+        lp[1] = fmaf(src->pos_sigma[1], n2, lp[1]);
+      }
+    } else if (src->pos_type == GH_POS_SURFACE_SPHERE) {
+      /* G4SPSPosDistribution "Surface" sphere: cos(theta) uniform in [-1, 1], phi uniform */
+      const float ct = fmaf(2.0f, uni(rs), -1.0f);
+      const float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+      float cp, sp;
+      ref_azimuth(rs, &cp, &sp);
+      nrm[0] = st * cp;
+      nrm[1] = st * sp;
+      nrm[2] = ct;
+      lp[0] = src->pos_param[0] * nrm[0];
+      lp[1] = src->pos_param[0] * nrm[1];
+      lp[2] = src->pos_param[0] * nrm[2];
     }
-    if (src->dir_type == GH_DIR_ISO) {
-      float ct = fmaf(-uni(rs), S->d_ct, src->cos_theta_min);
-      float st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+    if (src->dir_type == GH_DIR_ISO || src->dir_type == GH_DIR_COS) {
+      /* G4SPSAngDistribution::GenerateIsotropicFlux: cos t uniform in [cos t_max, cos t_min];
+         GenerateCosineLawFlux: sin^2 t uniform in [sin^2 t_min, sin^2 t_max] */
+      float ct, st;
+      if (src->dir_type == GH_DIR_ISO) {
+        ct = fmaf(-uni(rs), S->d_ct, src->cos_theta_min);
+        st = sqrtf(fmaxf(fmaf(-ct, ct, 1.0f), 0.0f));
+      } else {
+        const float s2 = fmaf(uni(rs), S->s2_d, S->s2_min);
+        st = sqrtf(s2);
+        ct = sqrtf(fmaxf(1.0f - s2, 0.0f));
+      }
       float cp, sp;
       if (src->phi_full) {
         ref_azimuth(rs, &cp, &sp);
@@ -590,6 +666,36 @@ static void emit(const sim_t *S, stream_t *rs, uint64_t h, photon_t *ph) {
         float phi = fmaf(uni(rs), S->d_phi, src->phi_min);
         ref_sincos(phi, &sp, &cp);
       }
+      ld[0] = -(st * cp);
+      ld[1] = -(st * sp);
+      ld[2] = -ct;
+      if (src->pos_type == GH_POS_SURFACE_SPHERE) tangent_frame(S->axis, nrm, ld);
+    } else if (src->dir_type == GH_DIR_BEAM2D) {
+      /* G4SPSAngDistribution::GenerateBeamFlux ("beam2d"): angles (a, b) Gaussian, polar
+         angle |(a, b)|, azimuth the angle of (a, b) (0 when both are 0) */
+      float n1, n2;
+      normal_pair(rs, &n1, &n2);
+      const float a = src->dir_sigma[0] * n1, b = src->dir_sigma[1] * n2;
+      const float t = sqrtf(fmaf(b, b, a * a));
+      float cp = 1.0f, sp = 0.0f;
+      if (t > 0.0f) {
+        cp = a / t;
+        sp = b / t;
+      }
+      float sn, cs;
+      ref_sincos(t, &sn, &cs);
+      ld[0] = -(sn * cp);
+      ld[1] = -(sn * sp);
+      ld[2] = -cs;
+    } else if (src->dir_type == GH_DIR_HISTOGRAM) {
+      /* G4SPSAngDistribution::GenerateUserDefFlux: theta, then phi, each from its table */
+      int k = first_above(src->theta_cdf, src->n_theta, uni(rs));
+      const float t = fmaf(uni(rs), src->theta_width[k], src->theta_value[k]);
+      k = first_above(src->phi_cdf, src->n_phi, uni(rs));
+      const float p = fmaf(uni(rs), src->phi_width[k], src->phi_value[k]);
+      float st, ct, sp, cp;
+      ref_sincos(t, &st, &ct);
+      ref_sincos(p, &sp, &cp);
       ld[0] = -(st * cp);
       ld[1] = -(st * sp);
       ld[2] = -ct;
@@ -603,6 +709,19 @@ static void emit(const sim_t *S, stream_t *rs, uint64_t h, photon_t *ph) {
       E = fmaf(uni(rs), src->energy_width[k], src->energy_value[k]);
     } else if (src->ene_type == GH_ENE_RANGE) {
       E = fmaf(uni(rs), src->energy_max - src->energy_mono, src->energy_mono);
+    } else if (src->ene_type == GH_ENE_GAUSS) {
+      /* G4SPSEneDistribution "Gauss": mean + sigma N(0,1), negative energies 0; the tail is
+         cut at mean + 8 sigma */
+      float n1, n2;
+      normal_pair(rs, &n1, &n2);
+      const float top = src->energy_mono + 8.0f * src->energy_sigma;
+      E = fminf(fmaxf(fmaf(src->energy_sigma, n1, src->energy_mono), 0.0f), top);
+    }
+    if (src->weight_sigma >= 0.0f) {
+      /* Gaussian history weight (GateGenericSource.cpp:247-259) */
+      float n1, n2;
+      normal_pair(rs, &n1, &n2);
+      w = fmaf(src->weight_sigma, n1, src->weight);
     }
   }
   const float *M = src->pos_mat;
@@ -885,8 +1004,7 @@ int ref_run(const gh_physics_desc *phys, const gh_phantom_desc *ph, const gh_sou
     S.L[i] = (float)((double)nn[i] * (double)ph->spacing[i]);
     S.tinv[i] = 1.0f / (S.aligned ? ph->spacing[i] : tal->spacing[i]);
   }
-  S.d_ct = src->cos_theta_min - src->cos_theta_max;
-  S.d_phi = src->phi_max - src->phi_min;
+  sim_source_init(&S, src);
   size_t ntal = (size_t)tal->nx * tal->ny * tal->nz;
   work_t *W = (work_t *)calloc((size_t)n_threads, sizeof(work_t));
   pthread_t *T = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));
@@ -976,6 +1094,28 @@ void ref_sample_rayleigh(const gh_physics_desc *phys, int m, float E, uint64_t n
     stream_t s;
     stream_open(&s, seed, i, 0);
     c_out[i] = rayleigh_sample(&s, &P, m, E);
+  }
+}
+
+/* primaries of histories first .. first + n - 1 as the source emits them (before any
+   transport): position, direction, energy and weight into out[0..7] */
+void ref_sample_source(const gh_source_desc *src, uint64_t n, uint64_t first, uint64_t seed, uint32_t sim_id,
+                       float *const out[8]) {
+  sim_t S;
+  memset(&S, 0, sizeof(S));
+  S.src = src;
+  sim_source_init(&S, src);
+  for (uint64_t i = 0; i < n; ++i) {
+    stream_t s;
+    stream_open(&s, seed, first + i, sim_id);
+    photon_t ph;
+    emit(&S, &s, i, &ph);
+    for (int k = 0; k < 3; ++k) {
+      out[k][i] = ph.p[k];
+      out[3 + k][i] = ph.d[k];
+    }
+    out[6][i] = ph.E;
+    out[7][i] = ph.w;
   }
 }
 

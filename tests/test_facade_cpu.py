@@ -127,7 +127,7 @@ def test_c_abi_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(lib, name), name
     assert set(A.SIGNATURES) == declared
-    assert lib.gh_abi_version() == 5
+    assert lib.gh_abi_version() == 6
     # no compute without a GPU: creating a context fails cleanly when no device is visible
     import torch
 

@@ -12,7 +12,7 @@ import pytest
 
 from oracle import oracle as O
 from opengate_amd import workloads
-from opengate_amd.transport import SourceSpec, spectrum_lines
+from opengate_amd.transport import SourceSpec, angle_histogram, spectrum_lines
 
 from helpers import aligned_tally, centre, small_phantom, tables
 from test_gpu_parity import INT_KEYS, compare, run_both
@@ -75,8 +75,8 @@ def test_config3_batch64_full_set_one_launch(engine):
 
 
 POS_CASES = [("point", ()), ("box", (14.0, 9.0, 20.0)), ("sphere", (9.0,)), ("ellipsoid", (12.0, 6.0, 9.0)),
-             ("disc", (11.0,)), ("cylinder", (8.0, 10.0))]
-DIR_CASES = ["iso", "momentum", "focused"]
+             ("disc", (11.0,)), ("cylinder", (8.0, 10.0)), ("surface_sphere", (9.0,))]
+DIR_CASES = ["iso", "momentum", "focused", "cos", "beam2d", "histogram"]
 
 
 @pytest.mark.gpu
@@ -96,9 +96,53 @@ def test_every_position_and_direction_type(engine, pos_type, param, dir_type):
                      pos_trans=(cx + 1.3, cy - 0.7, cz + 2.1), dir_rot=rot, dir_type=dir_type,
                      momentum=(0.3, -0.4, 0.866), focus=(cx - 20.0, cy + 5.0, cz),
                      ene_type="spectrum_discrete", energy_value=ev, energy_cdf=cdf)
-    if dir_type == "iso":
-        src.theta = (0.2, 2.5)
+    if dir_type in ("iso", "cos"):
+        src.theta = (0.2, 2.5) if dir_type == "iso" else (0.1, 1.3)
+        src.phi = (0.3, 4.0)
+        src.surface_axis = (0.2, -0.4, 0.9)
+    elif dir_type == "beam2d":
+        src.dir_sigma = (0.05, 0.3)
+    elif dir_type == "histogram":
+        src.theta_table = angle_histogram([0.2, 0.9, 1.4, 2.9], [1.0, 0.0, 2.0], 0.0, np.pi)
+        src.phi_table = angle_histogram([-0.5, 1.0, 2.0, 2.0001, 7.0], [1.0, 3.0, 2.0, 0.5], 0.0, 2 * np.pi)
     tal = aligned_tally(ph, counts=True)
     c, g, rc, r = run_both(engine, T, ph, src, tal, 60_000, 31)
     compare(c, rc, g, r)
     assert c["histories"] == 60_000
+
+
+SPREAD_CASES = {
+    # disc with its Gaussian spread (GateGenericSource.cpp:343-366) and a Gaussian energy
+    "disc-spread-gauss": dict(pos_type="disc", pos_param=(11.0, 0, 0, 0), pos_sigma=(3.0, 1.0),
+                              ene_type="gauss", energy=0.14, energy_sigma=0.008),
+    # Gaussian energy with 16 % of the draws negative (energy 0: stops at birth) and a
+    # Gaussian history weight (GateGenericSource.cpp:247-259)
+    "sphere-gauss-at-zero-weight": dict(pos_type="surface_sphere", pos_param=(9.0, 0, 0, 0),
+                                        ene_type="gauss", energy=0.02, energy_sigma=0.02,
+                                        weight=2.0, weight_sigma=0.5),
+    # Gaussian weight on a line spectrum: weights of both signs reach the tally
+    "box-weight": dict(pos_type="box", pos_param=(14.0, 9.0, 20.0, 0), weight=0.3, weight_sigma=0.3),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", list(SPREAD_CASES))
+def test_gaussian_spreads_energies_and_weights(engine, case):
+    """Gaussian source spreads (disc sigma_x / sigma_y, energy type gauss, weight_sigma) and
+    the surface sphere, with a rotated source frame, against the oracle bit for bit."""
+    T = tables(e_max=0.21)
+    ph = small_phantom(24, 2.5)
+    a = 0.4
+    rot = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    cx, cy, cz = centre(ph)
+    kw = dict(SPREAD_CASES[case])
+    if "ene_type" not in kw:
+        ev, _, cdf = spectrum_lines([0.11295, 0.20837], [0.062, 0.1038])
+        kw.update(ene_type="spectrum_discrete", energy_value=ev, energy_cdf=cdf)
+    src = SourceSpec(pos_rot=rot, pos_trans=(cx + 1.3, cy - 0.7, cz + 2.1), dir_rot=rot, theta=(0.0, 2.0), **kw)
+    tal = aligned_tally(ph, counts=True)
+    c, g, rc, r = run_both(engine, T, ph, src, tal, 60_000, 37)
+    compare(c, rc, g, r)
+    assert c["histories"] == 60_000
+    if kw.get("energy") == 0.02:
+        assert c["cut_kills"] > 0.1 * 60_000  # the energy-0 births stop at once
