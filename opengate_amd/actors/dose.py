@@ -163,7 +163,9 @@ class DoseActor(ActorBase):
         "size": ([10, 10, 10], {"doc": "voxels, or 'like_image_volume'"}),
         "spacing": ([1.0, 1.0, 1.0], {"doc": "mm, or 'like_image_volume'"}),
         "translation": ([0.0, 0.0, 0.0], {"doc": "grid centre relative to the attached volume"}),
-        "rotation": (np.eye(3), {"doc": "grid rotation (identity only)"}),
+        "rotation": (np.eye(3), {"doc": "rotation written into the output images' geometry; the "
+                                        "scoring grid stays aligned with the attached volume, as "
+                                        "GateDoseActor attaches it (GateDoseActor.cpp:83-118)"}),
         "repeated_volume_index": (0, {}),
         "hit_type": ("random", {"allowed_values": ("random", "pre", "post", "middle"),
                                 "doc": "photon deposits are at interaction points; all values give the same result"}),
@@ -219,26 +221,36 @@ class DoseActor(ActorBase):
         o = self._outputs
         return o["dose"].active or o["dose_squared"].active or o["dose_uncertainty"].active
 
-    def output_origin(self, size, spacing, volume):
-        """Origin of the output images (VoxelDepositActor._update_output_coordinate_system,
-        doseactors.py:170-232) for an identity rotation."""
+    def output_geometry(self, size, spacing, volume, global_transform):
+        """(origin, direction) of the output images (VoxelDepositActor.
+        _update_output_coordinate_system, doseactors.py:170-232): None gives the grid centred
+        at 0 without rotation; "local" the grid at `translation`, rotated by `rotation`;
+        "attached_to_image" that placed by the image's native origin and direction;
+        "global" that placed by the attached volume's world transform
+        (`global_transform` = (R, t), get_transform_world_to_local)."""
+        size = np.asarray(size, dtype=np.float64)
+        spacing = np.asarray(spacing, dtype=np.float64)
         origin = -size * spacing / 2.0 + spacing / 2.0
-        local = origin + np.asarray(self.translation, dtype=np.float64)
+        R = np.asarray(self.rotation, dtype=np.float64).reshape(3, 3)
+        local = R @ origin + np.asarray(self.translation, dtype=np.float64)
         if self.output_coordinate_system is None:
-            return origin
+            return origin, np.eye(3)
         if self.output_coordinate_system == "local":
-            return local
+            return local, R
         if self.output_coordinate_system == "attached_to_image":
-            return volume.native_rotation @ local + volume.native_translation
-        # global: attached volume placed at its translation (mother = world)
-        return np.asarray(volume.rotation) @ local + np.asarray(volume.translation, dtype=np.float64)
+            Rn = np.asarray(volume.native_rotation, dtype=np.float64)
+            return Rn @ local + np.asarray(volume.native_translation, dtype=np.float64), Rn @ R
+        Rg, tg = global_transform
+        return Rg @ local + tg, Rg @ R
 
-    def store_results(self, size, spacing, origin, edep, edep2, dose, dose2, counts, density, n_events):
+    def store_results(self, size, spacing, origin, edep, edep2, dose, dose2, counts, density, n_events,
+                      direction=None):
         self.number_of_samples = int(n_events)
         o = self._outputs
+        direction = np.eye(3) if direction is None else direction
 
         def img(a):
-            return Image(np.ascontiguousarray(a), spacing, origin)
+            return Image(np.ascontiguousarray(a), spacing, origin, direction)
 
         o["edep"]._data = img(edep)
         if edep2 is not None:

@@ -273,8 +273,6 @@ class SimulationEngine:
         a = self.dose
         if a.attached_to_name != vol.name:
             raise NotImplementedError("the DoseActor must be attached to the phantom volume")
-        if not np.allclose(_rot(a.rotation), np.eye(3)):
-            raise NotImplementedError("DoseActor.rotation other than identity")
         size, spacing = a.resolved_grid(vol)
         tr = np.asarray(a.translation, dtype=np.float64)
         aligned = (np.array_equal(size, self.size) and np.allclose(spacing, self.spacing) and not tr.any())
@@ -473,7 +471,7 @@ class SimulationEngine:
             a = self.dose
             size, spacing = self.tally_grid
             vvox = float(np.prod(spacing))
-            origin = a.output_origin(size, spacing, vol)
+            origin, direction = a.output_geometry(size, spacing, vol, sim.volume_manager.global_transform(vol.name))
             edep = t["edep"]
             edep2 = t.get("edep2")
             dose = dose2 = density = None
@@ -493,7 +491,8 @@ class SimulationEngine:
                     dose2 = t["dose2"] * k * k
             if a.density.active and density is None:
                 raise NotImplementedError("density output needs the dose grid of the image")
-            a.store_results(size, spacing, origin, edep, edep2, dose, dose2, t.get("counts"), density, n_events)
+            a.store_results(size, spacing, origin, edep, edep2, dose, dose2, t.get("counts"), density, n_events,
+                            direction)
             a.write_outputs()
         if self.spect is not None:
             self.spect.finish()

@@ -186,3 +186,47 @@ def test_uncertainty_goal_rule_matches_the_reference_loop():
     assert dose_actor_mean_uncertainty(np.ones(3), np.ones(3), n, 10, 0.5) == pytest.approx(1.0)
     assert next_uncertainty_check(0.1, 0.05, 1000, 1.05) == 4200
     assert next_uncertainty_check(0.0999, 0.05, 1001, 1.0) == int((0.0999 / 0.05) * (0.0999 / 0.05) * 1001)
+
+
+def test_dose_output_geometry_with_rotation(tmp_path):
+    """Output origin and direction of VoxelDepositActor._update_output_coordinate_system
+    (doseactors.py:170-232), worked by hand: size (4, 2, 2), spacing (2, 1, 3) give the
+    centred origin (-3, -0.5, -1.5); rotation Rz(90) maps it to (0.5, -3, -1.5), plus the
+    translation (10, 0, 0). The attached volume sits at (5, 0, 0) rotated by Rx(90) inside a
+    mother at (0, 0, 100): "global" is Rx (10.5, -3, -1.5) + (5, 0, 100) = (15.5, 1.5, 97)."""
+    sim = gate.Simulation()
+    mother = sim.add_volume("Box", "mother")
+    mother.size = [500.0, 500.0, 500.0]
+    mother.translation = [0.0, 0.0, 100.0]
+    box = sim.add_volume("Box", "box")
+    box.mother = "mother"
+    box.size = [100.0, 100.0, 100.0]
+    box.translation = [5.0, 0.0, 0.0]
+    Rx = np.array([[1.0, 0, 0], [0, 0, -1], [0, 1, 0]])
+    Rz = np.array([[0.0, -1, 0], [1, 0, 0], [0, 0, 1]])
+    box.rotation = Rx
+    dose = sim.add_actor("DoseActor", "dose")
+    dose.attached_to = "box"
+    dose.size = [4, 2, 2]
+    dose.spacing = [2.0, 1.0, 3.0]
+    dose.translation = [10.0, 0.0, 0.0]
+    dose.rotation = Rz
+    gt = sim.volume_manager.global_transform("box")
+    size, spacing = np.array([4, 2, 2]), np.array([2.0, 1.0, 3.0])
+    expect = {None: ([-3.0, -0.5, -1.5], np.eye(3)), "local": ([10.5, -3.0, -1.5], Rz),
+              "global": ([15.5, 1.5, 97.0], Rx @ Rz)}
+    for system, (o, d) in expect.items():
+        dose.output_coordinate_system = system
+        origin, direction = dose.output_geometry(size, spacing, box, gt)
+        np.testing.assert_allclose(origin, o, atol=1e-12)
+        np.testing.assert_allclose(direction, d, atol=1e-12)
+    # the direction reaches the written header (TransformMatrix) and comes back
+    dose.output_coordinate_system = "global"
+    origin, direction = dose.output_geometry(size, spacing, box, gt)
+    z = np.zeros((2, 2, 4))
+    dose.store_results(size, spacing, origin, z, None, None, None, None, None, 10, direction)
+    img = dose.edep.image
+    np.testing.assert_allclose(img.direction, Rx @ Rz, atol=1e-12)
+    back = read_image(write_image(img, tmp_path / "rot_edep.mhd"))
+    np.testing.assert_allclose(back.direction, Rx @ Rz, atol=1e-12)
+    np.testing.assert_allclose(back.origin, [15.5, 1.5, 97.0], atol=1e-12)
