@@ -690,7 +690,7 @@ transport_kernel(Launch L) {
   ph.mode = MODE_BOX; ph.sim = 0; ph.hist = 0;
   ph.x = ph.y = ph.z = ph.u = ph.v = ph.w = ph.E = ph.wt = 0.0f;
   ECache ec;
-  ec.b = 0; ec.frac = 0.0f; ec.maj = 1.0f; ec.inv_maj = 1.0f;
+  ec.b = 0; ec.frac = 0.0f; ec.maj24 = 0x1p-24f; ec.inv_maj = 1.0f;
   Rng rng;
   // the key is the run's seed in every lane from the start, so the compiler sees one
   // wave-uniform key (scalar registers) instead of a per-lane value
@@ -920,7 +920,9 @@ transport_kernel(Launch L) {
                 const float x = (ph.E * ph.wt) * (lerp_node_scaled(P.muen + mr.row * NB1, mr.rho, ec) * ec.inv_maj);
                 deposit(s0, sq, tally_voxel(s0, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
               }
-              if (u01w(k == 0 ? w1 : w3) * ec.maj < mu) {
+              // u01 x majorant < mu, with u01 = ((w >> 8) | 1) 2^-24 and the 2^-24 folded into
+              // maj24: the same single rounding of the same exact product
+              if ((float)(((k == 0 ? w1 : w3) >> 8) | 1u) * ec.maj24 < mu) {
                 mode = M_COLLIDE;
                 cvox = vox;
                 clab = lab;

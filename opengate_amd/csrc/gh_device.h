@@ -45,8 +45,10 @@ GH_DEV void philox10(uint32_t &x0, uint32_t &x1, uint32_t &x2, uint32_t &x3, uin
     if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
     const uint64_t p0 = (uint64_t)0xD2511F53u * x0;
     const uint64_t p1 = (uint64_t)0xCD9E8D57u * x2;
-    const uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
-    const uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+    // the round's two three-way XORs as one bitop3 each (truth table 0x96 = a ^ b ^ c):
+    // the compiler emits two XORs per output otherwise
+    const uint32_t y0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), x1, k0, 0x96);
+    const uint32_t y2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), x3, k1, 0x96);
     x1 = (uint32_t)p1;
     x3 = (uint32_t)p0;
     x0 = y0;
@@ -259,7 +261,7 @@ GH_DEV int cell_index(float p, float c, float s, float inv) {
 struct ECache {
   int b;         // bucket
   float frac;    // position inside the bucket
-  float maj;     // majorant at E
+  float maj24;   // majorant at E times 2^-24 (the accept test's uniform without its scaling)
   float inv_maj;
 };
 
@@ -268,8 +270,9 @@ GH_DEV void ecache_set(ECache &c, float E, const float *maj) {
   c.b = (int)(bits >> ESHIFT) - BUCKET_BASE;
   c.frac = (float)(bits & EMASK) * 0x1p-17f;
   const float m0 = maj[c.b], m1 = maj[c.b + 1];
-  c.maj = fmaf(c.frac, m1 - m0, m0);
-  c.inv_maj = 1.0f / c.maj;
+  const float m = fmaf(c.frac, m1 - m0, m0);
+  c.maj24 = m * 0x1p-24f;
+  c.inv_maj = 1.0f / m;
 }
 
 GH_DEV float lerp_node(const float *row, const ECache &c) {
