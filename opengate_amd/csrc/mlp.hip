@@ -376,7 +376,8 @@ extern "C" int gh_gemm_bf16_noalloc(const void *A, int64_t sam, int64_t sak, con
 // LDS. Layer l: Z = X W_l + b_l; hidden layers X_{l+1} = bf16(relu(Z)), the last layer is
 // written as FP32. Weights are given transposed, Wt_l[n][k] (k contiguous), so that each
 // lane's MFMA B fragment (column n, 8 consecutive k) is one 16-byte load from L2-resident
-// memory. A workgroup of 4 waves owns 16 RT rows; in a hidden layer wave w computes the
+// memory. This 4-wave form serves hidden width 128 (width 256 has the 8-wave form below).
+// A workgroup of 4 waves owns 16 RT rows; in a hidden layer wave w computes the
 // columns [w H/4, (w+1) H/4) for all rows (RT row tiles x H/64 column tiles of 16x16),
 // reading its A fragments (row, 8 consecutive k) from the LDS activation tile. Both
 // operands of k step ks+1 are loaded while the MFMAs of step ks run, into a second set of
@@ -394,9 +395,6 @@ constexpr int MAXL = 8;          // layers
 constexpr int K0 = 32;           // first-layer input width (padded)
 constexpr int NOUT = 16;         // last-layer output width (padded)
 // 16-row tiles per workgroup, per hidden width (measured best on MI355X)
-#ifndef GH_MLP_RT256
-#define GH_MLP_RT256 7           // 112 rows
-#endif
 #ifndef GH_MLP_RT128
 #define GH_MLP_RT128 8           // 128 rows
 #endif
@@ -593,17 +591,213 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) m
   }
 }
 
+// Fused MLP inference for hidden width 256: 8 waves (512 threads) own 224 rows, in two row
+// groups of 112 (rg = wave / 4) and four column groups of 64 (cg = wave % 4), each wave
+// the same 7 x 4 tiles of 16 x 16 as the 4-wave form. The weights of each k step of 32
+// (all 256 columns, 16 KB) are staged in LDS once for the 224 rows, half the L2 traffic
+// per row of the 4-wave form, by direct global-to-LDS loads (global_load_lds_dwordx4, no
+// registers in flight), double-buffered. Per k step: wait for this thread's loads of the
+// next step, a barrier (every thread's loads visible, every wave's reads of the stage to
+// be refilled done), the loads of the step after next, then the LDS reads of the next
+// step's fragments while the MFMAs of this step run on fragments read one step earlier.
+// The activation tile (224 x 256, pitch 264) is overwritten in place by each layer's
+// epilogue. Same MFMA sequence per output element as the 4-wave form and the layer-by-layer
+// GEMMs: bit-identical results.
+constexpr int W8_RT = 7;                                   // row tiles per wave
+constexpr int W8_ROWS = 2 * 16 * W8_RT;                    // 224 rows per workgroup
+constexpr int W8_P = 256 + 8;                              // activation row pitch (bf16)
+constexpr size_t W8_ACT = (size_t)W8_ROWS * W8_P * 2;      // 118 272 bytes
+constexpr size_t W8_STAGE = 256 * 32 * 2;                  // one k step of weights: 16 KB
+constexpr size_t W8_LDS = W8_ACT + 2 * W8_STAGE;           // 151 040 bytes
+
+// k step ks of a 256-column weight matrix Wt[n][k] (row length K) into a stage: 16-byte
+// slot s holds k group s / 256 (8 values) of column s % 256, so that the 16 lanes of an
+// MFMA fragment read (16 consecutive columns, one k group) read 256 consecutive bytes.
+// Each of the 512 threads moves two slots; lane i of an instruction writes its 16 bytes
+// at the instruction's LDS base + 16 i.
+__device__ __forceinline__ void w8_stage_load(unsigned char *stage, const __bf16 *W, int K, int ks, int tid) {
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int s = i * 512 + wave * 64 + lane;
+    const int n = s & 255, kg = s >> 8;
+    const __bf16 *src = W + (int64_t)n * K + ks * 32 + kg * 8;
+    unsigned char *dst = stage + (size_t)(i * 512 + wave * 64) * 16;
+    __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+  }
+}
+
+__global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__ X, int64_t ldx, int64_t M,
+                                                      MlpArgs a, float *__restrict__ Z, int64_t ldz) {
+  constexpr int H = 256, RT = W8_RT, CT = 4, P = W8_P, ROWS = W8_ROWS;
+  extern __shared__ __align__(16) unsigned char mlp_lds[];
+  __bf16 *act = (__bf16 *)mlp_lds;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int rg = wave >> 2, cg = wave & 3;
+  const int64_t row0 = (int64_t)blockIdx.x * ROWS;
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  // k steps numbered over the layers (step 0: the input layer's single step, then 8 per
+  // hidden layer); step g uses stage g % 2
+  int gs = 0;
+  auto stage = [&](int g) { return mlp_lds + W8_ACT + (size_t)(g & 1) * W8_STAGE; };
+  const int nsteps = 1 + (a.n_layers - 2) * (H / 32);
+  int issued = 0;  // steps below `issued` have their weight loads issued
+  auto issue_to = [&](int g) {
+    for (; issued <= g && issued < nsteps; ++issued) {
+      if (issued == 0) {
+        w8_stage_load(stage(0), a.wt[0], K0, 0, tid);
+      } else {
+        const int l = 1 + (issued - 1) / (H / 32), ks = (issued - 1) % (H / 32);
+        w8_stage_load(stage(issued), a.wt[l], H, ks, tid);
+      }
+    }
+  };
+  issue_to(0);  // the input layer's weights load while the input tile does
+  if (a.x_vec) {
+    for (int i = tid; i < ROWS * (K0 / 8); i += 512) {
+      const int r = i / (K0 / 8), q = (i - r * (K0 / 8)) * 8;
+      const int64_t gr = row0 + r;
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
+      if (gr < M && q < a.k_in) {
+        v = *(const bf16x8 *)(X + gr * ldx + q);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (q + j >= a.k_in) v[j] = (__bf16)0.0f;
+      }
+      *(bf16x8 *)(act + r * P + q) = v;
+    }
+  } else {
+    for (int i = tid; i < ROWS * K0; i += 512) {
+      const int r = i / K0, k = i - r * K0;
+      const int64_t gr = row0 + r;
+      act[r * P + k] = (gr < M && k < a.k_in) ? X[gr * ldx + k] : (__bf16)0.0f;
+    }
+  }
+  const int arow0 = rg * 16 * RT + fr;
+  for (int l = 0; l + 1 < a.n_layers; ++l) {
+    const float *__restrict__ bias = a.bias[l];
+    floatx4 acc[RT][CT];
+#pragma unroll
+    for (int i = 0; i < RT; ++i)
+#pragma unroll
+      for (int j = 0; j < CT; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    auto loadA = [&](bf16x8 *af, int ks) {
+#pragma unroll
+      for (int i = 0; i < RT; ++i) af[i] = *(const bf16x8 *)(act + (arow0 + i * 16) * P + ks * 32 + fk);
+    };
+    auto loadB = [&](bf16x8 *b, int g) {
+      const unsigned char *st = stage(g);
+#pragma unroll
+      for (int j = 0; j < CT; ++j) b[j] = *(const bf16x8 *)(st + (size_t)((lane >> 4) * 256 + cg * 64 + j * 16 + fr) * 16);
+    };
+    auto steps = [&](auto nk_c) {
+      constexpr int NK = decltype(nk_c)::value;
+      bf16x8 aA[RT], aB[RT], bA[CT], bB[CT];
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();  // the layer input is complete, and so are the weights of its first step
+      issue_to(gs + 1);
+      loadB(bA, gs);
+      loadA(aA, 0);
+      auto step = [&](int ks, const bf16x8 *aCur, const bf16x8 *bCur, bf16x8 *aNext, bf16x8 *bNext) {
+        if (ks + 1 < NK) {
+          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+          __syncthreads();
+          issue_to(gs + 2);
+          loadB(bNext, gs + 1);
+          loadA(aNext, ks + 1);
+        }
+#pragma unroll
+        for (int i = 0; i < RT; ++i)
+#pragma unroll
+          for (int j = 0; j < CT; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bCur[j], aCur[i], acc[i][j], 0, 0, 0);
+        ++gs;
+      };
+#pragma unroll
+      for (int ks = 0; ks < NK; ks += 2) {
+        step(ks, aA, bA, aB, bB);
+        if (ks + 1 < NK) step(ks + 1, aB, bB, aA, bA);
+      }
+    };
+    if (l == 0) steps(std::integral_constant<int, K0 / 32>{});
+    else steps(std::integral_constant<int, H / 32>{});
+    // every wave has read the layer input: overwrite it with the output
+    __syncthreads();
+    const int g4 = (lane >> 4) * 4;
+#pragma unroll
+    for (int j = 0; j < CT; ++j) {
+      const int c = cg * 64 + j * 16 + g4;
+      const floatx4 bb = *(const floatx4 *)(bias + c);
+#pragma unroll
+      for (int i = 0; i < RT; ++i) {
+        bf16x4 v;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = f2bf(relu(acc[i][j][r] + bb[r]));
+        *(bf16x4 *)(act + (arow0 + i * 16) * P + c) = v;
+      }
+    }
+  }
+  __syncthreads();
+  // output layer (NOUT = 16 columns): the 14 row tiles over the 8 waves
+  const int l = a.n_layers - 1;
+  const __bf16 *__restrict__ W = a.wt[l];
+  const float *__restrict__ bias = a.bias[l];
+  bf16x8 wout[H / 32];
+#pragma unroll
+  for (int k = 0; k < H / 32; ++k) wout[k] = *(const bf16x8 *)(W + (int64_t)fr * H + k * 32 + fk);
+  constexpr int NT = 2 * RT, NTW = (NT + 7) / 8;
+  floatx4 acc[NTW];
+#pragma unroll
+  for (int t = 0; t < NTW; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < H / 32; ++k) {
+#pragma unroll
+    for (int t = 0; t < NTW; ++t) {
+      const int i = wave + 8 * t;
+      if (i < NT) {
+        const bf16x8 af = *(const bf16x8 *)(act + (i * 16 + fr) * P + k * 32 + fk);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, wout[k], acc[t], 0, 0, 0);
+      }
+    }
+  }
+  const int c = fr;
+  if (c < a.n_out) {
+    const float bv = bias[c];
+#pragma unroll
+    for (int t = 0; t < NTW; ++t) {
+      const int i = wave + 8 * t;
+      if (i >= NT) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = row0 + i * 16 + (lane >> 4) * 4 + r;
+        if (row < M) Z[row * ldz + c] = acc[t][r] + bv;
+      }
+    }
+  }
+}
+
 template <int H>
 int launch_mlp(const __bf16 *X, int64_t ldx, int64_t M, const MlpArgs &a, float *Z, int64_t ldz, hipStream_t st) {
-  constexpr int RT = H == 256 ? GH_MLP_RT256 : GH_MLP_RT128, SB = GH_MLP_SB;
-  constexpr size_t lds = mlp_lds_bytes<H, RT, SB>();
-  const void *fn = (const void *)&mlp_forward_kernel<H, RT, SB>;
-  if (lds > 64 * 1024 &&
-      hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-    return GH_EHIP;
-  const dim3 grid((unsigned)((M + 16 * RT - 1) / (16 * RT)));
-  hipLaunchKernelGGL((mlp_forward_kernel<H, RT, SB>), grid, dim3(256), lds, st, X, ldx, M, a, Z, ldz);
-  return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
+  if constexpr (H == 256) {
+    const void *fn = (const void *)&mlp_forward_w8;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)W8_LDS) != hipSuccess)
+      return GH_EHIP;
+    const dim3 grid((unsigned)((M + W8_ROWS - 1) / W8_ROWS));
+    hipLaunchKernelGGL(mlp_forward_w8, grid, dim3(512), W8_LDS, st, X, ldx, M, a, Z, ldz);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
+  } else {
+    constexpr int RT = GH_MLP_RT128, SB = GH_MLP_SB;
+    constexpr size_t lds = mlp_lds_bytes<H, RT, SB>();
+    const void *fn = (const void *)&mlp_forward_kernel<H, RT, SB>;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return GH_EHIP;
+    const dim3 grid((unsigned)((M + 16 * RT - 1) / (16 * RT)));
+    hipLaunchKernelGGL((mlp_forward_kernel<H, RT, SB>), grid, dim3(256), lds, st, X, ldx, M, a, Z, ldz);
+    return hipGetLastError() == hipSuccess ? GH_OK : GH_EHIP;
+  }
 }
 
 }  // namespace
