@@ -325,16 +325,6 @@ __device__ __forceinline__ void normal_pair(Rng &g, float &n1, float &n2) {
   n2 = r * sp;
 }
 
-// smallest k < n - 1 with cdf[k] > u, else n - 1 (binary search)
-__device__ __forceinline__ int cdf_index(const float *cdf, int n, float u) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (cdf[mid] > u) hi = mid; else lo = mid + 1;
-  }
-  return lo;
-}
-
 __device__ __forceinline__ void unit_or_zero(float &x, float &y, float &z) {
   const float n2 = fmaf(z, z, fmaf(y, y, x * x));
   if (n2 > 0.0f) {
@@ -356,7 +346,14 @@ __device__ void tangent_frame(const float *A, float nx, float ny, float nz, floa
   w = fmaf(d0, az, fmaf(d1, bz, d2 * nz));
 }
 
-__device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
+// source-spectrum tables of a simulation: its global arrays, or copies in LDS
+struct SpecTab {
+  const float *val, *width, *cdf;
+  const int32_t *guide;
+};
+constexpr int SPEC_LDS_MAX = 256;  // spectra up to this many bins are staged in LDS (1024-thread forms)
+
+__device__ void sample_source(const SimDev &s, const SpecTab &sp, Rng &g, uint32_t h, Photon &p) {
   float x = 0.0f, y = 0.0f, z = 0.0f, u = 0.0f, v = 0.0f, w = 1.0f, E = s.e_mono, wt = s.weight;
   float nx = 0.0f, ny = 0.0f, nz = 1.0f;  // surface sphere: outward unit normal at the point
   if (s.pos_type == GH_POS_PHSP) {
@@ -485,11 +482,11 @@ __device__ void sample_source(const SimDev &s, Rng &g, uint32_t h, Photon &p) {
     }
     // ---- energy (GateSPSEneDistribution.cpp:16-159)
     if (s.ene_type == GH_ENE_LINES) {
-      const int k = cdf_index_guided(s.e_cdf, s.e_guide, u01(g));
-      E = s.e_val[k];
+      const int k = cdf_index_guided(sp.cdf, sp.guide, u01(g));
+      E = sp.val[k];
     } else if (s.ene_type == GH_ENE_HISTOGRAM) {
-      const int k = cdf_index_guided(s.e_cdf, s.e_guide, u01(g));
-      E = fmaf(u01(g), s.e_width[k], s.e_val[k]);
+      const int k = cdf_index_guided(sp.cdf, sp.guide, u01(g));
+      E = fmaf(u01(g), sp.width[k], sp.val[k]);
     } else if (s.ene_type == GH_ENE_RANGE) {
       E = fmaf(u01(g), s.e_max - s.e_mono, s.e_mono);
     } else if (s.ene_type == GH_ENE_GAUSS) {
@@ -672,6 +669,23 @@ transport_kernel(Launch L) {
   // the logarithm's table (a per-lane index: LDS rather than constant memory)
   __shared__ float2 s_logtab[64];
   if (threadIdx.x < 64) s_logtab[threadIdx.x] = c_logtab[threadIdx.x];
+  // the source spectrum of a single-simulation launch, read at every birth: in LDS when it
+  // is short (three dependent global reads per birth otherwise)
+  __shared__ float s_spec[BLOCK == 1024 ? 3 * SPEC_LDS_MAX : 1];
+  __shared__ int32_t s_sguide[BLOCK == 1024 ? SPEC_GUIDE + 1 : 1];
+  SpecTab spec0{L.sims[0].e_val, L.sims[0].e_width, L.sims[0].e_cdf, L.sims[0].e_guide};
+  if (BLOCK == 1024 && !BATCHED) {
+    const SimDev &s0 = L.sims[0];
+    if ((s0.ene_type == GH_ENE_LINES || s0.ene_type == GH_ENE_HISTOGRAM) && s0.n_energy <= SPEC_LDS_MAX) {
+      for (int i = threadIdx.x; i < s0.n_energy; i += (int)blockDim.x) {
+        s_spec[i] = s0.e_val[i];
+        s_spec[2 * SPEC_LDS_MAX + i] = s0.e_cdf[i];
+        if (s0.e_width) s_spec[SPEC_LDS_MAX + i] = s0.e_width[i];
+      }
+      for (int i = threadIdx.x; i <= SPEC_GUIDE; i += (int)blockDim.x) s_sguide[i] = s0.e_guide[i];
+      spec0 = SpecTab{s_spec, s_spec + SPEC_LDS_MAX, s_spec + 2 * SPEC_LDS_MAX, s_sguide};
+    }
+  }
   if (threadIdx.x < N_COUNTERS) s_cnt[threadIdx.x] = 0ull;
   for (int k = 0; k < N_ENERGIES; ++k) s_en[k * BLOCK + threadIdx.x] = 0.0;
   __syncthreads();
@@ -787,7 +801,7 @@ transport_kernel(Launch L) {
       bh = (uint32_t)(cur_next + lane);
       Rng g;
       rng_init(g, L.seed, s.first_hist + bh, (uint32_t)(BATCHED ? cur_sim : 0));
-      sample_source(s, g, bh, b);
+      sample_source(s, BATCHED ? SpecTab{s.e_val, s.e_width, s.e_cdf, s.e_guide} : spec0, g, bh, b);
       st.inc(C_HIST);
       const int bvox = sim_voxel(s, b.x, b.y, b.z);
       if (b.E < P.e_cut || (TLE && b.E <= tle_emin)) {
@@ -1900,11 +1914,13 @@ int gh_run(gh_ctx *ctx, const uint64_t *n_histories, const uint64_t *first_histo
   // staging level and block: all tables in LDS shared by one 1024-thread workgroup per CU
   // when they fit, else mu + majorant in LDS with 256-thread workgroups, else none. The
   // budgets leave room for the kernel's static LDS (per-lane energy slots, birth queues and
-  // counters: 72 KiB at 1024 threads, 18 KiB at 256) within the 160 KiB of a CU; a
+  // counters: 72 KiB at 1024 threads, 18 KiB at 256; a short source spectrum: 4 KiB at 1024)
+  // within the 160 KiB of a CU; a
   // 256-thread workgroup keeps to 40 KiB so that four of them share a CU.
   auto static_lds = [](size_t block) {
     return (size_t)N_ENERGIES * block * sizeof(double) + (size_t)QWORDS * block * sizeof(uint32_t) +
-           N_COUNTERS * sizeof(unsigned long long) + 64 * sizeof(float2);
+           N_COUNTERS * sizeof(unsigned long long) + 64 * sizeof(float2) +
+           (block == 1024 ? (3 * SPEC_LDS_MAX + SPEC_GUIDE + 1) * sizeof(float) : 0);
   };
   const size_t budget_big = 160 * 1024 - static_lds(1024) - 512;
   const size_t budget_small = 40 * 1024 - static_lds(256) - 256;
