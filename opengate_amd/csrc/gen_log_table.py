@@ -1,20 +1,18 @@
 """Prints the 64-entry table of the library's logarithm (glog in gh_device.h; the oracle's
-ref_log holds the same literals). x = 2^e m1 with m1 in [1, 2); i = the top six mantissa
-bits of x. Entries 0..31 cover m = m1 in [1, 1.5), entries 32..63 cover m = m1 / 2 in
-[0.75, 1): c_i = 1 + (i + 1/2) / 64 (halved for i >= 32), except that the two entries
-next to 1 (i = 0 and i = 63) take c = 1 exactly, so that log x near 1 is log1p of an exact
-r = m - 1. Entry i holds rcp_i = the float nearest 1 / c_i and lg_i = the float nearest
--log(rcp_i). Usage: python gen_log_table.py"""
+ref_log holds the same literals). x = 2^e m with m in [0.5, 1) (frexp); i = the top six
+mantissa bits of x, so entry i covers m in [1/2 + i/128, 1/2 + (i+1)/128) and takes
+c_i = 1/2 + (i + 1/2)/128, the middle of that interval, except the last entry (next to
+1), which takes c = 1 exactly, so that log x just below 1 is log1p of an exact r = m - 1.
+Entry i holds rcp_i = the float nearest 1 / c_i and lg_i = the float nearest -log(rcp_i).
+Usage: python gen_log_table.py"""
 import numpy as np
 
 
 def table():
     out = []
     for i in range(64):
-        c = 1.0 + (i + 0.5) / 64.0
-        if i >= 32:
-            c *= 0.5
-        if i in (0, 63):
+        c = 0.5 + (i + 0.5) / 128.0
+        if i == 63:
             c = 1.0
         rcp = np.float32(1.0 / c)
         lg = np.float32(-np.log(np.float64(rcp)))

@@ -80,20 +80,19 @@ GH_DEV uint32_t rng_next(Rng &g) {
 GH_DEV float u01(Rng &g) { return (float)((rng_next(g) >> 8) | 1u) * 0x1p-24f; }
 
 // ---------------------------------------------------------------- math
-// Table of glog (written by gen_log_table.py). x = 2^e m1, m1 in [1, 2), i = the top six
-// mantissa bits of x: entries 0..31 cover m = m1 in [1, 1.5), entries 32..63 m = m1 / 2 in
-// [0.75, 1) (with e + 1). Entry i holds rcp_i, the float nearest 1 / c_i with c_i the
-// middle of its interval (c = 1 for the two intervals next to 1), and lg_i, the float
-// nearest -log(rcp_i).
+// Table of glog (written by gen_log_table.py). x = 2^e m with m in [0.5, 1) (frexp), i =
+// the top six mantissa bits of x: entry i covers m in [1/2 + i/128, 1/2 + (i+1)/128) and
+// holds rcp_i, the float nearest 1 / c_i with c_i the middle of its interval (c = 1 for the
+// last interval, next to 1), and lg_i, the float nearest -log(rcp_i).
 static __device__ __constant__ float2 c_logtab[64] = {
-    {0x1p+0f, -0x0p+0f}, {0x1.f4465ap-1f, 0x1.7b91acp-6f}, {0x1.ecc07cp-1f, 0x1.39e86ep-5f}, {0x1.e573acp-1f, 0x1.b42dep-5f},
-    {0x1.de5d6ep-1f, 0x1.165372p-4f}, {0x1.d77b66p-1f, 0x1.51b06ep-4f}, {0x1.d0cb58p-1f, 0x1.8c3466p-4f}, {0x1.ca4b3p-1f, 0x1.c5e54cp-4f},
-    {0x1.c3f8fp-1f, 0x1.fec914p-4f}, {0x1.bdd2b8p-1f, 0x1.1b72bp-3f}, {0x1.b7d6c4p-1f, 0x1.371fc2p-3f}, {0x1.b20364p-1f, 0x1.526e5ep-3f},
-    {0x1.ac5702p-1f, 0x1.6d60fcp-3f}, {0x1.a6d01ap-1f, 0x1.87fa08p-3f}, {0x1.a16d4p-1f, 0x1.a23bcp-3f}, {0x1.9c2d14p-1f, 0x1.bc286cp-3f},
-    {0x1.970e5p-1f, 0x1.d5c214p-3f}, {0x1.920fb4p-1f, 0x1.ef0aep-3f}, {0x1.8d3018p-1f, 0x1.04025cp-2f}, {0x1.886e6p-1f, 0x1.1058bep-2f},
-    {0x1.83c978p-1f, 0x1.1c898cp-2f}, {0x1.7f406p-1f, 0x1.2895ap-2f}, {0x1.7ad22p-1f, 0x1.347ddcp-2f}, {0x1.767dcep-1f, 0x1.40430ap-2f},
-    {0x1.724288p-1f, 0x1.4be5fap-2f}, {0x1.6e1f76p-1f, 0x1.576774p-2f}, {0x1.6a13cep-1f, 0x1.62c82cp-2f}, {0x1.661ec6p-1f, 0x1.6e08ecp-2f},
-    {0x1.623fa8p-1f, 0x1.792a54p-2f}, {0x1.5e75bcp-1f, 0x1.842d1cp-2f}, {0x1.5ac056p-1f, 0x1.8f11eap-2f}, {0x1.571ed4p-1f, 0x1.99d958p-2f},
+    {0x1.fc07fp+0f, -0x1.5ee82ap-1f}, {0x1.f4465ap+0f, -0x1.5707a2p-1f}, {0x1.ecc07cp+0f, -0x1.4f45aap-1f}, {0x1.e573acp+0f, -0x1.47a152p-1f},
+    {0x1.de5d6ep+0f, -0x1.4019c2p-1f}, {0x1.d77b66p+0f, -0x1.38ae22p-1f}, {0x1.d0cb58p+0f, -0x1.315da4p-1f}, {0x1.ca4b3p+0f, -0x1.2a2786p-1f},
+    {0x1.c3f8fp+0f, -0x1.230b0ep-1f}, {0x1.bdd2b8p+0f, -0x1.1c0784p-1f}, {0x1.b7d6c4p+0f, -0x1.151c4p-1f}, {0x1.b20364p+0f, -0x1.0e4898p-1f},
+    {0x1.ac5702p+0f, -0x1.078bfp-1f}, {0x1.a6d01ap+0f, -0x1.00e5aep-1f}, {0x1.a16d4p+0f, -0x1.f4aa8p-2f}, {0x1.9c2d14p+0f, -0x1.e7b42ap-2f},
+    {0x1.970e5p+0f, -0x1.dae756p-2f}, {0x1.920fb4p+0f, -0x1.ce42fp-2f}, {0x1.8d3018p+0f, -0x1.c1c604p-2f}, {0x1.886e6p+0f, -0x1.b56fa2p-2f},
+    {0x1.83c978p+0f, -0x1.a93ed4p-2f}, {0x1.7f406p+0f, -0x1.9d32cp-2f}, {0x1.7ad22p+0f, -0x1.914a84p-2f}, {0x1.767dcep+0f, -0x1.858556p-2f},
+    {0x1.724288p+0f, -0x1.79e266p-2f}, {0x1.6e1f76p+0f, -0x1.6e60ecp-2f}, {0x1.6a13cep+0f, -0x1.630034p-2f}, {0x1.661ec6p+0f, -0x1.57bf74p-2f},
+    {0x1.623fa8p+0f, -0x1.4c9e0cp-2f}, {0x1.5e75bcp+0f, -0x1.419b44p-2f}, {0x1.5ac056p+0f, -0x1.36b676p-2f}, {0x1.571ed4p+0f, -0x1.2bef08p-2f},
     {0x1.539094p+0f, -0x1.214456p-2f}, {0x1.501502p+0f, -0x1.16b5cep-2f}, {0x1.4cab88p+0f, -0x1.0c42d6p-2f}, {0x1.49539ep+0f, -0x1.01eae4p-2f},
     {0x1.460cbcp+0f, -0x1.ef5adcp-3f}, {0x1.42d662p+0f, -0x1.db13d8p-3f}, {0x1.3fb014p+0f, -0x1.c6ffbcp-3f}, {0x1.3c995ap+0f, -0x1.b31d84p-3f},
     {0x1.3991c2p+0f, -0x1.9f6c3cp-3f}, {0x1.3698ep+0f, -0x1.8beb04p-3f}, {0x1.33ae46p+0f, -0x1.7898dap-3f}, {0x1.30d19p+0f, -0x1.6574ecp-3f},
@@ -105,13 +104,14 @@ static __device__ __constant__ float2 c_logtab[64] = {
 };
 
 // log(x) for x > 0 normal: log x = e ln2 + lg_i + log1p(r), r = m rcp_i - 1 (one fma,
-// |r| < 2^-6; exact next to 1), log1p by its series to r^4 (the r^5 term is below 2e-10).
-// No division. tab: c_logtab or a copy in LDS.
+// |r| <= 2^-7; exact just below 1), log1p by its series to r^4 (the r^5 term is below
+// 1e-11). No division. Relative error below 2e-7 on (0, 1], the arguments it is used for
+// (logarithms of uniform variates); above 1 the result keeps an absolute error of a few
+// 1e-8 (ln 2 - ln 2 cancels next to 1). tab: c_logtab or a copy in LDS.
 GH_DEV float glog(float x, const float2 *tab) {
-  const uint32_t bits = __float_as_uint(x);
-  const uint32_t i = (bits >> 17) & 63u, half = i >> 5;
-  const float ef = (float)((int)(bits >> 23) - 127 + (int)half);
-  const float m = __uint_as_float(((bits & 0x007FFFFFu) | 0x3F800000u) - (half << 23));
+  const uint32_t i = (__float_as_uint(x) >> 17) & 63u;
+  const float m = __builtin_amdgcn_frexp_mantf(x);
+  const float ef = (float)__builtin_amdgcn_frexp_expf(x);
   const float2 t = tab[i];
   const float r = fmaf(m, t.x, -1.0f);
   float p = fmaf(r, -0.25f, 0x1.555556p-2f);

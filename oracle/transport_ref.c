@@ -89,23 +89,22 @@ void ref_stream_words(uint64_t seed, uint64_t history, uint32_t sim, uint32_t *o
 
 /* ------------------------------------------------------------------ math */
 static uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
-static float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
-/* log(x), x > 0 normal: x = 2^e m1, m1 in [1, 2), i = the top six mantissa bits; m = m1
- * (i < 32) or m1/2 with e + 1 (i >= 32), so m is in [0.75, 1.5). The table entry (rcp_i,
- * lg_i) -- rcp_i the float nearest 1/c_i, c_i the middle of entry i's interval (1 for the
- * two intervals next to 1), lg_i the float nearest -log(rcp_i); literals from
- * opengate_amd/csrc/gen_log_table.py -- gives log x = e ln2 + lg_i + log1p(r) with
- * r = m rcp_i - 1 and log1p(r) = r (1 - r/2 + r^2/3 - r^3/4). */
+/* log(x), x > 0 normal: x = 2^e m with m in [0.5, 1) (frexp), i = the top six mantissa
+ * bits. The table entry (rcp_i, lg_i) -- rcp_i the float nearest 1/c_i, c_i the middle of
+ * entry i's interval [1/2 + i/128, 1/2 + (i+1)/128) (1 for the last one, next to 1), lg_i
+ * the float nearest -log(rcp_i); literals from opengate_amd/csrc/gen_log_table.py -- gives
+ * log x = e ln2 + lg_i + log1p(r) with r = m rcp_i - 1 and
+ * log1p(r) = r (1 - r/2 + r^2/3 - r^3/4). Accurate to 2e-7 relative on (0, 1]. */
 static const float LOG_TAB[64][2] = {
-    {0x1p+0f, -0x0p+0f}, {0x1.f4465ap-1f, 0x1.7b91acp-6f}, {0x1.ecc07cp-1f, 0x1.39e86ep-5f}, {0x1.e573acp-1f, 0x1.b42dep-5f},
-    {0x1.de5d6ep-1f, 0x1.165372p-4f}, {0x1.d77b66p-1f, 0x1.51b06ep-4f}, {0x1.d0cb58p-1f, 0x1.8c3466p-4f}, {0x1.ca4b3p-1f, 0x1.c5e54cp-4f},
-    {0x1.c3f8fp-1f, 0x1.fec914p-4f}, {0x1.bdd2b8p-1f, 0x1.1b72bp-3f}, {0x1.b7d6c4p-1f, 0x1.371fc2p-3f}, {0x1.b20364p-1f, 0x1.526e5ep-3f},
-    {0x1.ac5702p-1f, 0x1.6d60fcp-3f}, {0x1.a6d01ap-1f, 0x1.87fa08p-3f}, {0x1.a16d4p-1f, 0x1.a23bcp-3f}, {0x1.9c2d14p-1f, 0x1.bc286cp-3f},
-    {0x1.970e5p-1f, 0x1.d5c214p-3f}, {0x1.920fb4p-1f, 0x1.ef0aep-3f}, {0x1.8d3018p-1f, 0x1.04025cp-2f}, {0x1.886e6p-1f, 0x1.1058bep-2f},
-    {0x1.83c978p-1f, 0x1.1c898cp-2f}, {0x1.7f406p-1f, 0x1.2895ap-2f}, {0x1.7ad22p-1f, 0x1.347ddcp-2f}, {0x1.767dcep-1f, 0x1.40430ap-2f},
-    {0x1.724288p-1f, 0x1.4be5fap-2f}, {0x1.6e1f76p-1f, 0x1.576774p-2f}, {0x1.6a13cep-1f, 0x1.62c82cp-2f}, {0x1.661ec6p-1f, 0x1.6e08ecp-2f},
-    {0x1.623fa8p-1f, 0x1.792a54p-2f}, {0x1.5e75bcp-1f, 0x1.842d1cp-2f}, {0x1.5ac056p-1f, 0x1.8f11eap-2f}, {0x1.571ed4p-1f, 0x1.99d958p-2f},
+    {0x1.fc07fp+0f, -0x1.5ee82ap-1f}, {0x1.f4465ap+0f, -0x1.5707a2p-1f}, {0x1.ecc07cp+0f, -0x1.4f45aap-1f}, {0x1.e573acp+0f, -0x1.47a152p-1f},
+    {0x1.de5d6ep+0f, -0x1.4019c2p-1f}, {0x1.d77b66p+0f, -0x1.38ae22p-1f}, {0x1.d0cb58p+0f, -0x1.315da4p-1f}, {0x1.ca4b3p+0f, -0x1.2a2786p-1f},
+    {0x1.c3f8fp+0f, -0x1.230b0ep-1f}, {0x1.bdd2b8p+0f, -0x1.1c0784p-1f}, {0x1.b7d6c4p+0f, -0x1.151c4p-1f}, {0x1.b20364p+0f, -0x1.0e4898p-1f},
+    {0x1.ac5702p+0f, -0x1.078bfp-1f}, {0x1.a6d01ap+0f, -0x1.00e5aep-1f}, {0x1.a16d4p+0f, -0x1.f4aa8p-2f}, {0x1.9c2d14p+0f, -0x1.e7b42ap-2f},
+    {0x1.970e5p+0f, -0x1.dae756p-2f}, {0x1.920fb4p+0f, -0x1.ce42fp-2f}, {0x1.8d3018p+0f, -0x1.c1c604p-2f}, {0x1.886e6p+0f, -0x1.b56fa2p-2f},
+    {0x1.83c978p+0f, -0x1.a93ed4p-2f}, {0x1.7f406p+0f, -0x1.9d32cp-2f}, {0x1.7ad22p+0f, -0x1.914a84p-2f}, {0x1.767dcep+0f, -0x1.858556p-2f},
+    {0x1.724288p+0f, -0x1.79e266p-2f}, {0x1.6e1f76p+0f, -0x1.6e60ecp-2f}, {0x1.6a13cep+0f, -0x1.630034p-2f}, {0x1.661ec6p+0f, -0x1.57bf74p-2f},
+    {0x1.623fa8p+0f, -0x1.4c9e0cp-2f}, {0x1.5e75bcp+0f, -0x1.419b44p-2f}, {0x1.5ac056p+0f, -0x1.36b676p-2f}, {0x1.571ed4p+0f, -0x1.2bef08p-2f},
     {0x1.539094p+0f, -0x1.214456p-2f}, {0x1.501502p+0f, -0x1.16b5cep-2f}, {0x1.4cab88p+0f, -0x1.0c42d6p-2f}, {0x1.49539ep+0f, -0x1.01eae4p-2f},
     {0x1.460cbcp+0f, -0x1.ef5adcp-3f}, {0x1.42d662p+0f, -0x1.db13d8p-3f}, {0x1.3fb014p+0f, -0x1.c6ffbcp-3f}, {0x1.3c995ap+0f, -0x1.b31d84p-3f},
     {0x1.3991c2p+0f, -0x1.9f6c3cp-3f}, {0x1.3698ep+0f, -0x1.8beb04p-3f}, {0x1.33ae46p+0f, -0x1.7898dap-3f}, {0x1.30d19p+0f, -0x1.6574ecp-3f},
@@ -117,10 +116,10 @@ static const float LOG_TAB[64][2] = {
 };
 
 float ref_log(float x) {
-  const uint32_t b = fbits(x);
-  const uint32_t i = (b >> 17) & 63u, half = i >> 5;
-  const float ef = (float)((int)(b >> 23) - 127 + (int)half);
-  const float m = bitsf(((b & 0x007FFFFFu) | 0x3F800000u) - (half << 23));
+  const uint32_t i = (fbits(x) >> 17) & 63u;
+  int e;
+  const float m = frexpf(x, &e);
+  const float ef = (float)e;
   const float *t = LOG_TAB[i];
   const float r = fmaf(m, t[0], -1.0f);
   float p = fmaf(r, -0.25f, 0x1.555556p-2f);
