@@ -139,6 +139,14 @@ struct Hot {
   __device__ __forceinline__ int label(int vox) const { return packed_label(labels, lb, vox); }
 };
 
+// (int)floorf(x) in one instruction (floor and conversion; out-of-range values clamp to the
+// int range as the conversion alone does, and any of them lies outside a phantom)
+__device__ __forceinline__ int floor_int(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // reciprocal direction: the slab tests of a photon reuse one set of divisions
 struct InvDir {
   float x, y, z;
@@ -911,8 +919,7 @@ transport_kernel(Launch L) {
           if (mode == M_STEP) {
             const float d = -glog(u01w(k == 0 ? w0 : w2), s_logtab) * ec.inv_maj;
             const float nx = fmaf(d, ph.u, ph.x), ny = fmaf(d, ph.v, ph.y), nz = fmaf(d, ph.w, ph.z);
-            const int ix = (int)floorf(nx * hot.isx), iy = (int)floorf(ny * hot.isy),
-                      iz = (int)floorf(nz * hot.isz);
+            const int ix = floor_int(nx * hot.isx), iy = floor_int(ny * hot.isy), iz = floor_int(nz * hot.isz);
             if ((unsigned)ix < (unsigned)hot.nx && (unsigned)iy < (unsigned)hot.ny &&
                 (unsigned)iz < (unsigned)hot.nz) {
               ph.x = nx; ph.y = ny; ph.z = nz;
