@@ -591,24 +591,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) m
   }
 }
 
-// Fused MLP inference for hidden width 256: 8 waves (512 threads) own 224 rows, in two row
-// groups of 112 (rg = wave / 4) and four column groups of 64 (cg = wave % 4), each wave
-// the same 7 x 4 tiles of 16 x 16 as the 4-wave form. The weights of each k step of 32
-// (all 256 columns, 16 KB) are staged in LDS once for the 224 rows, half the L2 traffic
-// per row of the 4-wave form, by direct global-to-LDS loads (global_load_lds_dwordx4, no
-// registers in flight), double-buffered. Per k step: wait for this thread's loads of the
-// next step, a barrier (every thread's loads visible, every wave's reads of the stage to
-// be refilled done), the loads of the step after next, then the LDS reads of the next
-// step's fragments while the MFMAs of this step run on fragments read one step earlier.
-// The activation tile (224 x 256, pitch 264) is overwritten in place by each layer's
-// epilogue. Same MFMA sequence per output element as the 4-wave form and the layer-by-layer
-// GEMMs: bit-identical results.
-constexpr int W8_RT = 7;                                   // row tiles per wave
-constexpr int W8_ROWS = 2 * 16 * W8_RT;                    // 224 rows per workgroup
-constexpr int W8_P = 256 + 8;                              // activation row pitch (bf16)
-constexpr size_t W8_ACT = (size_t)W8_ROWS * W8_P * 2;      // 118 272 bytes
+// Fused MLP inference for hidden width 256: 8 waves (512 threads) own 256 rows, in two row
+// groups of 128 (rg = wave / 4) and four column groups of 64 (cg = wave % 4), each wave
+// 8 x 4 tiles of 16 x 16. The weights of each k step of 32 (all 256 columns, 16 KB) are
+// staged in LDS once for the 256 rows (a quarter of the L2 traffic per row of the 4-wave
+// form) by direct global-to-LDS loads (global_load_lds_dwordx4, no registers in flight),
+// double-buffered. Per k step: wait for this thread's loads of the next step, a barrier
+// (every thread's loads visible, every wave's reads of the stage to be refilled done), the
+// loads of the step after next, then the LDS reads of the next step's fragments while the
+// MFMAs of this step run on fragments read one step earlier. The activation tile
+// (256 x 256 bf16, no padding: 16-byte unit u of row r is stored at unit u ^ (r mod 16), so
+// that the 16 rows of a fragment read fall in different banks) is overwritten in place by
+// each layer's epilogue; with the two stages it fills the 160 KiB of LDS of a CU. Same MFMA
+// sequence per output element as the 4-wave form and the layer-by-layer GEMMs:
+// bit-identical results.
+constexpr int W8_RT = 8;                                   // row tiles per wave
+constexpr int W8_ROWS = 2 * 16 * W8_RT;                    // 256 rows per workgroup
+constexpr int W8_P = 256;                                  // activation row pitch (bf16)
+constexpr size_t W8_ACT = (size_t)W8_ROWS * W8_P * 2;      // 131 072 bytes
 constexpr size_t W8_STAGE = 256 * 32 * 2;                  // one k step of weights: 16 KB
-constexpr size_t W8_LDS = W8_ACT + 2 * W8_STAGE;           // 151 040 bytes
+constexpr size_t W8_LDS = W8_ACT + 2 * W8_STAGE;           // 163 840 bytes
+
+// element (r, k) of the swizzled activation tile
+__device__ __forceinline__ int w8_at(int r, int k) { return r * W8_P + ((((k >> 3) ^ (r & 15)) << 3) | (k & 7)); }
 
 // k step ks of a 256-column weight matrix Wt[n][k] (row length K) into a stage: 16-byte
 // slot s holds k group s / 256 (8 values) of column s % 256, so that the 16 lanes of an
@@ -629,7 +634,7 @@ __device__ __forceinline__ void w8_stage_load(unsigned char *stage, const __bf16
 
 __global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__ X, int64_t ldx, int64_t M,
                                                       MlpArgs a, float *__restrict__ Z, int64_t ldz) {
-  constexpr int H = 256, RT = W8_RT, CT = 4, P = W8_P, ROWS = W8_ROWS;
+  constexpr int H = 256, RT = W8_RT, CT = 4, ROWS = W8_ROWS;
   extern __shared__ __align__(16) unsigned char mlp_lds[];
   __bf16 *act = (__bf16 *)mlp_lds;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -666,13 +671,13 @@ __global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__
         for (int j = 0; j < 8; ++j)
           if (q + j >= a.k_in) v[j] = (__bf16)0.0f;
       }
-      *(bf16x8 *)(act + r * P + q) = v;
+      *(bf16x8 *)(act + w8_at(r, q)) = v;
     }
   } else {
     for (int i = tid; i < ROWS * K0; i += 512) {
       const int r = i / K0, k = i - r * K0;
       const int64_t gr = row0 + r;
-      act[r * P + k] = (gr < M && k < a.k_in) ? X[gr * ldx + k] : (__bf16)0.0f;
+      act[w8_at(r, k)] = (gr < M && k < a.k_in) ? X[gr * ldx + k] : (__bf16)0.0f;
     }
   }
   const int arow0 = rg * 16 * RT + fr;
@@ -685,7 +690,7 @@ __global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__
       for (int j = 0; j < CT; ++j) acc[i][j] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     auto loadA = [&](bf16x8 *af, int ks) {
 #pragma unroll
-      for (int i = 0; i < RT; ++i) af[i] = *(const bf16x8 *)(act + (arow0 + i * 16) * P + ks * 32 + fk);
+      for (int i = 0; i < RT; ++i) af[i] = *(const bf16x8 *)(act + w8_at(arow0 + i * 16, ks * 32 + fk));
     };
     auto loadB = [&](bf16x8 *b, int g) {
       const unsigned char *st = stage(g);
@@ -735,12 +740,12 @@ __global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__
         bf16x4 v;
 #pragma unroll
         for (int r = 0; r < 4; ++r) v[r] = f2bf(relu(acc[i][j][r] + bb[r]));
-        *(bf16x4 *)(act + (arow0 + i * 16) * P + c) = v;
+        *(bf16x4 *)(act + w8_at(arow0 + i * 16, c)) = v;
       }
     }
   }
   __syncthreads();
-  // output layer (NOUT = 16 columns): the 14 row tiles over the 8 waves
+  // output layer (NOUT = 16 columns): the 16 row tiles over the 8 waves
   const int l = a.n_layers - 1;
   const __bf16 *__restrict__ W = a.wt[l];
   const float *__restrict__ bias = a.bias[l];
@@ -757,7 +762,7 @@ __global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__
     for (int t = 0; t < NTW; ++t) {
       const int i = wave + 8 * t;
       if (i < NT) {
-        const bf16x8 af = *(const bf16x8 *)(act + (i * 16 + fr) * P + k * 32 + fk);
+        const bf16x8 af = *(const bf16x8 *)(act + w8_at(i * 16 + fr, k * 32 + fk));
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, wout[k], acc[t], 0, 0, 0);
       }
     }

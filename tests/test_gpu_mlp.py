@@ -165,7 +165,7 @@ def test_fused_adam_step_against_fp64():
 
 @pytest.mark.parametrize("sizes,M", [([8, 256, 256, 256, 256, 256, 6], 1000), ([3, 128, 128, 128, 4], 4099),
                                      ([8, 256, 256, 6], 64), ([5, 128, 128, 16], 130),
-                                     ([8, 256, 256, 256, 6], 224 * 3 + 17), ([8, 256, 6], 300), ([8, 256, 256, 6], 1)])
+                                     ([8, 256, 256, 256, 6], 256 * 3 + 17), ([8, 256, 6], 300), ([8, 256, 256, 6], 1)])
 def test_fused_forward_equals_layered(sizes, M):
     """gh_mlp_forward_bf16 (all layers in one kernel) gives the layer-by-layer GEMM values
     bit for bit, and both agree with the FP64 model of the bf16 network."""
