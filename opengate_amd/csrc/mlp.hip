@@ -642,22 +642,11 @@ __global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__
   const int64_t row0 = (int64_t)blockIdx.x * ROWS;
   const int fr = lane & 15, fk = (lane >> 4) * 8;
   // k steps numbered over the layers (step 0: the input layer's single step, then 8 per
-  // hidden layer); step g uses stage g % 2
-  int gs = 0;
+  // hidden layer) alternate between the two stages: the input layer's step uses stage 0,
+  // step ks of a hidden layer stage (ks + 1) % 2. Every stage index and weight offset below
+  // is a compile-time constant; only the layer's weight pointers are read per layer.
   auto stage = [&](int g) { return mlp_lds + W8_ACT + (size_t)(g & 1) * W8_STAGE; };
-  const int nsteps = 1 + (a.n_layers - 2) * (H / 32);
-  int issued = 0;  // steps below `issued` have their weight loads issued
-  auto issue_to = [&](int g) {
-    for (; issued <= g && issued < nsteps; ++issued) {
-      if (issued == 0) {
-        w8_stage_load(stage(0), a.wt[0], K0, 0, tid);
-      } else {
-        const int l = 1 + (issued - 1) / (H / 32), ks = (issued - 1) % (H / 32);
-        w8_stage_load(stage(issued), a.wt[l], H, ks, tid);
-      }
-    }
-  };
-  issue_to(0);  // the input layer's weights load while the input tile does
+  w8_stage_load(stage(0), a.wt[0], K0, 0, tid);  // the input layer's weights load while the input tile does
   if (a.x_vec) {
     for (int i = tid; i < ROWS * (K0 / 8); i += 512) {
       const int r = i / (K0 / 8), q = (i - r * (K0 / 8)) * 8;
@@ -697,20 +686,35 @@ __global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__
 #pragma unroll
       for (int j = 0; j < CT; ++j) b[j] = *(const bf16x8 *)(st + (size_t)((lane >> 4) * 256 + cg * 64 + j * 16 + fr) * 16);
     };
+    // this layer's weights and, when the next layer is a hidden one, the next layer's (its
+    // first step is staged during this layer's second-to-last step)
+    const __bf16 *Wl = a.wt[l];
+    const __bf16 *Wn = (l + 2 < a.n_layers) ? a.wt[l + 1] : nullptr;
     auto steps = [&](auto nk_c) {
-      constexpr int NK = decltype(nk_c)::value;
+      constexpr int NK = decltype(nk_c)::value;  // 1: the input layer, 8: a hidden layer
+      auto stg = [](int ks) { return NK == 1 ? 0 : ((ks + 1) & 1); };
       bf16x8 aA[RT], aB[RT], bA[CT], bB[CT];
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();  // the layer input is complete, and so are the weights of its first step
-      issue_to(gs + 1);
-      loadB(bA, gs);
+      if (NK == 1) {
+        if (Wn) w8_stage_load(stage(1), Wn, H, 0, tid);
+      } else {
+        w8_stage_load(stage(stg(1)), Wl, H, 1, tid);
+      }
+      loadB(bA, stg(0));
       loadA(aA, 0);
-      auto step = [&](int ks, const bf16x8 *aCur, const bf16x8 *bCur, bf16x8 *aNext, bf16x8 *bNext) {
+      auto step = [&](int ks, const bf16x8 *aCur, bf16x8 *bCur, bf16x8 *aNext, bf16x8 *bNext) {
         if (ks + 1 < NK) {
           asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
           __syncthreads();
-          issue_to(gs + 2);
-          loadB(bNext, gs + 1);
+          // the MFMAs of this step read registers only; tie them to the barrier, or the
+          // compiler hoists them into the step before, next to the reads that feed them
+#pragma unroll
+          for (int j = 0; j < CT; ++j) asm volatile("" : "+v"(bCur[j]) :: "memory");
+          // refill the stage that step ks used (its fragments were read one step earlier)
+          if (ks + 2 < NK) w8_stage_load(stage(stg(ks + 2)), Wl, H, ks + 2, tid);
+          else if (Wn) w8_stage_load(stage(1), Wn, H, 0, tid);
+          loadB(bNext, stg(ks + 1));
           loadA(aNext, ks + 1);
         }
 #pragma unroll
@@ -718,7 +722,6 @@ __global__ void __launch_bounds__(512) mlp_forward_w8(const __bf16 *__restrict__
 #pragma unroll
           for (int j = 0; j < CT; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bCur[j], aCur[i], acc[i][j], 0, 0, 0);
-        ++gs;
       };
 #pragma unroll
       for (int ks = 0; ks < NK; ks += 2) {
