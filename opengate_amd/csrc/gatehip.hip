@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "gh_device.h"
@@ -542,8 +543,20 @@ enum { M_DEAD = 0, M_STEP = 1, M_COLLIDE = 2, M_EXIT = 3, M_WORLD = 4, M_COMPTON
 // global history id of the lane's current history (event-stream counter words)
 __device__ __forceinline__ uint64_t history_of(const Rng &g) { return ((uint64_t)g.c1 << 32) | g.c0; }
 
-// u01 of a raw Philox word (odd multiples of 2^-24)
-__device__ __forceinline__ float u01w(uint32_t w) { return (float)((w >> 8) | 1u) * 0x1p-24f; }
+// the 21-bit fields of one step-stream Philox block (words w0..w3) for tentative step k of
+// three: step 0 takes the low 21 bits of w0 (distance) and w1 (accept), step 1 those of w2
+// and w3, step 2 the top 11 bits of w0 | w2 (distance) and of w1 | w3 (accept), the w2 / w3
+// part above (its top bit dropped)
+template <int K>
+__device__ __forceinline__ void step_fields(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t &bd,
+                                            uint32_t &ba) {
+  constexpr uint32_t M21 = 0x1FFFFFu;
+  if (K == 0) { bd = w0 & M21; ba = w1 & M21; }
+  else if (K == 1) { bd = w2 & M21; ba = w3 & M21; }
+  else { bd = ((w0 >> 21) | (w2 >> 21 << 11)) & M21; ba = ((w1 >> 21) | (w3 >> 21 << 11)) & M21; }
+}
+// odd multiple of 2^-22 in (0, 1) from a 21-bit field, as an exact integer
+__device__ __forceinline__ float odd22(uint32_t b) { return (float)((b << 1) | 1u); }
 
 // Persistent transport kernel. The loop of a wave has three phases, each entered by
 // all 64 lanes together (the conditions are ballots, so they are wave-uniform):
@@ -554,7 +567,7 @@ __device__ __forceinline__ float u01w(uint32_t w) { return (float)((w >> 8) | 1u
 //           phantom is carried to it through the world; photons that stop at birth (below
 //           the tracking cut, or missing the phantom) are finished in the pass and never
 //           take a lane. So the source sampling runs on full waves however few lanes wait.
-//   step    every lane in M_STEP takes up to two Woodcock steps per iteration from one
+//   step    every lane in M_STEP takes up to three Woodcock steps per iteration from one
 //           Philox block of its step stream; a real collision (M_COLLIDE) or a box exit
 //           (M_EXIT) parks the lane; the phase ends when slow_min lanes are parked;
 //   slow    parked lanes process their collision, exit or world event together, using
@@ -712,7 +725,7 @@ transport_kernel(Launch L) {
   ph.mode = MODE_BOX; ph.sim = 0; ph.hist = 0;
   ph.x = ph.y = ph.z = ph.u = ph.v = ph.w = ph.E = ph.wt = 0.0f;
   ECache ec;
-  ec.b = 0; ec.frac = 0.0f; ec.maj24 = 0x1p-24f; ec.inv_maj = 1.0f;
+  ec.b = 0; ec.frac = 0.0f; ec.maj22 = 0x1p-22f; ec.inv_maj = 1.0f;
   Rng rng;
   // the key is the run's seed in every lane from the start, so the compiler sees one
   // wave-uniform key (scalar registers) instead of a per-lane value
@@ -914,10 +927,11 @@ transport_kernel(Launch L) {
         uint32_t w0 = rng.c0, w1 = rng.c1, w2 = rng.c2, w3 = 0x80000000u | sblock;
         philox10(w0, w1, w2, w3, rng.k0, rng.k1);
         sblock++;
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
+        auto woodcock = [&](auto k_c) {
           if (mode == M_STEP) {
-            const float d = -glog(u01w(k == 0 ? w0 : w2), s_logtab) * ec.inv_maj;
+            uint32_t bd, ba;
+            step_fields<decltype(k_c)::value>(w0, w1, w2, w3, bd, ba);
+            const float d = -glog(odd22(bd) * 0x1p-22f, s_logtab) * ec.inv_maj;
             const float nx = fmaf(d, ph.u, ph.x), ny = fmaf(d, ph.v, ph.y), nz = fmaf(d, ph.w, ph.z);
             const int ix = floor_int(nx * hot.isx), iy = floor_int(ny * hot.isy), iz = floor_int(nz * hot.isz);
             if ((unsigned)ix < (unsigned)hot.nx && (unsigned)iy < (unsigned)hot.ny &&
@@ -941,9 +955,9 @@ transport_kernel(Launch L) {
                 const float x = (ph.E * ph.wt) * (lerp_node_scaled(P.muen + mr.row * NB1, mr.rho, ec) * ec.inv_maj);
                 deposit(s0, sq, tally_voxel(s0, ph.x, ph.y, ph.z), x, x * P.inv_rho[lab], st);
               }
-              // u01 x majorant < mu, with u01 = ((w >> 8) | 1) 2^-24 and the 2^-24 folded into
-              // maj24: the same single rounding of the same exact product
-              if ((float)(((k == 0 ? w1 : w3) >> 8) | 1u) * ec.maj24 < mu) {
+              // u01 x majorant < mu, with u01 = odd22(ba) 2^-22 and the 2^-22 folded into
+              // maj22: the same single rounding of the same exact product
+              if (odd22(ba) * ec.maj22 < mu) {
                 mode = M_COLLIDE;
                 cvox = vox;
                 clab = lab;
@@ -952,7 +966,10 @@ transport_kernel(Launch L) {
               mode = M_EXIT;
             }
           }
-        }
+        };
+        woodcock(std::integral_constant<int, 0>{});
+        woodcock(std::integral_constant<int, 1>{});
+        woodcock(std::integral_constant<int, 2>{});
       }
     }
 

@@ -261,7 +261,7 @@ GH_DEV int cell_index(float p, float c, float s, float inv) {
 struct ECache {
   int b;         // bucket
   float frac;    // position inside the bucket
-  float maj24;   // majorant at E times 2^-24 (the accept test's uniform without its scaling)
+  float maj22;   // majorant at E times 2^-22 (the accept test's uniform without its scaling)
   float inv_maj;
 };
 
@@ -271,7 +271,7 @@ GH_DEV void ecache_set(ECache &c, float E, const float *maj) {
   c.frac = (float)(bits & EMASK) * 0x1p-17f;
   const float m0 = maj[c.b], m1 = maj[c.b + 1];
   const float m = fmaf(c.frac, m1 - m0, m0);
-  c.maj24 = m * 0x1p-24f;
+  c.maj22 = m * 0x1p-22f;
   c.inv_maj = 1.0f / m;
 }
 

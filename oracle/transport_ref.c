@@ -52,6 +52,21 @@ void ref_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
   out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
 }
 
+/* 21-bit fields of one step-stream block w[0..3] for tentative step k of three: step 0
+ * takes the low 21 bits of w0 (distance) and w1 (accept), step 1 those of w2 and w3,
+ * step 2 the top 11 bits of w0 | w2 (distance) and of w1 | w3 (accept), the w2 / w3 part
+ * above with its top bit dropped. The uniform is then (2 b + 1) 2^-22. */
+static void step_fields(const uint32_t w[4], int k, uint32_t *bd, uint32_t *ba) {
+  const uint32_t m21 = 0x1FFFFFu;
+  if (k < 2) {
+    *bd = w[2 * k] & m21;
+    *ba = w[2 * k + 1] & m21;
+  } else {
+    *bd = ((w[0] >> 21) | (w[2] >> 21 << 11)) & m21;
+    *ba = ((w[1] >> 21) | (w[3] >> 21 << 11)) & m21;
+  }
+}
+
 typedef struct {
   uint32_t key[2];
   uint32_t ctr[4];
@@ -792,8 +807,9 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
   energy_index(P, ph.E, &ex);
   const int wm = S->ph->world_material;
   /* Woodcock steps draw from a separate "step stream" of the history: one Philox
-   * block (counter word 3 = 0x80000000 | step block) serves two tentative steps;
-   * its last two words are unused when the first step collides or leaves the box.
+   * block (counter word 3 = 0x80000000 | step block) serves three tentative steps
+   * through 21-bit fields (step_fields); what is left of a block when a step collides
+   * or leaves the box is unused.
    * Everything else (source, interactions, world) uses the event stream rs. */
   const uint32_t hid = (uint32_t)(first + h), hid_hi = (uint32_t)((first + h) >> 32);
   uint32_t step_block = 0;
@@ -805,8 +821,10 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
       step_block++;
       int outcome = 0; /* 0: both steps virtual, 1: real collision, 2: leaves the box */
       int vox = -1, lab = 0;
-      for (int k = 0; k < 2; ++k) {
-        float step = -ref_log((float)((w[2 * k] >> 8) | 1u) * 0x1p-24f) * ex.inv_maj;
+      for (int k = 0; k < 3; ++k) {
+        uint32_t bd, ba;
+        step_fields(w, k, &bd, &ba);
+        float step = -ref_log((float)((bd << 1) | 1u) * 0x1p-22f) * ex.inv_maj;
         float q[3];
         for (int i = 0; i < 3; ++i) q[i] = fmaf(step, ph.d[i], ph.p[i]);
         vox = voxel_of(S, q);
@@ -819,7 +837,7 @@ static void run_history(const sim_t *S, const phys_t *P, uint64_t seed, uint64_t
           const float x = (ph.E * ph.w) * (material_node_interp(P, d->mu_en, lab, &ex) * ex.inv_maj);
           score(t, &sl, tally_index(S, ph.p), x, x * P->inv_rho[lab], st);
         }
-        if ((float)((w[2 * k + 1] >> 8) | 1u) * 0x1p-24f * ex.maj < mu) { outcome = 1; break; }
+        if ((float)((ba << 1) | 1u) * 0x1p-22f * ex.maj < mu) { outcome = 1; break; }
       }
       if (outcome == 0) continue;
       if (outcome == 1) {
