@@ -108,8 +108,8 @@ def main():
     ap.add_argument("--ct256-steps", type=int, default=5, help="headline: launches of the ct256 plain leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--refill", type=int, default=64)
-    ap.add_argument("--slow-min", type=int, default=52)
-    ap.add_argument("--refill-min", type=int, default=16)
+    ap.add_argument("--slow-min", type=int, default=48)
+    ap.add_argument("--refill-min", type=int, default=12)
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--block", type=int, default=0,
                     help="ct256: workgroup size (0: automatic staging, all tables in LDS for one simulation)")

@@ -421,7 +421,7 @@ class GpuEngine:
         self._check(self.lib.gh_reset_tally(self.ctx, int(sim_id)), "gh_reset_tally")
 
     def run(self, n_histories, seed, first_history=None, block=0, grid=0, refill_interval=64, chunk=0,
-            slow_min=52, refill_min=12, scatter_tries=0):
+            slow_min=48, refill_min=12, scatter_tries=0):
         n = np.ascontiguousarray(np.atleast_1d(n_histories), dtype=np.uint64)
         if len(n) != len(self.sims):
             raise GhError("one history count per simulation is required")
