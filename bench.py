@@ -122,7 +122,7 @@ def main():
     ap.add_argument("--spect-histories", type=float, default=1e8, help="spect16: histories per phantom")
     ap.add_argument("--ep-train-photons", type=float, default=1e8,
                     help="exitplane: recorded collimator photons used to train the generator")
-    ap.add_argument("--ep-fit-steps", type=int, default=2000, help="exitplane: generator training steps")
+    ap.add_argument("--ep-fit-steps", type=int, default=4000, help="exitplane: generator training steps")
     ap.add_argument("--ep-chunk", type=float, default=1e8, help="exitplane: photons per step (transport chunk)")
     ap.add_argument("--autotune", action="store_true", help="pick launch parameters with gh_autotune first")
     ap.add_argument("--copy-calibration", action="store_true",
